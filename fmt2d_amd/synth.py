@@ -1,0 +1,195 @@
+"""Deterministic synthetic marine models of the FMT2D model family.
+
+The files written here follow the reference's text formats
+(``src/read_tomo2d.f90``): ``.smesh`` (``:8-79``), interface ``x z`` lines
+(``:83-137``) and ``geom`` (``:141-248``), so the same inputs can be fed to the
+reference executables and to this package.
+
+Model family (all depths in km below sea level, velocities in km/s):
+
+* water 1.5 km/s above a seafloor ``3 + 1*sin(2*pi*x/80)``;
+* sediments 1.8 -> 3.0 km/s, linear in depth below the seafloor, thickness
+  ``1.5 + 0.5*sin(2*pi*x/60 + 0.7)`` (1-2 km);
+* crust 4.5 -> 7.0 km/s, linear from the sediment base to the Moho;
+* Moho ``15 + 2*sin(2*pi*x/70)``; mantle 8.0 km/s;
+* optional checkerboard ``v *= 1 + a*sign(sin(pi*x/Lx)*sin(pi*z/Lz))`` below
+  the seafloor;
+* optional seeded random Fourier perturbation of the sediment base and Moho.
+
+The ``.smesh`` depth axis is relative to the seafloor with the same spacing
+as x; its last node is placed so that the reference's propagation grid
+(``zmax = zrel(nz) + max(topo) + 5``, ``src/fmt2d_main.f90:200-205``) ends at
+``z_bottom``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+VWATER = 1.5
+VAIR = 0.33
+
+
+@dataclasses.dataclass
+class Model:
+    xpos: np.ndarray          # (nx,)
+    topo: np.ndarray          # (nx,) seafloor depth
+    zrel: np.ndarray          # (nzr,) depth below seafloor of .smesh rows
+    vel: np.ndarray           # (nx, nzr) smesh velocities (ix-major like the file)
+    sed_z: np.ndarray         # (nx,) sediment-base depth
+    moho_z: np.ndarray        # (nx,) Moho depth
+    vwater: float = VWATER
+    vair: float = VAIR
+
+
+@dataclasses.dataclass
+class Geometry:
+    obs_x: np.ndarray                     # (nobs,)
+    obs_z: np.ndarray                     # (nobs,)
+    # per OBS: list of (x, z, ray_type, t_obs, unc)
+    receivers: List[List[Tuple[float, float, int, float, float]]]
+
+    @property
+    def nrec(self) -> int:
+        return sum(len(r) for r in self.receivers)
+
+
+def seafloor(x):
+    return 3.0 + 1.0 * np.sin(2.0 * np.pi * x / 80.0)
+
+
+def sediment_thickness(x):
+    return 1.5 + 0.5 * np.sin(2.0 * np.pi * x / 60.0 + 0.7)
+
+
+def moho(x, wavelength=70.0):
+    return 15.0 + 2.0 * np.sin(2.0 * np.pi * x / wavelength)
+
+
+def fourier_perturbation(x, rng, nmodes=6, amp_total=1.5, lmin=10.0, lmax=100.0):
+    """Sum of ``nmodes`` sines with wavelengths U[lmin,lmax] and sum|a_k|<=amp_total."""
+    lam = rng.uniform(lmin, lmax, nmodes)
+    phi = rng.uniform(0.0, 2.0 * np.pi, nmodes)
+    a = rng.uniform(-1.0, 1.0, nmodes)
+    a *= amp_total / max(np.sum(np.abs(a)), 1e-12)
+    out = np.zeros_like(x)
+    for k in range(nmodes):
+        out += a[k] * np.sin(2.0 * np.pi * x / lam[k] + phi[k])
+    return out
+
+
+def make_model(dx: float = 0.5, xmax: float = 200.0, z_bottom: float = 30.0,
+               checker_amp: float = 0.05, checker_lx: float = 20.0,
+               checker_lz: float = 5.0, fourier_seed: Optional[int] = None) -> Model:
+    nx = int(round(xmax / dx)) + 1
+    xpos = np.round(np.arange(nx) * dx, 9)
+    topo = seafloor(xpos)
+    sed_z = topo + sediment_thickness(xpos)
+    moho_z = moho(xpos)
+    if fourier_seed is not None:
+        rng = np.random.default_rng(fourier_seed)
+        sed_z = sed_z + fourier_perturbation(xpos, rng, amp_total=0.5)
+        moho_z = moho_z + fourier_perturbation(xpos, rng, amp_total=1.5)
+        sed_z = np.maximum(sed_z, topo + 0.3)
+        moho_z = np.maximum(moho_z, sed_z + 2.0)
+    zrel_max = z_bottom - 5.0 - topo.max()
+    nzr = int(np.floor(zrel_max / dx + 1e-9)) + 1
+    zrel = np.round(np.arange(nzr) * dx, 9)
+    zrel[-1] = zrel_max
+    vel = np.zeros((nx, nzr))
+    for ix in range(nx):
+        z = topo[ix] + zrel
+        hs = sed_z[ix] - topo[ix]
+        v = np.where(zrel < hs, 1.8 + 1.2 * zrel / hs,
+                     np.where(z < moho_z[ix],
+                              4.5 + 2.5 * (z - sed_z[ix]) / (moho_z[ix] - sed_z[ix]),
+                              8.0))
+        if checker_amp:
+            sgn = np.sign(np.sin(np.pi * xpos[ix] / checker_lx) * np.sin(np.pi * z / checker_lz))
+            v = np.where(zrel > 0.0, v * (1.0 + checker_amp * sgn), v)
+        vel[ix] = v
+    return Model(xpos=xpos, topo=topo, zrel=zrel, vel=vel, sed_z=sed_z, moho_z=moho_z)
+
+
+def make_geometry(obs_x: Sequence[float], shot_x: Sequence[float], model: Model,
+                  shot_z: float = 0.010, ray_types: Sequence[int] = (1,),
+                  max_offset: Optional[float] = None) -> Geometry:
+    obs_x = np.asarray(obs_x, dtype=float)
+    obs_z = np.interp(obs_x, model.xpos, model.topo)
+    recs = []
+    for ox in obs_x:
+        lst = []
+        for rt in ray_types:
+            for sx in shot_x:
+                if max_offset is not None and abs(sx - ox) > max_offset:
+                    continue
+                lst.append((float(sx), float(shot_z), int(rt), 0.0, 0.05))
+        recs.append(lst)
+    return Geometry(obs_x=obs_x, obs_z=obs_z, receivers=recs)
+
+
+# ---------------------------------------------------------------- writers
+
+def write_smesh(path: str, m: Model) -> None:
+    nx, nz = m.vel.shape
+    with open(path, "w") as f:
+        f.write(f" {nx:d} {nz:d} {m.vwater:.6f} {m.vair:.6f}\n")
+        f.write(" ".join(f"{v:.6f}" for v in m.xpos) + "\n")
+        f.write(" ".join(f"{v:.10f}" for v in m.topo) + "\n")
+        f.write(" ".join(f"{v:.10f}" for v in m.zrel) + "\n")
+        for ix in range(nx):
+            f.write(" ".join(f"{v:.10f}" for v in m.vel[ix]) + "\n")
+
+
+def write_interface(path: str, x: np.ndarray, z: np.ndarray) -> None:
+    with open(path, "w") as f:
+        for xi, zi in zip(x, z):
+            f.write(f"{xi:.6f} {zi:.10f}\n")
+
+
+def write_geom(path: str, g: Geometry) -> None:
+    with open(path, "w") as f:
+        f.write(f"{len(g.obs_x)}\n")
+        for i, (ox, oz) in enumerate(zip(g.obs_x, g.obs_z)):
+            f.write(f"S {ox:.6f} {oz:.6f} {len(g.receivers[i])}\n")
+            for (rx, rz, rt, t, u) in g.receivers[i]:
+                f.write(f"R {rx:.6f} {rz:.6f} {rt:d} {t:.6f} {u:.6f}\n")
+
+
+def write_fmt2d_case(outdir: str, m: Model, g: Geometry, write_rays: bool = False) -> None:
+    """Write the model, geometry and an ``fmt2d.in`` (8 lines,
+    ``src/fmt2d_main.f90:59-68``) into ``outdir``."""
+    os.makedirs(outdir, exist_ok=True)
+    write_smesh(os.path.join(outdir, "model.smesh"), m)
+    write_interface(os.path.join(outdir, "moho.dat"), m.xpos, m.moho_z)
+    write_interface(os.path.join(outdir, "sed.dat"), m.xpos, m.sed_z)
+    write_geom(os.path.join(outdir, "geom.dat"), g)
+    with open(os.path.join(outdir, "fmt2d.in"), "w") as f:
+        f.write("model.smesh\nmoho.dat\nsed.dat\ngeom.dat\n")
+        f.write("T\n" if write_rays else "F\n")
+        f.write("arrivals.dat\nobs_times.dat\nrays_src\n")
+
+
+# ---------------------------------------------------------------- named workloads
+
+def plain_pg_case(dx: float = 0.5):
+    """The CPU-runnable plain case: 200 km x 30 km at ``dx`` (401x61 at 0.5 km),
+    20 OBS every 10 km, shots every 0.5 km at 10 m depth, Pg only."""
+    m = make_model(dx=dx)
+    obs = np.arange(5.0, 200.0, 10.0)
+    shots = np.round(np.arange(0.5, 200.0 - 0.25, 0.5), 6)
+    g = make_geometry(obs, shots, m, ray_types=(1,))
+    return m, g
+
+
+def multistage_case(dx: float = 0.1, nobs: int = 60):
+    """The full multistage forward: 0.1 km grid (2001x301), ``nobs`` OBS,
+    1000 shots, the five phases with the reference's offset windows."""
+    m = make_model(dx=dx)
+    obs = 2.0 + 3.3 * np.arange(nobs)
+    shots = np.round(np.arange(1, 2000) * 0.1, 6)[:1000]
+    g = make_geometry(obs, shots, m, ray_types=(1,))
+    return m, g

@@ -1,0 +1,55 @@
+#!/usr/bin/env bash
+# Builds the reference FMT2D executables (fmt2d, invert2d) from the Fortran
+# sources under /root/reference/src into oracle/_ref/, plus the field-dump
+# driver oracle/ref_fields.f90 linked against the reference's own modules.
+#
+# The sources are compiled where they lie, except src/rays2d.f90: it relies on
+# short-circuit evaluation of `.and.` at four sites (rays2d.f90:471-472,
+# 540-541, 604-605, 645-646: `associated(raysec%istart) .and.
+# raysec%istart%id == ...`), which the Fortran standard does not guarantee and
+# amdflang does not provide (the unpatched binary faults on the first Pg ray).
+# A temporary copy of that one file is rewritten so that the id is read once,
+# guarded by `associated`, before the loop; this is a no-op wherever the
+# compiler short-circuits.  The copy lives in a temp dir and is deleted.
+#
+# Compiler: gfortran -O3 when present, otherwise amdflang -O3.
+set -euo pipefail
+REF=${FMT2D_REF:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+if [ ! -d "$REF/src" ]; then
+  echo "reference sources not found at $REF/src; skipping" >&2
+  exit 0
+fi
+if command -v gfortran >/dev/null 2>&1; then FC=gfortran; else FC=amdflang; fi
+FFLAGS="-O3 -cpp"
+mkdir -p "$OUT"
+TMP=$(mktemp -d)
+trap 'rm -rf "$TMP"' EXIT
+cp "$REF/src/rays2d.f90" "$TMP/rays2d.f90"
+chmod u+w "$TMP/rays2d.f90"
+# declare and set the guarded id (after the last declaration / the pt allocation)
+sed -i 's/^\(  real(kind=dp) *:: seed_dist, frac_seed\)$/\1\n  integer :: istart_id_sc/' "$TMP/rays2d.f90"
+sed -i 's/^\(  allocate(pt(2, maxpoints))\)$/\1\n  istart_id_sc = -huge(1)\n  if (associated(raysec%istart)) istart_id_sc = raysec%istart%id/' "$TMP/rays2d.f90"
+# the four two-line conditions
+perl -0pi -e 's/associated\(raysec%istart\) \.and\. &\n(\s*)raysec%istart%id == (iftop|ifbot)%id\) then/.true. .and. &\n$1istart_id_sc == $2%id) then/g' "$TMP/rays2d.f90"
+n=$(grep -c "istart_id_sc ==" "$TMP/rays2d.f90")
+if [ "$n" != 4 ]; then echo "patch applied at $n sites, expected 4" >&2; exit 1; fi
+cd "$TMP"
+S="$REF/src"
+$FC $FFLAGS -c "$S/mod_fmt2d.f90"
+$FC $FFLAGS -c "$S/propagate2d.f90"
+$FC $FFLAGS -c rays2d.f90
+$FC $FFLAGS -c "$S/read_tomo2d.f90"
+$FC $FFLAGS -c "$S/frechet2d.f90"
+$FC $FFLAGS -c "$S/inversion2d.f90"
+$FC $FFLAGS -c "$S/fmt2d_main.f90"
+$FC $FFLAGS -c "$S/invert2d_main.f90"
+$FC $FFLAGS -o "$OUT/fmt2d" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o fmt2d_main.o
+$FC $FFLAGS -o "$OUT/invert2d" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o frechet2d.o inversion2d.o invert2d_main.o
+if [ -f "$HERE/ref_fields.f90" ]; then
+  $FC $FFLAGS -c "$HERE/ref_fields.f90"
+  $FC $FFLAGS -o "$OUT/ref_fields" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o ref_fields.o
+fi
+echo "$FC $FFLAGS" > "$OUT/COMPILER"
+echo "built reference into $OUT with $FC"
