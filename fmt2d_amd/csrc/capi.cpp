@@ -1,0 +1,517 @@
+// C ABI of the MI355X forward path: context, model upload, batched stage
+// schedule of the eikonal solver, field read-back.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "../../include/fmt2d_hip.h"
+#include "eik_kernels.h"
+#include "heap_exit.h"
+#include "model.h"
+
+using namespace fmt2d;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(FMT2D_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+// stage table of run_forward2d (src/invert2d_main.f90:564-811,
+// src/inversion2d.f90:4184-4416): region, start interface, previous stage,
+// water-layer top-row fix (+1 up-going, -1 down-going)
+const int TF_REGION[13] = {0, 5, 2, 2, 6, 6, 1, 1, 1, 1, 1, 5, 1};
+const int TF_ISTART[13] = {0, 2, 2, 3, 2, 4, 2, 1, 2, 2, 1, 2, 2};
+const int TF_PREV[13] = {0, 0, 0, 2, 0, 4, 1, 6, 7, 0, 9, 10, 11};
+const int TF_WATERFIX[13] = {0, 0, 0, 0, 0, 0, 1, -1, 1, 1, -1, 0, 1};
+const int LEVELS[4][4] = {{1, 2, 4, 9}, {3, 5, 6, 10}, {7, 11, 0, 0}, {8, 12, 0, 0}};
+const int LEVEL_N[4] = {4, 4, 2, 2};
+
+}  // namespace
+
+struct fmt2d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool have_model = false;
+  HostModel m;
+  // model on the device
+  DBuf<double> xg, zg, rvel[NREG + 1], ifx[NREG + 1][2], ifz[NREG + 1][2];
+  DBuf<uint16_t> rflags[NREG + 1];
+  DBuf<int> ifcc[NREG + 1][2];
+  DBuf<uint8_t> ifok[NREG + 1][2];
+  DBuf<RegionDev> regs;
+  DBuf<double> isn[NIF + 1][4];  // nx, nz, vel_top, vel_bot
+  DBuf<IsecDev> isecs;
+  DBuf<int> hw_slot;
+  DBuf<double> hw_vhw;
+  DBuf<HeadwaveMap> hw_map;
+  // fields of the last forward
+  int nsrc = 0, nfield = 0;
+  std::vector<int> field_of;           // [isrc*12 + itf-1] -> field index
+  DBuf<double> fT, isT, fGX, fGZ, fOGX, fOGZ, isGX, isGZ;
+  DBuf<FieldDesc> descs;
+  DBuf<StageLink> links;
+  DBuf<uint8_t> tflags[2];
+  DBuf<int2> tlist[2];
+  DBuf<int> tcount;
+  DBuf<int> sel_which, sel_up;
+  DBuf<double> hw_scratch, texit;
+  int* h_count = nullptr;              // pinned
+  fmt2d_forward_stats stats{};
+  std::vector<int> trace;              // active tiles per launch (all levels)
+  ~fmt2d_ctx() {
+    if (h_count) (void)hipHostFree(h_count);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+template <typename T>
+static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.alloc(v.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+// Exit time of one field (see heap_exit.h): seeds are the nodes the
+// reference puts in the heap before marching -- the point-source cell corners
+// and the diagonal neighbours of an on-node source (:2337-2383), or every
+// start-interface node and the grid nodes the sweep seeds (:2141-2222).
+static double field_exit_time(const HostModel& m, const FieldDesc& d, const double* T, const double* maxnb) {
+  const int nx = m.nx, nz = m.nz;
+  const HostRegion& R = m.reg[d.region];
+  std::vector<double> tt, mx;
+  std::vector<char> sd, fx;
+  const int fi = d.fixed_key >= 0 ? d.fixed_key % nx : -10, fj = d.fixed_key >= 0 ? d.fixed_key / nx : -10;
+  const HostInterface* side_if = nullptr;
+  if (d.kind == SEED_IFACE) side_if = &m.isec[d.side == 0 ? R.itop : R.ibot];
+  for (int j = 0; j < nz + 2; j++)
+    for (int i = 0; i < nx; i++) {
+      const int q = j * nx + i;
+      bool seed = false;
+      if (j < nz) {
+        if (!(R.flags[q] & RF_IN)) continue;
+        if (d.kind == SEED_POINT) {
+          seed = (i >= d.ix0 && i <= d.ix0 + 1 && j >= d.iz0 && j <= d.iz0 + 1) ||
+                 (abs(i - fi) == 1 && abs(j - fj) == 1);
+        } else {
+          for (int n = i - 1; n <= i + 1 && !seed; n++) {
+            if (n < 0 || n >= nx) continue;
+            const int cc = side_if->cc[n];
+            if (cc < 0) continue;
+            seed = (d.side == 1) ? (j <= cc && j >= cc - 2) : (j >= cc + 1 && j <= cc + 3);
+          }
+        }
+      } else {
+        seed = d.kind == SEED_IFACE && (j - nz) == d.side;
+      }
+      tt.push_back(T[q]);
+      mx.push_back(maxnb[q]);
+      sd.push_back(seed);
+      fx.push_back(q == d.fixed_key);
+    }
+  return heap_exit_time(tt, mx, sd, fx);
+}
+
+extern "C" {
+
+const char* fmt2d_last_error(void) { return g_err.c_str(); }
+
+int fmt2d_tf_region(int itf) { return (itf >= 1 && itf <= 12) ? TF_REGION[itf] : 0; }
+
+int fmt2d_ctx_create(int device, fmt2d_ctx** out) {
+  if (!out) return fail(FMT2D_EINVAL, "out is NULL");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(FMT2D_EINVAL, "no such HIP device");
+  HIPCHK(hipSetDevice(device));
+  fmt2d_ctx* c = new fmt2d_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_count, sizeof(int) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FMT2D_EHIP, std::string("stream/pinned allocation: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return FMT2D_OK;
+}
+
+int fmt2d_ctx_destroy(fmt2d_ctx* ctx) {
+  if (!ctx) return FMT2D_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, const double* topo, const double* zrel,
+                    const double* vel, double vwater, int nsed, const double* sed_x, const double* sed_z, int nmoho,
+                    const double* moho_x, const double* moho_z) {
+  if (!ctx || !xpos || !topo || !zrel || !vel || !sed_x || !sed_z || !moho_x || !moho_z)
+    return fail(FMT2D_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->have_model = false;
+  ctx->m = HostModel();
+  std::string e = build_model(ctx->m, nx_s, nz_s, xpos, topo, zrel, vel, vwater, nsed, sed_x, sed_z, nmoho, moho_x,
+                              moho_z);
+  if (!e.empty()) return fail(FMT2D_EINVAL, e);
+  HostModel& m = ctx->m;
+  hipStream_t st = ctx->stream;
+  const int nx = m.nx, nz = m.nz;
+  HIPCHK(upload(ctx->xg, m.xg, st));
+  HIPCHK(upload(ctx->zg, m.zg, st));
+  std::vector<RegionDev> rd(NREG + 1);
+  for (int r = 1; r <= NREG; r++) {
+    HostRegion& R = m.reg[r];
+    HIPCHK(upload(ctx->rflags[r], R.flags, st));
+    HIPCHK(upload(ctx->rvel[r], R.vel, st));
+    RegionDev& d = rd[r];
+    d.nx = nx; d.nz = nz; d.dx = m.dx; d.dz = m.dz; d.tol = m.tol;
+    d.xg = ctx->xg.p; d.zg = ctx->zg.p; d.flags = ctx->rflags[r].p; d.vel = ctx->rvel[r].p;
+    const int ids[2] = {R.itop, R.ibot};
+    for (int s = 0; s < 2; s++) {
+      const HostInterface& is = m.isec[ids[s]];
+      HIPCHK(upload(ctx->ifx[r][s], is.x, st));
+      HIPCHK(upload(ctx->ifz[r][s], is.z, st));
+      HIPCHK(upload(ctx->ifcc[r][s], is.cc, st));
+      HIPCHK(upload(ctx->ifok[r][s], is.ok, st));
+      d.ifx[s] = ctx->ifx[r][s].p; d.ifz[s] = ctx->ifz[r][s].p;
+      d.ifcc[s] = ctx->ifcc[r][s].p; d.ifok[s] = ctx->ifok[r][s].p;
+    }
+  }
+  rd[0] = rd[1];
+  HIPCHK(upload(ctx->regs, rd, st));
+  std::vector<IsecDev> idv(NIF + 1);
+  for (int k = 1; k <= NIF; k++) {
+    const HostInterface& is = m.isec[k];
+    HIPCHK(upload(ctx->isn[k][0], is.nrm_x, st));
+    HIPCHK(upload(ctx->isn[k][1], is.nrm_z, st));
+    HIPCHK(upload(ctx->isn[k][2], is.vel_top, st));
+    HIPCHK(upload(ctx->isn[k][3], is.vel_bot, st));
+    idv[k].nx = ctx->isn[k][0].p; idv[k].nz = ctx->isn[k][1].p;
+    idv[k].vel_top = ctx->isn[k][2].p; idv[k].vel_bot = ctx->isn[k][3].p;
+  }
+  idv[0] = idv[1];
+  HIPCHK(upload(ctx->isecs, idv, st));
+  // PmP head-wave map.  The reference addresses the PmP region (6) with the
+  // Moho's rright_node_id, i.e. region 4's top-node numbering
+  // (inversion2d.f90:4298-4301, propagate2d.f90:1593-1597).
+  {
+    const HostRegion& R4 = m.reg[4];
+    const HostRegion& R6 = m.reg[6];
+    const HostInterface& moho = m.isec[4];
+    const int n = moho.n;
+    std::vector<int> slot(n, -1);
+    std::vector<double> vhw(n, 0.0);
+    for (int c = 0; c < n; c++) {
+      const int mref = R4.ngnode + c + 1;  // 1-based node id
+      if (mref < 1 || mref > R6.nnode) continue;
+      const int q = R6.ref_order[mref - 1];
+      slot[c] = q;
+      if (q >= nz * nx) {
+        // interface node of region 6: fastest grid velocity of the Moho cut cell
+        // of its column, below the Moho node of column c
+        const int i3 = q % nx;
+        const int ccz = moho.cc[i3];
+        if (ccz >= 0) {
+          for (int ii = 0; ii <= 1; ii++)
+            for (int jj = 0; jj <= 1; jj++) {
+              const int ixx = i3 + ii, izz = ccz + jj;
+              if (ixx < nx && izz < nz && m.node_region[(size_t)izz * nx + ixx] == 6) {
+                if (m.zg[izz] > moho.z[c] + 0.5 * m.dz)
+                  vhw[c] = std::max(vhw[c], R6.vel[(size_t)izz * nx + ixx]);
+              }
+            }
+        }
+      } else {
+        vhw[c] = R6.vel[q];
+      }
+    }
+    HIPCHK(upload(ctx->hw_slot, slot, st));
+    HIPCHK(upload(ctx->hw_vhw, vhw, st));
+    HeadwaveMap hm;
+    hm.n = n; hm.dxif = moho.x[1] - moho.x[0]; hm.slot = ctx->hw_slot.p; hm.vhw = ctx->hw_vhw.p;
+    std::vector<HeadwaveMap> hv(1, hm);
+    HIPCHK(upload(ctx->hw_map, hv, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->have_model = true;
+  return FMT2D_OK;
+}
+
+int fmt2d_grid_info_get(fmt2d_ctx* ctx, fmt2d_grid_info* info) {
+  if (!ctx || !info) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  const HostModel& m = ctx->m;
+  info->nx = m.nx; info->nz = m.nz; info->dx = m.dx; info->dz = m.dz; info->x0 = m.x0; info->z0 = m.z0;
+  info->region_nnode[0] = 0;
+  for (int r = 1; r <= NREG; r++) info->region_nnode[r] = m.reg[r].nnode;
+  return FMT2D_OK;
+}
+
+int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
+  if (!ctx || (nsrc > 0 && !src_x)) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  if (nsrc < 1) return fail(FMT2D_EINVAL, "nsrc must be >= 1");
+  HIPCHK(hipSetDevice(ctx->device));
+  HostModel& m = ctx->m;
+  hipStream_t st = ctx->stream;
+  const int nx = m.nx, nz = m.nz;
+  const size_t nslot = (size_t)(nz + 2) * nx;
+  const int nfield = nsrc * FMT2D_NTF;
+  const auto t_start = std::chrono::steady_clock::now();
+  // ---- field storage, ordered by level so every level is one contiguous batch
+  std::vector<int> order;  // field index -> (isrc*12 + itf-1)
+  std::vector<int> level_off(5, 0);
+  ctx->field_of.assign(nfield, -1);
+  for (int L = 0; L < 4; L++) {
+    level_off[L] = (int)order.size();
+    for (int s = 0; s < nsrc; s++)
+      for (int k = 0; k < LEVEL_N[L]; k++) {
+        const int itf = LEVELS[L][k];
+        ctx->field_of[s * FMT2D_NTF + itf - 1] = (int)order.size();
+        order.push_back(s * FMT2D_NTF + itf - 1);
+      }
+  }
+  level_off[4] = (int)order.size();
+  if (ctx->nfield != nfield || ctx->fT.n != nslot * nfield) {
+    HIPCHK(ctx->fT.alloc(nslot * nfield));
+    HIPCHK(ctx->fGX.alloc(nslot * nfield));
+    HIPCHK(ctx->fGZ.alloc(nslot * nfield));
+    HIPCHK(ctx->fOGX.alloc(nslot * nfield));
+    HIPCHK(ctx->fOGZ.alloc(nslot * nfield));
+    HIPCHK(ctx->isT.alloc((size_t)nx * nfield));
+    HIPCHK(ctx->isGX.alloc((size_t)nx * nfield));
+    HIPCHK(ctx->isGZ.alloc((size_t)nx * nfield));
+  }
+  ctx->nsrc = nsrc;
+  ctx->nfield = nfield;
+  std::vector<FieldDesc> fd(nfield);
+  std::vector<StageLink> lk(nfield);
+  long long region_nodes = 0;
+  for (int f = 0; f < nfield; f++) {
+    const int s = order[f] / FMT2D_NTF, itf = order[f] % FMT2D_NTF + 1;
+    FieldDesc& d = fd[f];
+    memset(&d, 0, sizeof(d));
+    d.region = TF_REGION[itf];
+    const HostRegion& R = m.reg[d.region];
+    region_nodes += R.nnode;
+    d.T = ctx->fT.p + (size_t)f * nslot;
+    d.GX = ctx->fGX.p + (size_t)f * nslot;
+    d.GZ = ctx->fGZ.p + (size_t)f * nslot;
+    d.OGX = ctx->fOGX.p + (size_t)f * nslot;
+    d.OGZ = ctx->fOGZ.p + (size_t)f * nslot;
+    d.isT = ctx->isT.p + (size_t)f * nx;
+    d.isGX = ctx->isGX.p + (size_t)f * nx;
+    d.isGZ = ctx->isGZ.p + (size_t)f * nx;
+    d.fixed_key = -1;
+    if (TF_PREV[itf] == 0) {
+      // point source on the seafloor (fmt2d_main.f90:545-546), cell corners
+      d.kind = SEED_POINT;
+      d.sx = src_x[s];
+      d.sz = m.iface_z_at(2, d.sx);
+      d.t0 = 0.0;
+      int ix0 = (int)floor((d.sx - m.x0) / m.dx), iz0 = (int)floor((d.sz - m.z0) / m.dz);
+      ix0 = std::max(0, std::min(ix0, nx - 2));
+      iz0 = std::max(0, std::min(iz0, nz - 2));
+      d.ix0 = ix0; d.iz0 = iz0;
+      for (int i = ix0; i <= ix0 + 1; i++)
+        for (int j = iz0; j <= iz0 + 1; j++) {
+          if (!(R.flags[(size_t)j * nx + i] & RF_IN)) continue;
+          const double dx = m.xg[i] - d.sx, dz = m.zg[j] - d.sz;
+          if (sqrt(dx * dx + dz * dz) < 0.01 * m.tol) d.fixed_key = j * nx + i;
+        }
+    } else {
+      d.kind = SEED_IFACE;
+      const int I = TF_ISTART[itf];
+      const int pr = TF_PREV[itf];
+      const HostRegion& Rp = m.reg[TF_REGION[pr]];
+      StageLink& L = lk[f];
+      L.prev = ctx->field_of[s * FMT2D_NTF + pr - 1];
+      if (I == Rp.ibot) L.prev_side = 1;
+      else if (I == Rp.itop) L.prev_side = 0;
+      else return fail(FMT2D_EINVAL, "stage transfer from a non-boundary interface is not supported");
+      if (I == R.itop) d.side = 0;
+      else if (I == R.ibot) d.side = 1;
+      else return fail(FMT2D_EINVAL, "stage sweep from a non-boundary interface is not supported");
+      L.isec = I;
+      L.reflect = 1;
+      L.target_side = (I == R.itop) ? 1 : ((I == R.ibot) ? -1 : 0);
+      L.target_has_iface = 1;
+      L.target_vel = ctx->rvel[d.region].p + (size_t)(nz + d.side) * nx;
+    }
+  }
+  ctx->stats = fmt2d_forward_stats{};
+  ctx->trace.clear();
+  ctx->stats.nsrc = nsrc;
+  ctx->stats.nfield = nfield;
+  ctx->stats.region_nodes = region_nodes;
+  HIPCHK(upload(ctx->descs, fd, st));
+  HIPCHK(upload(ctx->links, lk, st));
+  // ---- tile bookkeeping sized for the largest level
+  const int ntx = (nx + 7) / 8, ntz = (nz + 7) / 8, ntiles = ntx * ntz;
+  int maxlev = 0;
+  for (int L = 0; L < 4; L++) maxlev = std::max(maxlev, level_off[L + 1] - level_off[L]);
+  const size_t nflag = ((size_t)maxlev * ntiles + 3) & ~(size_t)3;
+  for (int b = 0; b < 2; b++) {
+    if (ctx->tflags[b].n < nflag) {
+      HIPCHK(ctx->tflags[b].alloc(nflag));
+      HIPCHK(hipMemsetAsync(ctx->tflags[b].p, 0, nflag, st));
+    }
+    if (ctx->tlist[b].n < (size_t)maxlev * ntiles) HIPCHK(ctx->tlist[b].alloc((size_t)maxlev * ntiles));
+  }
+  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
+  HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
+  const int max_launch = 64 * (ntx + ntz) + 1000;
+  const bool headwave = !(flags & 1);
+  for (int L = 0; L < 4; L++) {
+    const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
+    const FieldDesc* dl = ctx->descs.p + f0;
+    if (L > 0) HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz, ctx->isecs.p, st));
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 2, st));
+    HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tflags[0].p, ctx->tlist[0].p, ctx->tcount.p, st));
+    HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int cur = 0, n = ctx->h_count[0], launches = 0;
+    while (n > 0) {
+      if (launches >= max_launch) {
+        // report a few of the tiles still active
+        std::vector<int2> tail(std::min(n, 8));
+        (void)hipMemcpy(tail.data(), ctx->tlist[cur].p, tail.size() * sizeof(int2), hipMemcpyDeviceToHost);
+        std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
+                          ", " + std::to_string(n) + " tiles active; field/tile:";
+        for (auto& p : tail) {
+          const int itf = order[f0 + p.x] % FMT2D_NTF + 1;
+          msg += " tf" + std::to_string(itf) + "/src" + std::to_string(order[f0 + p.x] / FMT2D_NTF) + "/t" +
+                 std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * 8) + "," + std::to_string((p.y / ntx) * 8) + ")";
+        }
+        return fail(FMT2D_ENOCONV, msg + ")");
+      }
+      const int nxt = cur ^ 1;
+      HIPCHK(hipMemsetAsync(ctx->tcount.p + nxt, 0, sizeof(int), st));
+      HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[cur].p, n, ntx, ntz, ctx->tflags[cur].p, ctx->tflags[nxt].p,
+                        ctx->tlist[nxt].p, ctx->tcount.p + nxt, 64, st));
+      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p + nxt, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      ctx->stats.tiles += n;
+      ctx->trace.push_back(n);
+      n = ctx->h_count[0];
+      cur = nxt;
+      launches++;
+    }
+    ctx->stats.launches += launches;
+    if (cur != 0) {
+      // the seed kernel of the next level writes into list/flags 0: both flag
+      // arrays are clear after convergence, only the parity matters
+    }
+    // heap-exhaustion stop of the reference (propagate2d.f90:80)
+    {
+      HIPCHK(launch_member_time(dl, ctx->regs.p, nf, nx, nz, st));
+      std::vector<double> hT((size_t)nf * nslot), hM((size_t)nf * nslot), te(nf);
+      HIPCHK(hipMemcpyAsync(hT.data(), ctx->fT.p + (size_t)f0 * nslot, hT.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(hM.data(), ctx->fOGX.p + (size_t)f0 * nslot, hM.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int k = 0; k < nf; k++) te[k] = field_exit_time(m, fd[f0 + k], hT.data() + (size_t)k * nslot,
+                                                           hM.data() + (size_t)k * nslot);
+      HIPCHK(upload(ctx->texit, te, st));
+      HIPCHK(launch_truncate(dl, ctx->texit.p, nf, (int)nslot, st));
+    }
+    HIPCHK(launch_refit(dl, ctx->regs.p, nf, nx, nz, st));
+    // water-layer top-row fixes and the PmP head-wave post-pass
+    std::vector<int> which, up, hw;
+    for (int f = f0; f < f0 + nf; f++) {
+      const int itf = order[f] % FMT2D_NTF + 1;
+      if (TF_WATERFIX[itf]) { which.push_back(f - f0); up.push_back(TF_WATERFIX[itf] > 0); }
+      if (itf == 5 && headwave) hw.push_back(f - f0);
+    }
+    if (!which.empty()) {
+      HIPCHK(upload(ctx->sel_which, which, st));
+      HIPCHK(upload(ctx->sel_up, up, st));
+      HIPCHK(launch_water_fix(dl, ctx->regs.p, ctx->sel_which.p, ctx->sel_up.p, (int)which.size(), nx, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    if (!hw.empty()) {
+      HIPCHK(upload(ctx->sel_which, hw, st));
+      if (ctx->hw_scratch.n < (size_t)hw.size() * 2 * nx) HIPCHK(ctx->hw_scratch.alloc((size_t)hw.size() * 2 * nx));
+      HIPCHK(launch_headwave(dl, ctx->regs.p, ctx->sel_which.p, (int)hw.size(), ctx->hw_map.p, ctx->hw_scratch.p, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->stats.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return FMT2D_OK;
+}
+
+int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int max, int* n) {
+  if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
+  *n = (int)ctx->trace.size();
+  if (counts)
+    for (int i = 0; i < std::min(max, *n); i++) counts[i] = ctx->trace[i];
+  return FMT2D_OK;
+}
+
+int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st) {
+  if (!ctx || !st) return fail(FMT2D_EINVAL, "NULL argument");
+  *st = ctx->stats;
+  return FMT2D_OK;
+}
+
+int fmt2d_get_field(fmt2d_ctx* ctx, int isrc, int itf, double* T, double* dTdx, double* dTdz) {
+  if (!ctx || !T || !dTdx || !dTdz) return fail(FMT2D_EINVAL, "NULL argument");
+  if (isrc < 0 || isrc >= ctx->nsrc || itf < 1 || itf > 12) return fail(FMT2D_EINVAL, "field index out of range");
+  HIPCHK(hipSetDevice(ctx->device));
+  const HostModel& m = ctx->m;
+  const size_t nslot = (size_t)(m.nz + 2) * m.nx;
+  const int f = ctx->field_of[isrc * FMT2D_NTF + itf - 1];
+  std::vector<double> t(nslot), gx(nslot), gz(nslot);
+  HIPCHK(hipMemcpyAsync(t.data(), ctx->fT.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(gx.data(), ctx->fOGX.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(gz.data(), ctx->fOGZ.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const HostRegion& R = m.reg[TF_REGION[itf]];
+  for (int k = 0; k < R.nnode; k++) {
+    const int q = R.ref_order[k];
+    T[k] = t[q]; dTdx[k] = gx[q]; dTdz[k] = gz[q];
+  }
+  return FMT2D_OK;
+}
+
+}  // extern "C"

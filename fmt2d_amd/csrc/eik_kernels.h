@@ -1,0 +1,73 @@
+// Device-side descriptors and kernel launchers of the eikonal path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eik_common.h"
+
+namespace fmt2d {
+
+enum { SEED_POINT = 0, SEED_IFACE = 1 };
+
+// One traveltime field (one source x one stage) resident in HBM.
+struct FieldDesc {
+  int region;        // 1..6
+  int kind;          // SEED_POINT / SEED_IFACE
+  double sx, sz, t0; // point source
+  int ix0, iz0;      // lower-left node of the source cell (0-based)
+  int fixed_key;     // grid slot of an on-node source, -1 none
+  int side;          // start interface: 0 = region top, 1 = region bottom
+  const double* isT; // start-interface times after transfer/reflection (nx)
+  const double* isGX;
+  const double* isGZ;
+  double* T;         // (nz+2)*nx traveltime
+  double* GX;        // raw gradient (fixed-point iteration)
+  double* GZ;
+  double* OGX;       // gradient after the refit / fix post-passes (output)
+  double* OGZ;
+};
+
+// Interface stage: where the start-interface values come from.
+struct StageLink {
+  int prev;           // field index of the previous stage
+  int prev_side;      // row of the start interface in the previous region (0 top, 1 bottom)
+  int isec;           // intersection id 1..5 (index into IsecDev array)
+  int reflect;        // apply Snell re-projection
+  int target_side;    // +1 start interface is the target's top, -1 bottom, 0 by gradient sign
+  int target_has_iface;
+  const double* target_vel; // target region's velocity row of that interface (nx)
+};
+
+struct IsecDev {
+  const double* nx;
+  const double* nz;
+  const double* vel_top;
+  const double* vel_bot;
+};
+
+// Moho head-wave post-pass map (src/inversion2d.f90:4296-4345)
+struct HeadwaveMap {
+  int n;
+  double dxif;
+  const int* slot;     // field slot of the node the reference addresses, -1 none
+  const double* vhw;   // head-wave velocity per Moho node
+};
+
+hipError_t launch_fill(const FieldDesc* descs, int nfield, int nslot, hipStream_t st);
+hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const StageLink* links_all, int nfield, int nx,
+                                   int nz, const IsecDev* isecs, hipStream_t st);
+hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
+                             uint8_t* flags_next, int2* list_next, int* count_next, hipStream_t st);
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nlist, int ntx, int ntz,
+                      uint8_t* flags_cur, uint8_t* flags_next, int2* list_next, int* count_next, int max_sweeps,
+                      hipStream_t st);
+hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
+                              hipStream_t st);
+hipError_t launch_truncate(const FieldDesc* descs, const double* texit, int nfield, int nslot, hipStream_t st);
+hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, hipStream_t st);
+hipError_t launch_water_fix(const FieldDesc* descs, const RegionDev* regs, const int* which, const int* up, int nsel,
+                            int nx, hipStream_t st);
+hipError_t launch_headwave(const FieldDesc* descs, const RegionDev* regs, const int* which, int nsel,
+                           const HeadwaveMap* hm, double* scratch, hipStream_t st);
+
+}  // namespace fmt2d

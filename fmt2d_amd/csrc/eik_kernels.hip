@@ -1,0 +1,600 @@
+// MI355X kernels of the multistage eikonal path.
+//
+//  * fill_fields      : reset a batch of fields (T=huge, gradients 0)
+//  * transfer_reflect : interface times of the previous stage, Snell
+//                       re-projection (src/propagate2d.f90:2414-2741)
+//  * seed_tiles       : first active tiles of every field
+//  * fim_tiles        : block fast-iterative solve, one wavefront per 8x8
+//                       tile with a 2-node halo staged in LDS; tiles whose
+//                       border changed activate their neighbours for the
+//                       next launch (deduplicated by per-launch flag arrays)
+//  * refit_gradients  : the reference's post-pass refit of one-sided
+//                       gradients (src/propagate2d.f90:119-123)
+//  * water_top_fix    : top-row gradient fix of water-layer stages
+//                       (src/inversion2d.f90:4264-4292)
+//  * pmp_headwave     : head-wave min-plus sweep along the Moho on the PmP
+//                       up-going stage (src/inversion2d.f90:4294-4409)
+//
+// Fields are FP64 and every update follows the reference's operation order
+// (this file is compiled with -ffp-contract=off): an FP32 traveltime field
+// accumulates ~1.5e-4 s of rounding over 400 nodes, above the 1e-4 s bar, and
+// exact FP64 reproduces the reference's tie decisions (gradient refits).
+#include <hip/hip_runtime.h>
+
+#include "eik_kernels.h"
+#include "eik_update.h"
+
+namespace fmt2d {
+
+// ---------------------------------------------------------------- seeds
+
+// Seed of grid node (i,j) of field d: point-source cell corners
+// (initialize_region_from_source2d, :2307-2411) or the grid nodes 1-3 rows
+// into the region from the start interface, first interface node wins
+// (sweep_region_from_interface2d mapped branch, :2141-2222).
+__device__ inline void seed_grid(const FieldDesc& d, const RegionDev& R, int i, int j, double& t0, double& gx,
+                                 double& gz) {
+  t0 = FMT_HUGE_TIME; gx = 0.0; gz = 0.0;
+  if (!(R.flags[j * R.nx + i] & RF_IN)) return;
+  double v = R.vel[j * R.nx + i];
+  if (v < 0.01) v = 0.01;
+  if (d.kind == SEED_POINT) {
+    if (i < d.ix0 || i > d.ix0 + 1 || j < d.iz0 || j > d.iz0 + 1) return;
+    const double dx = R.xg[i] - d.sx, dz = R.zg[j] - d.sz;
+    const double dist = sqrt(dx * dx + dz * dz);
+    if (dist < 0.01 * R.tol) { t0 = d.t0; return; }
+    t0 = d.t0 + dist / v;
+    gx = dx / (dist * v); gz = dz / (dist * v);
+    return;
+  }
+  const int s = d.side;
+  for (int n = i - 1; n <= i + 1; n++) {
+    if (n < 0 || n >= R.nx) continue;
+    const int cc = R.ifcc[s][n];
+    if (cc < 0) continue;
+    const bool hit = (s == 1) ? (j <= cc && j >= cc - 2) : (j >= cc + 1 && j <= cc + 3);
+    if (!hit) continue;
+    const double px = R.ifx[s][n], pz = R.ifz[s][n];
+    const double dx = R.xg[i] - px, dz = R.zg[j] - pz;
+    const double dist = sqrt(dx * dx + dz * dz);
+    t0 = d.isT[n] + dist / v;
+    gx = dx / (dist * v); gz = dz / (dist * v);
+    return;
+  }
+}
+
+__device__ inline void seed_iface(const FieldDesc& d, int s, int c, double& t0, double& gx, double& gz) {
+  t0 = FMT_HUGE_TIME; gx = 0.0; gz = 0.0;
+  if (d.kind == SEED_IFACE && s == d.side) { t0 = d.isT[c]; gx = d.isGX[c]; gz = d.isGZ[c]; }
+}
+
+// ---------------------------------------------------------------- views
+
+// Global-memory view of one field (post-passes).
+struct GView {
+  const RegionDev* R;
+  const FieldDesc* D;
+  __device__ const RegionDev& reg() const { return *R; }
+  __device__ bool in(int i, int j) const {
+    return i >= 0 && i < R->nx && j >= 0 && j < R->nz && (R->flags[j * R->nx + i] & RF_IN);
+  }
+  __device__ uint16_t flags(int i, int j) const { return R->flags[j * R->nx + i]; }
+  __device__ double T(int i, int j) const { return D->T[j * R->nx + i]; }
+  __device__ double GX(int i, int j) const { return D->GX[j * R->nx + i]; }
+  __device__ double GZ(int i, int j) const { return D->GZ[j * R->nx + i]; }
+  __device__ double vel(int i, int j) const { return R->vel[j * R->nx + i]; }
+  __device__ double xg(int i) const { return R->xg[i]; }
+  __device__ double zg(int j) const { return R->zg[j]; }
+  __device__ double iT(int s, int c) const { return D->T[(R->nz + s) * R->nx + c]; }
+  __device__ double iGX(int s, int c) const { return D->GX[(R->nz + s) * R->nx + c]; }
+  __device__ double iGZ(int s, int c) const { return D->GZ[(R->nz + s) * R->nx + c]; }
+  __device__ double ivel(int s, int c) const { return R->vel[(R->nz + s) * R->nx + c]; }
+  __device__ double ifx(int s, int c) const { return R->ifx[s][c]; }
+  __device__ double ifz(int s, int c) const { return R->ifz[s][c]; }
+  __device__ int ifcc(int s, int c) const { return R->ifcc[s][c]; }
+  __device__ int ifok(int s, int c) const { return R->ifok[s][c]; }
+  __device__ double T0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return t; }
+  __device__ double GX0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return a; }
+  __device__ double GZ0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return b; }
+  __device__ double iT0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return t; }
+  __device__ double iGX0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return a; }
+  __device__ double iGZ0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return b; }
+  __device__ bool fixed(const Member& m) const {
+    return m.kind == 0 && (m.j * R->nx + m.i) == D->fixed_key;
+  }
+};
+
+// LDS view of one tile window (TILE+2*HALO)^2 plus the interface nodes of
+// its columns.  Accessors clamp to the window; the operator never reaches
+// beyond two nodes from an inner node, which the halo covers.
+constexpr int TILE = 8, HALO = 2, WIN = TILE + 2 * HALO, WIN2 = WIN * WIN;
+
+struct TileLds {
+  double T[WIN2];
+  double GX[WIN2], GZ[WIN2], vel[WIN2];
+  uint16_t flags[WIN2];
+  double xg[WIN], zg[WIN];
+  double iT[2 * WIN];
+  double iGX[2 * WIN], iGZ[2 * WIN], ivel[2 * WIN], ifx[2 * WIN], ifz[2 * WIN];
+  int ifcc[2 * WIN];
+  uint8_t ifok[2 * WIN];
+};
+
+struct LView {
+  const RegionDev* R;
+  const FieldDesc* D;
+  const TileLds* L;
+  int wx0, wz0;
+  __device__ int w(int i, int j) const {
+    int a = i - wx0, b = j - wz0;
+    a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a);
+    b = b < 0 ? 0 : (b >= WIN ? WIN - 1 : b);
+    return b * WIN + a;
+  }
+  __device__ int wc(int s, int c) const {
+    int a = c - wx0;
+    a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a);
+    return s * WIN + a;
+  }
+  __device__ const RegionDev& reg() const { return *R; }
+  __device__ bool in(int i, int j) const {
+    return i >= 0 && i < R->nx && j >= 0 && j < R->nz && (L->flags[w(i, j)] & RF_IN);
+  }
+  __device__ uint16_t flags(int i, int j) const { return L->flags[w(i, j)]; }
+  __device__ double T(int i, int j) const { return L->T[w(i, j)]; }
+  __device__ double GX(int i, int j) const { return L->GX[w(i, j)]; }
+  __device__ double GZ(int i, int j) const { return L->GZ[w(i, j)]; }
+  __device__ double vel(int i, int j) const { return L->vel[w(i, j)]; }
+  __device__ double xg(int i) const { int a = i - wx0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->xg[a]; }
+  __device__ double zg(int j) const { int a = j - wz0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->zg[a]; }
+  __device__ double iT(int s, int c) const { return L->iT[wc(s, c)]; }
+  __device__ double iGX(int s, int c) const { return L->iGX[wc(s, c)]; }
+  __device__ double iGZ(int s, int c) const { return L->iGZ[wc(s, c)]; }
+  __device__ double ivel(int s, int c) const { return L->ivel[wc(s, c)]; }
+  __device__ double ifx(int s, int c) const { return L->ifx[wc(s, c)]; }
+  __device__ double ifz(int s, int c) const { return L->ifz[wc(s, c)]; }
+  __device__ int ifcc(int s, int c) const { return L->ifcc[wc(s, c)]; }
+  __device__ int ifok(int s, int c) const { return L->ifok[wc(s, c)]; }
+  __device__ double T0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return t; }
+  __device__ double GX0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return a; }
+  __device__ double GZ0(int i, int j) const { double t, a, b; seed_grid(*D, *R, i, j, t, a, b); return b; }
+  __device__ double iT0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return t; }
+  __device__ double iGX0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return a; }
+  __device__ double iGZ0(int s, int c) const { double t, a, b; seed_iface(*D, s, c, t, a, b); return b; }
+  __device__ bool fixed(const Member& m) const {
+    return m.kind == 0 && (m.j * R->nx + m.i) == D->fixed_key;
+  }
+};
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void fill_fields_kernel(const FieldDesc* __restrict__ descs, int nfield, int nslot) {
+  const long long total = (long long)nfield * nslot;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslot), q = (int)(k % nslot);
+    const FieldDesc& d = descs[f];
+    d.T[q] = FMT_HUGE_TIME;
+    d.GX[q] = 0.0; d.GZ[q] = 0.0;
+    d.OGX[q] = 0.0; d.OGZ[q] = 0.0;
+  }
+}
+
+// Interface times of the previous stage copied onto the start interface
+// (transfer_times_to_intersection mapped branch), then re-projected by
+// Snell's law into the target region when requested.  One thread per
+// (field, column).
+__global__ void transfer_reflect_kernel(const FieldDesc* __restrict__ descs_all, int f0,
+                                        const StageLink* __restrict__ links_all, int nfield, int nx, int nz,
+                                        const IsecDev* __restrict__ isecs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfield * nx) return;
+  const int f = f0 + k / nx, c = k % nx;
+  const FieldDesc& d = descs_all[f];
+  const StageLink& L = links_all[f];
+  if (d.kind != SEED_IFACE) return;
+  const FieldDesc& p = descs_all[L.prev];
+  const int q = (nz + L.prev_side) * nx + c;
+  double t = p.T[q];
+  // the previous stage's gradient after its post-pass (the reference copies
+  // the field as saved, i.e. refitted)
+  double gx = p.OGX[q], gz = p.OGZ[q];
+  if (L.reflect) {
+    const IsecDev& I = isecs[L.isec];
+    const double n0 = I.nx[c], n1 = I.nz[c];
+    if (t >= 0.5e20) { gx = 0.0; gz = 0.0; }
+    else {
+      const double gp = gx * n0 + gz * n1;
+      const double p0 = gx - gp * n0, p1 = gz - gp * n1;
+      int side = L.target_side;                    // +1 top of target, -1 bottom, 0 decide
+      if (side == 0) side = gp < 0.0 ? -1 : 1;
+      double vt = side > 0 ? I.vel_bot[c] : I.vel_top[c];
+      if (!(vt > 0.01)) vt = L.target_has_iface ? (double)L.target_vel[c] : 0.0;
+      if (vt <= 0.01) { t = 1.0e20; gx = 0.0; gz = 0.0; }
+      else {
+        const double pt = 1.0 / vt, pn2 = pt * pt - (p0 * p0 + p1 * p1);
+        if (pn2 < -1.0e-10) { t = 1.0e20; gx = 0.0; gz = 0.0; }
+        else {
+          const double sq = sqrt(pn2 < 0.0 ? 0.0 : pn2);
+          gx = p0 + side * sq * n0; gz = p1 + side * sq * n1;
+        }
+      }
+    }
+  }
+  double* isT = const_cast<double*>(d.isT);
+  double* isGX = const_cast<double*>(d.isGX);
+  double* isGZ = const_cast<double*>(d.isGZ);
+  isT[c] = t; isGX[c] = gx; isGZ[c] = gz;
+}
+
+__device__ inline void activate(int f, int t, int ntiles, uint8_t* flags_next, int2* list_next, int* count_next) {
+  const long long key = (long long)f * ntiles + t;
+  // byte flags: set with an atomic OR on the containing word
+  unsigned int* word = (unsigned int*)((uintptr_t)(flags_next + key) & ~(uintptr_t)3);
+  const unsigned int bit = 1u << (8 * (unsigned int)((uintptr_t)(flags_next + key) & 3));
+  const unsigned int old = atomicOr(word, bit);
+  if (!(old & bit)) {
+    const int pos = atomicAdd(count_next, 1);
+    list_next[pos] = make_int2(f, t);
+  }
+}
+
+// First active tiles: the point-source cell, or every tile within three
+// rows of a start-interface node that carries a time.
+__global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                  int nfield, int ntx, int ntz, uint8_t* flags_next, int2* list_next,
+                                  int* count_next) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const RegionDev& R0 = regs[0];
+  const int nx = R0.nx, nz = R0.nz;
+  if (k >= nfield * nx) return;
+  const int f = k / nx, c = k % nx;
+  const FieldDesc& d = descs[f];
+  const RegionDev& R = regs[d.region];
+  const int ntiles = ntx * ntz;
+  if (d.kind == SEED_POINT) {
+    if (c != 0) return;
+    if (d.fixed_key >= 0) {  // on-node source: alive from the start (:2366-2369)
+      d.T[d.fixed_key] = d.t0; d.GX[d.fixed_key] = 0.0; d.GZ[d.fixed_key] = 0.0;
+    }
+    for (int i = d.ix0; i <= d.ix0 + 1; i++)
+      for (int j = d.iz0; j <= d.iz0 + 1; j++)
+        if (i >= 0 && i < nx && j >= 0 && j < nz) activate(f, (j / TILE) * ntx + i / TILE, ntiles, flags_next, list_next, count_next);
+    return;
+  }
+  if (!(d.isT[c] < 0.5e20)) return;
+  const int cc = R.ifcc[d.side][c];
+  if (cc < 0) return;
+  const int j0 = max(0, cc - 3), j1 = min(nz - 1, cc + 4);
+  for (int i = max(0, c - 1); i <= min(nx - 1, c + 1); i++)
+    for (int j = j0; j <= j1; j += 1)
+      if ((j % TILE) == 0 || j == j0) activate(f, (j / TILE) * ntx + i / TILE, ntiles, flags_next, list_next, count_next);
+}
+
+__global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
+                                                       const RegionDev* __restrict__ regs, const int2* __restrict__ list,
+                                                       int ntx, int ntz, uint8_t* flags_cur, uint8_t* flags_next,
+                                                       int2* list_next, int* count_next, int max_sweeps) {
+  __shared__ TileLds L;
+  const int2 ft = list[blockIdx.x];
+  const int f = ft.x, t = ft.y;
+  const FieldDesc& d = descs[f];
+  const RegionDev& R = regs[d.region];
+  const int nx = R.nx, nz = R.nz, lane = threadIdx.x;
+  const int ntiles = ntx * ntz;
+  const int tx0 = (t % ntx) * TILE, tz0 = (t / ntx) * TILE;
+  const int wx0 = tx0 - HALO, wz0 = tz0 - HALO;
+  if (lane == 0) flags_cur[(long long)f * ntiles + t] = 0;
+  // ---- stage the window
+  for (int k = lane; k < WIN2; k += 64) {
+    const int gi = wx0 + k % WIN, gj = wz0 + k / WIN;
+    if (gi >= 0 && gi < nx && gj >= 0 && gj < nz) {
+      const int q = gj * nx + gi;
+      L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
+      L.vel[k] = R.vel[q]; L.flags[k] = R.flags[q];
+    } else {
+      L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0;
+    }
+  }
+  if (lane < WIN) { const int gi = wx0 + lane; L.xg[lane] = (gi >= 0 && gi < nx) ? R.xg[gi] : 0.0; }
+  else if (lane < 2 * WIN) { const int gj = wz0 + lane - WIN; L.zg[lane - WIN] = (gj >= 0 && gj < nz) ? R.zg[gj] : 0.0; }
+  if (lane < 2 * WIN) {
+    const int s = lane / WIN, gi = wx0 + lane % WIN;
+    if (gi >= 0 && gi < nx) {
+      const int q = (nz + s) * nx + gi;
+      L.iT[lane] = d.T[q]; L.iGX[lane] = d.GX[q]; L.iGZ[lane] = d.GZ[q];
+      L.ivel[lane] = R.vel[q]; L.ifx[lane] = R.ifx[s][gi]; L.ifz[lane] = R.ifz[s][gi];
+      L.ifcc[lane] = R.ifcc[s][gi]; L.ifok[lane] = R.ifok[s][gi];
+    } else {
+      L.iT[lane] = FMT_HUGE_TIME; L.iGX[lane] = 0.0; L.iGZ[lane] = 0.0; L.ivel[lane] = 0.0;
+      L.ifx[lane] = 0.0; L.ifz[lane] = 0.0; L.ifcc[lane] = -1; L.ifok[lane] = 0;
+    }
+  }
+  __syncthreads();
+  LView v{&R, &d, &L, wx0, wz0};
+  // ---- my grid node and (lanes 0..15) my interface node
+  const int gi = tx0 + lane % TILE, gj = tz0 + lane / TILE;
+  const int wg = (gj - wz0) * WIN + (gi - wx0);
+  bool g_act = gi < nx && gj < nz && (L.flags[wg] & RF_IN) && (gj * nx + gi) != d.fixed_key;
+  double g_t0 = FMT_HUGE_TIME, g_gx0 = 0.0, g_gz0 = 0.0;
+  if (g_act) seed_grid(d, R, gi, gj, g_t0, g_gx0, g_gz0);
+  const int is = lane >> 3, ic = tx0 + (lane & 7);
+  bool i_act = false;
+  double i_t0 = FMT_HUGE_TIME, i_gx0 = 0.0, i_gz0 = 0.0;
+  int i_cc = -1;
+  if (lane < 16 && ic < nx - 1) {
+    i_cc = L.ifcc[is * WIN + ic - wx0];
+    i_act = i_cc >= tz0 && i_cc < tz0 + TILE && i_cc >= 0;
+    if (i_act) seed_iface(d, is, ic, i_t0, i_gx0, i_gz0);
+  }
+  const int wi = is * WIN + (ic - wx0);
+  bool g_dirty = false, i_dirty = false;
+  int sweep = 0;
+  bool more = true;
+  for (; sweep < max_sweeps; sweep++) {
+    Best<double> bg, bi;
+    bg.T = g_t0; bg.gx = g_gx0; bg.gz = g_gz0; bg.susp = 0; bg.reg_set = 0;
+    bi.T = i_t0; bi.gx = i_gx0; bi.gz = i_gz0; bi.susp = 0; bi.reg_set = 0;
+    if (g_act) update_grid_node<double>(v, gi, gj, bg);
+    if (i_act) update_iface_node<double>(v, is, ic, bi);
+    __syncthreads();
+    bool ch = false;
+    // bitwise comparison: NaN gradients (which the reference produces in
+    // some stages) must not count as a change on every sweep
+    if (g_act) {
+      if (__double_as_longlong(bg.T) != __double_as_longlong(L.T[wg]) ||
+          __double_as_longlong(bg.gx) != __double_as_longlong(L.GX[wg]) ||
+          __double_as_longlong(bg.gz) != __double_as_longlong(L.GZ[wg])) {
+        L.T[wg] = bg.T; L.GX[wg] = bg.gx; L.GZ[wg] = bg.gz; ch = true; g_dirty = true;
+      }
+    }
+    if (i_act) {
+      if (__double_as_longlong(bi.T) != __double_as_longlong(L.iT[wi]) ||
+          __double_as_longlong(bi.gx) != __double_as_longlong(L.iGX[wi]) ||
+          __double_as_longlong(bi.gz) != __double_as_longlong(L.iGZ[wi])) {
+        L.iT[wi] = bi.T; L.iGX[wi] = bi.gx; L.iGZ[wi] = bi.gz; ch = true; i_dirty = true;
+      }
+    }
+    more = __syncthreads_or(ch);
+    if (!more) break;
+  }
+  // ---- write back and activate neighbours whose halo changed
+  if (g_dirty) {
+    const int q = gj * nx + gi;
+    d.T[q] = L.T[wg]; d.GX[q] = L.GX[wg]; d.GZ[q] = L.GZ[wg];
+  }
+  if (i_dirty) {
+    const int q = (nz + is) * nx + ic;
+    d.T[q] = L.iT[wi]; d.GX[q] = L.iGX[wi]; d.GZ[q] = L.iGZ[wi];
+  }
+  // border position of each dirty node (interface nodes at their cell corner)
+  int li = -1, lj = -1;
+  if (g_dirty) { li = gi - tx0; lj = gj - tz0; }
+  const int li2 = i_dirty ? ic - tx0 : -1, lj2 = i_dirty ? i_cc - tz0 : -1;
+  unsigned mask = 0;
+  auto edge = [&](int a, int b) {
+    unsigned m = 0;
+    if (a < 0) return m;
+    const bool L0 = a < HALO, R0 = a >= TILE - HALO, T0 = b < HALO, B0 = b >= TILE - HALO;
+    if (L0) m |= 1; if (R0) m |= 2; if (T0) m |= 4; if (B0) m |= 8;
+    if (L0 && T0) m |= 16; if (R0 && T0) m |= 32; if (L0 && B0) m |= 64; if (R0 && B0) m |= 128;
+    return m;
+  };
+  mask = edge(li, lj) | edge(li2, lj2);
+  if (more) mask |= 256;  // sweep cap reached: run this tile again
+  // wave-wide OR of the masks
+  for (int off = 32; off > 0; off >>= 1) mask |= (unsigned)__shfl_xor((int)mask, off, 64);
+  if (lane < 9 && (mask & (1u << lane))) {
+    const int tix = t % ntx, tiz = t / ntx;
+    const int dxs[9] = {-1, 1, 0, 0, -1, 1, -1, 1, 0};
+    const int dzs[9] = {0, 0, -1, 1, -1, -1, 1, 1, 0};
+    const int nxi = tix + dxs[lane], nzi = tiz + dzs[lane];
+    if (nxi >= 0 && nxi < ntx && nzi >= 0 && nzi < ntz)
+      activate(f, nzi * ntx + nxi, ntiles, flags_next, list_next, count_next);
+  }
+}
+
+// Post-pass: refit one-sided gradients of grid nodes; everything else keeps
+// the raw gradient.  One thread per (field, slot).
+__global__ void refit_gradients_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                       int nfield, int nx, int nz) {
+  const int nslot = (nz + 2) * nx;
+  const long long total = (long long)nfield * nslot;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslot), q = (int)(k % nslot);
+    const FieldDesc& d = descs[f];
+    const RegionDev& R = regs[d.region];
+    double ox = d.GX[q], oz = d.GZ[q];
+    if (q < nz * nx) {
+      const int i = q % nx, j = q / nx;
+      if ((R.flags[q] & RF_IN) && q != d.fixed_key && d.T[q] < 0.5e20) {
+        GView v{&R, &d};
+        double t0, a, b;
+        seed_grid(d, R, i, j, t0, a, b);
+        if (grid_suspect<double>(v, i, j, t0)) {
+          double gx, gz;
+          fit_gradient_grid<double>(v, i, j, gx, gz);
+          ox = gx; oz = gz;
+        }
+      }
+    }
+    d.OGX[q] = ox; d.OGZ[q] = oz;
+  }
+}
+
+// Top-row gradient fix of water-layer stages: an up-going stage needs
+// dT/dz < 0 at the sea surface, a down-going one dT/dz > 0.
+__global__ void water_top_fix_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                     const int* __restrict__ which, const int* __restrict__ up, int nsel) {
+  const RegionDev& R0 = regs[0];
+  const int nx = R0.nx;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsel * nx) return;
+  const int f = which[k / nx], c = k % nx;
+  const FieldDesc& d = descs[f];
+  const RegionDev& R = regs[d.region];
+  if (!(R.flags[c] & RF_IN) || !(R.flags[nx + c] & RF_IN)) return;
+  const double g = d.OGZ[c];
+  const bool bad = up[k / nx] ? (g > 0.0 || fabs(g) < 0.01) : (g < 0.0 || fabs(g) < 0.01);
+  if (bad) { d.OGX[c] = d.OGX[nx + c]; d.OGZ[c] = d.OGZ[nx + c]; }
+}
+
+// PmP head-wave post-pass (one thread per field): min-plus sweeps of the
+// Moho node times with the fastest velocity just below, then gradient
+// refit of the alive Moho nodes.  The node ids come from the Moho's
+// rright_node_id / rleft_node_id maps of the reference (region 4's top nodes
+// first), evaluated against the PmP region's own node numbering exactly as
+// the reference does.
+__global__ void pmp_headwave_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                    const int* __restrict__ which, int nsel, const HeadwaveMap* __restrict__ hm,
+                                    double* __restrict__ scratch) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsel) return;
+  const FieldDesc& d = descs[which[k]];
+  const RegionDev& R = regs[d.region];
+  const int nx = R.nx, n = hm->n;
+  double* thw = scratch + (size_t)k * 2 * n;
+  double* vhw = thw + n;
+  for (int c = 0; c < n; c++) {
+    const int q = hm->slot[c];
+    thw[c] = q >= 0 ? d.T[q] : 1.0e20;
+    vhw[c] = hm->vhw[c];
+  }
+  const double dxif = hm->dxif;
+  for (int c = 1; c < n; c++) { const double v = vhw[c]; if (v > 0.0) { const double x = thw[c - 1] + dxif / v; if (x < thw[c]) thw[c] = x; } }
+  for (int c = n - 2; c >= 0; c--) { const double v = vhw[c]; if (v > 0.0) { const double x = thw[c + 1] + dxif / v; if (x < thw[c]) thw[c] = x; } }
+  for (int c = 0; c < n; c++) {
+    const int q = hm->slot[c];
+    if (q >= 0 && thw[c] < d.T[q] - 1.0e-10) d.T[q] = thw[c];
+  }
+  GView v{&R, &d};
+  for (int c = 0; c < n; c++) {
+    const int q = hm->slot[c];
+    if (q < 0 || !(d.T[q] < 0.5e20)) continue;   // alive nodes only
+    if (q < R.nz * nx) {
+      double gx, gz;
+      fit_gradient_grid<double>(v, q % nx, q / nx, gx, gz);
+      d.OGX[q] = gx; d.OGZ[q] = gz;
+    } else {
+      double dtdx;
+      if (c > 0 && c < n - 1) dtdx = (thw[c + 1] - thw[c - 1]) / (2.0 * dxif);
+      else if (c == 0 && n > 1) dtdx = (thw[1] - thw[0]) / dxif;
+      else if (c == n - 1 && n > 1) dtdx = (thw[n - 1] - thw[n - 2]) / dxif;
+      else dtdx = 0.0;
+      double dtdz = 0.0;
+      const double vv = vhw[c];
+      if (vv > 0.0) { const double s = 1.0 / (vv * vv) - dtdx * dtdx; dtdz = s > 0.0 ? sqrt(s) : 0.0; }
+      d.OGX[q] = dtdx; d.OGZ[q] = dtdz;
+    }
+  }
+}
+
+// Largest time among each node's connected nodes (written into OGX, which the
+// refit pass overwrites afterwards), the input of the heap-exhaustion rule.
+__global__ void member_time_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                   int nfield, int nx, int nz) {
+  const int nslot = (nz + 2) * nx;
+  const long long total = (long long)nfield * nslot;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslot), q = (int)(k % nslot);
+    const FieldDesc& d = descs[f];
+    const RegionDev& R = regs[d.region];
+    GView v{&R, &d};
+    double mx = -1.0;
+    if (q < nz * nx) {
+      if (R.flags[q] & RF_IN) mx = max_member_time_grid<double>(v, q % nx, q / nx);
+    } else {
+      mx = max_member_time_iface<double>(v, q / nx - nz, q % nx);
+    }
+    d.OGX[q] = mx;
+  }
+}
+
+// Nodes the reference never reaches once its heap ran empty (times above the
+// exit time of their field) go back to huge_time with a zero gradient.
+__global__ void truncate_kernel(const FieldDesc* __restrict__ descs, const double* __restrict__ texit, int nfield,
+                                int nslot) {
+  const long long total = (long long)nfield * nslot;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslot), q = (int)(k % nslot);
+    const FieldDesc& d = descs[f];
+    const double t = d.T[q];
+    if (t > texit[f] && t < 0.5e20) { d.T[q] = FMT_HUGE_TIME; d.GX[q] = 0.0; d.GZ[q] = 0.0; }
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
+                              hipStream_t st) {
+  const long long total = (long long)nfield * (nz + 2) * nx;
+  int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(member_time_kernel, dim3(blocks), dim3(256), 0, st, descs, regs, nfield, nx, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_truncate(const FieldDesc* descs, const double* texit, int nfield, int nslot, hipStream_t st) {
+  const long long total = (long long)nfield * nslot;
+  int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(truncate_kernel, dim3(blocks), dim3(256), 0, st, descs, texit, nfield, nslot);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(const FieldDesc* descs, int nfield, int nslot, hipStream_t st) {
+  const long long total = (long long)nfield * nslot;
+  int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(fill_fields_kernel, dim3(blocks), dim3(256), 0, st, descs, nfield, nslot);
+  return hipGetLastError();
+}
+
+hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const StageLink* links_all, int nfield, int nx,
+                                   int nz, const IsecDev* isecs, hipStream_t st) {
+  const int total = nfield * nx;
+  hipLaunchKernelGGL(transfer_reflect_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs_all, f0, links_all,
+                     nfield, nx, nz, isecs);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
+                             uint8_t* flags_next, int2* list_next, int* count_next, hipStream_t st) {
+  const int total = nfield * nx;
+  hipLaunchKernelGGL(seed_tiles_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, nfield, ntx, ntz,
+                     flags_next, list_next, count_next);
+  return hipGetLastError();
+}
+
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nlist, int ntx, int ntz,
+                      uint8_t* flags_cur, uint8_t* flags_next, int2* list_next, int* count_next, int max_sweeps,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(fim_tiles_kernel, dim3(nlist), dim3(64), 0, st, descs, regs, list, ntx, ntz, flags_cur,
+                     flags_next, list_next, count_next, max_sweeps);
+  return hipGetLastError();
+}
+
+hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, hipStream_t st) {
+  const long long total = (long long)nfield * (nz + 2) * nx;
+  int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(refit_gradients_kernel, dim3(blocks), dim3(256), 0, st, descs, regs, nfield, nx, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_water_fix(const FieldDesc* descs, const RegionDev* regs, const int* which, const int* up, int nsel,
+                            int nx, hipStream_t st) {
+  const int total = nsel * nx;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(water_top_fix_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, which, up, nsel);
+  return hipGetLastError();
+}
+
+hipError_t launch_headwave(const FieldDesc* descs, const RegionDev* regs, const int* which, int nsel,
+                           const HeadwaveMap* hm, double* scratch, hipStream_t st) {
+  if (nsel == 0) return hipSuccess;
+  hipLaunchKernelGGL(pmp_headwave_kernel, dim3((nsel + 63) / 64), dim3(64), 0, st, descs, regs, which, nsel, hm,
+                     scratch);
+  return hipGetLastError();
+}
+
+}  // namespace fmt2d

@@ -1,0 +1,131 @@
+"""ctypes binding of the MI355X C ABI (include/fmt2d_hip.h).
+
+The product path runs only through libfmt2d_hip.so; there is no CPU
+fallback.  Loading fails loudly when the library is missing or was built
+for another architecture.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import formats
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfmt2d_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("x0", C.c_double), ("z0", C.c_double), ("region_nnode", C.c_int * 7)]
+
+
+class ForwardStats(C.Structure):
+    _fields_ = [("nsrc", C.c_int), ("nfield", C.c_int), ("launches", C.c_int),
+                ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double)]
+
+
+class Fmt2dError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise Fmt2dError(f"{LIB_PATH} is missing: build it with `python -m fmt2d_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    lib.fmt2d_last_error.restype = C.c_char_p
+    lib.fmt2d_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    lib.fmt2d_ctx_destroy.argtypes = [C.c_void_p]
+    lib.fmt2d_set_model.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                                    C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    lib.fmt2d_grid_info_get.argtypes = [C.c_void_p, C.POINTER(GridInfo)]
+    lib.fmt2d_forward.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
+    lib.fmt2d_forward_stats_get.argtypes = [C.c_void_p, C.POINTER(ForwardStats)]
+    lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
+    lib.fmt2d_tf_region.argtypes = [C.c_int]
+    lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+    _LIB = lib
+    return lib
+
+
+# exported symbols declared in include/fmt2d_hip.h
+EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
+           "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
+           "fmt2d_tf_region",
+           "fmt2d_get_field")
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise Fmt2dError(f"fmt2d error {rc}: {load().fmt2d_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """Device context: model, resident traveltime fields."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = C.c_void_p()
+        _check(self._lib.fmt2d_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.info: Optional[GridInfo] = None
+
+    def close(self):
+        if self._h:
+            self._lib.fmt2d_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, smesh: formats.Smesh, sed_x, sed_z, moho_x, moho_z):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (smesh.xpos, smesh.topo, smesh.zrel, smesh.vel, sed_x, sed_z, moho_x, moho_z)]
+        xp, tp, zr, vl, sx, sz, mx, mz = keep
+        _check(self._lib.fmt2d_set_model(self._h, len(xp), len(zr), _p(xp), _p(tp), _p(zr), _p(vl),
+                                         float(smesh.vwater), len(sx), _p(sx), _p(sz), len(mx), _p(mx), _p(mz)))
+        info = GridInfo()
+        _check(self._lib.fmt2d_grid_info_get(self._h, C.byref(info)))
+        self.info = info
+
+    def forward(self, src_x: Sequence[float], headwave: bool = True):
+        xs = np.ascontiguousarray(src_x, dtype=np.float64)
+        _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
+
+    def stats(self) -> ForwardStats:
+        st = ForwardStats()
+        _check(self._lib.fmt2d_forward_stats_get(self._h, C.byref(st)))
+        return st
+
+    def trace(self):
+        n = C.c_int()
+        _check(self._lib.fmt2d_forward_trace(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(self._lib.fmt2d_forward_trace(self._h, out.ctypes.data_as(_ip), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def field(self, isrc: int, itf: int):
+        """(T, dT/dx, dT/dz) of field itf (1..12) of source isrc, reference node order."""
+        n = self.info.region_nnode[self._lib.fmt2d_tf_region(itf)]
+        T, gx, gz = np.empty(n), np.empty(n), np.empty(n)
+        _check(self._lib.fmt2d_get_field(self._h, isrc, itf, _p(T), _p(gx), _p(gz)))
+        return T, gx, gz

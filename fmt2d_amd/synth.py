@@ -7,11 +7,11 @@ reference executables and to this package.
 
 Model family (all depths in km below sea level, velocities in km/s):
 
-* water 1.5 km/s above a seafloor ``3 + 1*sin(2*pi*x/80)``;
+* water 1.5 km/s above a seafloor ``3 + 1*sin(2*pi*x/80 + 0.1)``;
 * sediments 1.8 -> 3.0 km/s, linear in depth below the seafloor, thickness
   ``1.5 + 0.5*sin(2*pi*x/60 + 0.7)`` (1-2 km);
 * crust 4.5 -> 7.0 km/s, linear from the sediment base to the Moho;
-* Moho ``15 + 2*sin(2*pi*x/70)``; mantle 8.0 km/s;
+* Moho ``15 + 2*sin(2*pi*x/70 + 0.1)``; mantle 8.0 km/s;
 * optional checkerboard ``v *= 1 + a*sign(sin(pi*x/Lx)*sin(pi*z/Lz))`` below
   the seafloor;
 * optional seeded random Fourier perturbation of the sediment base and Moho.
@@ -57,8 +57,15 @@ class Geometry:
         return sum(len(r) for r in self.receivers)
 
 
+# Phases of the seafloor and Moho sinusoids keep interface nodes off the grid
+# rows: an interface node lying exactly on a grid node makes the reference's
+# interface re-seeding produce a 0/0 gradient and a heap-order tie.
+SEAFLOOR_PHASE = 0.1
+MOHO_PHASE = 0.1
+
+
 def seafloor(x):
-    return 3.0 + 1.0 * np.sin(2.0 * np.pi * x / 80.0)
+    return 3.0 + 1.0 * np.sin(2.0 * np.pi * x / 80.0 + SEAFLOOR_PHASE)
 
 
 def sediment_thickness(x):
@@ -66,7 +73,7 @@ def sediment_thickness(x):
 
 
 def moho(x, wavelength=70.0):
-    return 15.0 + 2.0 * np.sin(2.0 * np.pi * x / wavelength)
+    return 15.0 + 2.0 * np.sin(2.0 * np.pi * x / wavelength + MOHO_PHASE)
 
 
 def fourier_perturbation(x, rng, nmodes=6, amp_total=1.5, lmin=10.0, lmax=100.0):

@@ -921,7 +921,12 @@ void orc_propagate(Model *M, Region *R) {
     F.newalive = F.tree[1];
     R->status[F.newalive] = 0;
     heap_remove_root(&F);
-    if (F.ntree == 0 && !first) break;
+    if (F.ntree == 0 && !first) {
+      if (getenv("ORC_DEBUG_EXIT"))
+        fprintf(stderr, "early exit: region %d node %d (i1=%d i2=%d i3=%d) T=%.9f\n", R->id, F.newalive,
+                R->i1[F.newalive], R->i2[F.newalive], R->i3[F.newalive], R->T[F.newalive]);
+      break;
+    }
     first = 0;
     find_connected_nodes(&F, F.newalive);
     int ntrial = F.n_connode, trial[64];
