@@ -83,15 +83,21 @@ struct fmt2d_ctx {
   DBuf<double> fT, isT, fGX, fGZ, fOGX, fOGZ, isGX, isGZ;
   DBuf<FieldDesc> descs;
   DBuf<StageLink> links;
-  DBuf<uint8_t> tflags[2];
-  DBuf<int2> tlist[2];
-  DBuf<int> tcount;
+  DBuf<unsigned int> tin;             // tile in active set
+  DBuf<unsigned long long> tkey;      // tile arrival key
+  DBuf<unsigned long long> tfront;    // per-field front key
+  DBuf<int2> tlist[3];                // active set, its next state, due tiles
+  DBuf<int> tcount;                   // [due, next]
+  double window_tiles = 2.0;          // window in tile crossing times
+  int max_sweeps = 64;
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch, texit;
   int* h_count = nullptr;              // pinned
+  std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
   fmt2d_forward_stats stats{};
   std::vector<int> trace;              // active tiles per launch (all levels)
   ~fmt2d_ctx() {
+    for (auto e : ev) (void)hipEventDestroy(e);
     if (h_count) (void)hipHostFree(h_count);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -384,30 +390,33 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   const int ntx = (nx + 7) / 8, ntz = (nz + 7) / 8, ntiles = ntx * ntz;
   int maxlev = 0;
   for (int L = 0; L < 4; L++) maxlev = std::max(maxlev, level_off[L + 1] - level_off[L]);
-  const size_t nflag = ((size_t)maxlev * ntiles + 3) & ~(size_t)3;
-  for (int b = 0; b < 2; b++) {
-    if (ctx->tflags[b].n < nflag) {
-      HIPCHK(ctx->tflags[b].alloc(nflag));
-      HIPCHK(hipMemsetAsync(ctx->tflags[b].p, 0, nflag, st));
-    }
-    if (ctx->tlist[b].n < (size_t)maxlev * ntiles) HIPCHK(ctx->tlist[b].alloc((size_t)maxlev * ntiles));
+  const size_t ntt = (size_t)maxlev * ntiles;
+  if (ctx->tin.n < ntt) {
+    HIPCHK(ctx->tin.alloc(ntt));
+    HIPCHK(ctx->tkey.alloc(ntt));
+    HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
+    HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
+    for (int b = 0; b < 3; b++) HIPCHK(ctx->tlist[b].alloc(ntt));
   }
+  if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
   if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
-  const int max_launch = 64 * (ntx + ntz) + 1000;
+  const int max_launch = 64 * (ntx + ntz) + 4000;
   const bool headwave = !(flags & 1);
+  // arrival-time window: `window_tiles` tile crossings at 4 km/s
+  const double window = ctx->window_tiles * 8.0 * std::max(m.dx, m.dz) / 4.0;
   for (int L = 0; L < 4; L++) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
     const FieldDesc* dl = ctx->descs.p + f0;
     if (L > 0) HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz, ctx->isecs.p, st));
     HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 2, st));
-    HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tflags[0].p, ctx->tlist[0].p, ctx->tcount.p, st));
-    HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
+                             ctx->tcount.p + 1, st));
+    HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     int cur = 0, n = ctx->h_count[0], launches = 0;
     while (n > 0) {
       if (launches >= max_launch) {
-        // report a few of the tiles still active
         std::vector<int2> tail(std::min(n, 8));
         (void)hipMemcpy(tail.data(), ctx->tlist[cur].p, tail.size() * sizeof(int2), hipMemcpyDeviceToHost);
         std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
@@ -420,22 +429,29 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         return fail(FMT2D_ENOCONV, msg + ")");
       }
       const int nxt = cur ^ 1;
-      HIPCHK(hipMemsetAsync(ctx->tcount.p + nxt, 0, sizeof(int), st));
-      HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[cur].p, n, ntx, ntz, ctx->tflags[cur].p, ctx->tflags[nxt].p,
-                        ctx->tlist[nxt].p, ctx->tcount.p + nxt, 64, st));
-      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p + nxt, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 2, st));
+      HIPCHK(launch_schedule(ctx->tlist[cur].p, n, nf, ntx, ntiles, ctx->tkey.p, ctx->tfront.p, window,
+                             launches & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p, ctx->tlist[nxt].p,
+                             ctx->tcount.p + 1, st));
+      const size_t ei = 2 * (size_t)ctx->stats.launches + 2 * (size_t)launches;
+      while (ctx->ev.size() < ei + 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ctx->ev.push_back(e);
+      }
+      HIPCHK(hipEventRecord(ctx->ev[ei], st));
+      HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, n, ctx->tcount.p, ntx, ntz, ctx->tin.p, ctx->tkey.p,
+                        ctx->tlist[nxt].p, ctx->tcount.p + 1, ctx->max_sweeps, st));
+      HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
+      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      ctx->stats.tiles += n;
-      ctx->trace.push_back(n);
-      n = ctx->h_count[0];
+      ctx->stats.tiles += ctx->h_count[0];
+      ctx->trace.push_back(ctx->h_count[0]);
+      n = ctx->h_count[1];
       cur = nxt;
       launches++;
     }
     ctx->stats.launches += launches;
-    if (cur != 0) {
-      // the seed kernel of the next level writes into list/flags 0: both flag
-      // arrays are clear after convergence, only the parity matters
-    }
     // heap-exhaustion stop of the reference (propagate2d.f90:80)
     {
       HIPCHK(launch_member_time(dl, ctx->regs.p, nf, nx, nz, st));
@@ -474,6 +490,20 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   HIPCHK(hipStreamSynchronize(st));
   ctx->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  double fim_ms = 0.0;
+  for (int k = 0; k < ctx->stats.launches; k++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
+    fim_ms += ms;
+  }
+  ctx->stats.ms_fim = fim_ms;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps) {
+  if (!ctx || !(window_tiles > 0.0) || max_sweeps < 1) return fail(FMT2D_EINVAL, "bad tuning parameters");
+  ctx->window_tiles = window_tiles;
+  ctx->max_sweeps = max_sweeps;
   return FMT2D_OK;
 }
 

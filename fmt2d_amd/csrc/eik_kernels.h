@@ -57,10 +57,14 @@ hipError_t launch_fill(const FieldDesc* descs, int nfield, int nslot, hipStream_
 hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const StageLink* links_all, int nfield, int nx,
                                    int nz, const IsecDev* isecs, hipStream_t st);
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
-                             uint8_t* flags_next, int2* list_next, int* count_next, hipStream_t st);
-hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nlist, int ntx, int ntz,
-                      uint8_t* flags_cur, uint8_t* flags_next, int2* list_next, int* count_next, int max_sweeps,
-                      hipStream_t st);
+                             unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
+                             hipStream_t st);
+hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int ntiles, unsigned long long* tkey,
+                           unsigned long long* front, double window, int color, unsigned int* in_set, int2* due,
+                           int* count_due, int2* carry, int* count_carry, hipStream_t st);
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
+                      int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
+                      int* count_next, int max_sweeps, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 hipError_t launch_truncate(const FieldDesc* descs, const double* texit, int nfield, int nslot, hipStream_t st);

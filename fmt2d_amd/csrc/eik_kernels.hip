@@ -227,23 +227,39 @@ __global__ void transfer_reflect_kernel(const FieldDesc* __restrict__ descs_all,
   isT[c] = t; isGX[c] = gx; isGZ[c] = gz;
 }
 
-__device__ inline void activate(int f, int t, int ntiles, uint8_t* flags_next, int2* list_next, int* count_next) {
-  const long long key = (long long)f * ntiles + t;
-  // byte flags: set with an atomic OR on the containing word
-  unsigned int* word = (unsigned int*)((uintptr_t)(flags_next + key) & ~(uintptr_t)3);
-  const unsigned int bit = 1u << (8 * (unsigned int)((uintptr_t)(flags_next + key) & 3));
-  const unsigned int old = atomicOr(word, bit);
-  if (!(old & bit)) {
+// ---------------------------------------------------------------- tile schedule
+//
+// Active tiles form a set (in_set flag per field x tile) with an arrival key
+// (smallest time of the change that activated it, as ordered bits of a
+// non-negative double).  Each launch processes only the tiles whose key lies
+// within `window` seconds of their field's front (smallest key): tiles are
+// solved roughly in arrival-time order, as the reference's heap does, so most
+// tiles are final after one or two visits instead of being re-solved for
+// every transient upstream value.  Tiles outside the window are carried over.
+
+// "no key": a byte-repeated pattern (about 3.4e296 as a double) so the key
+// array can be reset with a memset
+constexpr unsigned long long KEY_INF = 0x7F7F7F7F7F7F7F7Full;
+
+__device__ inline unsigned long long key_of(double t) {
+  return t <= 0.0 ? 0ull : (unsigned long long)__double_as_longlong(t);
+}
+
+__device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key, unsigned int* in_set,
+                                unsigned long long* tkey, int2* list_next, int* count_next) {
+  const long long k = (long long)f * ntiles + t;
+  atomicMin(tkey + k, key);
+  if (atomicExch(in_set + k, 1u) == 0u) {
     const int pos = atomicAdd(count_next, 1);
     list_next[pos] = make_int2(f, t);
   }
 }
 
-// First active tiles: the point-source cell, or every tile within three
-// rows of a start-interface node that carries a time.
+// First active tiles: the point-source cell, or every tile within three rows
+// of a start-interface node that carries a time.
 __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
-                                  int nfield, int ntx, int ntz, uint8_t* flags_next, int2* list_next,
-                                  int* count_next) {
+                                  int nfield, int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey,
+                                  int2* list_next, int* count_next) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const RegionDev& R0 = regs[0];
   const int nx = R0.nx, nz = R0.nz;
@@ -259,23 +275,64 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
     }
     for (int i = d.ix0; i <= d.ix0 + 1; i++)
       for (int j = d.iz0; j <= d.iz0 + 1; j++)
-        if (i >= 0 && i < nx && j >= 0 && j < nz) activate(f, (j / TILE) * ntx + i / TILE, ntiles, flags_next, list_next, count_next);
+        if (i >= 0 && i < nx && j >= 0 && j < nz)
+          add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key_of(d.t0), in_set, tkey, list_next, count_next);
     return;
   }
   if (!(d.isT[c] < 0.5e20)) return;
   const int cc = R.ifcc[d.side][c];
   if (cc < 0) return;
   const int j0 = max(0, cc - 3), j1 = min(nz - 1, cc + 4);
+  const unsigned long long key = key_of(d.isT[c]);
   for (int i = max(0, c - 1); i <= min(nx - 1, c + 1); i++)
-    for (int j = j0; j <= j1; j += 1)
-      if ((j % TILE) == 0 || j == j0) activate(f, (j / TILE) * ntx + i / TILE, ntiles, flags_next, list_next, count_next);
+    for (int j = j0; j <= j1; j++)
+      if ((j % TILE) == 0 || j == j0) add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key, in_set, tkey, list_next, count_next);
+}
+
+__global__ void reset_front_kernel(unsigned long long* front, int nfield) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nfield) front[f] = KEY_INF;
+}
+
+__global__ void front_kernel(const int2* __restrict__ list, int n, int ntiles, const unsigned long long* __restrict__ tkey,
+                             unsigned long long* front) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 e = list[k];
+  atomicMin(front + e.x, tkey[(long long)e.x * ntiles + e.y]);
+}
+
+// Splits the active set into tiles due this launch and tiles carried over.
+// Tiles of one checkerboard colour run per launch, so two edge-adjacent
+// tiles never iterate against each other's stale halos (which can lock a
+// pair into a two-cycle of the non-monotone second-order switch).
+__global__ void split_kernel(const int2* __restrict__ list, int n, int ntx, int ntiles, unsigned long long* tkey,
+                             const unsigned long long* __restrict__ front, double window, int color,
+                             unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 e = list[k];
+  const long long q = (long long)e.x * ntiles + e.y;
+  const double key = __longlong_as_double((long long)tkey[q]);
+  const double fr = __longlong_as_double((long long)front[e.x]);
+  const int tc = ((e.y % ntx) + (e.y / ntx)) & 1;
+  if (key <= fr + window && tc == color) {
+    tkey[q] = KEY_INF;
+    in_set[q] = 0u;
+    due[atomicAdd(count_due, 1)] = e;
+  } else {
+    carry[atomicAdd(count_carry, 1)] = e;
+  }
 }
 
 __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
                                                        const RegionDev* __restrict__ regs, const int2* __restrict__ list,
-                                                       int ntx, int ntz, uint8_t* flags_cur, uint8_t* flags_next,
+                                                       const int* __restrict__ count_due, int ntx, int ntz,
+                                                       unsigned int* in_set, unsigned long long* tkey,
                                                        int2* list_next, int* count_next, int max_sweeps) {
   __shared__ TileLds L;
+  // the grid is sized for the whole active set; only the due tiles run
+  if ((int)blockIdx.x >= *count_due) return;
   const int2 ft = list[blockIdx.x];
   const int f = ft.x, t = ft.y;
   const FieldDesc& d = descs[f];
@@ -284,7 +341,6 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   const int ntiles = ntx * ntz;
   const int tx0 = (t % ntx) * TILE, tz0 = (t / ntx) * TILE;
   const int wx0 = tx0 - HALO, wz0 = tz0 - HALO;
-  if (lane == 0) flags_cur[(long long)f * ntiles + t] = 0;
   // ---- stage the window
   for (int k = lane; k < WIN2; k += 64) {
     const int gi = wx0 + k % WIN, gj = wz0 + k / WIN;
@@ -315,7 +371,8 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   // ---- my grid node and (lanes 0..15) my interface node
   const int gi = tx0 + lane % TILE, gj = tz0 + lane / TILE;
   const int wg = (gj - wz0) * WIN + (gi - wx0);
-  bool g_act = gi < nx && gj < nz && (L.flags[wg] & RF_IN) && (gj * nx + gi) != d.fixed_key;
+  const bool g_act = gi < nx && gj < nz && (L.flags[wg] & RF_IN) && (gj * nx + gi) != d.fixed_key;
+  const int g_color = (gi & 1) | ((gj & 1) << 1);
   double g_t0 = FMT_HUGE_TIME, g_gx0 = 0.0, g_gz0 = 0.0;
   if (g_act) seed_grid(d, R, gi, gj, g_t0, g_gx0, g_gz0);
   const int is = lane >> 3, ic = tx0 + (lane & 7);
@@ -329,31 +386,34 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   }
   const int wi = is * WIN + (ic - wx0);
   bool g_dirty = false, i_dirty = false;
-  int sweep = 0;
   bool more = true;
-  for (; sweep < max_sweeps; sweep++) {
-    Best<double> bg, bi;
-    bg.T = g_t0; bg.gx = g_gx0; bg.gz = g_gz0; bg.susp = 0; bg.reg_set = 0;
-    bi.T = i_t0; bi.gx = i_gx0; bi.gz = i_gz0; bi.susp = 0; bi.reg_set = 0;
-    if (g_act) update_grid_node<double>(v, gi, gj, bg);
-    if (i_act) update_iface_node<double>(v, is, ic, bi);
-    __syncthreads();
+  // Gauss-Seidel sweeps in five phases: the four parity colours of the grid
+  // nodes (a node's x/z neighbours and cell diagonals have other colours),
+  // then the interface nodes
+  for (int sweep = 0; sweep < max_sweeps; sweep++) {
     bool ch = false;
-    // bitwise comparison: NaN gradients (which the reference produces in
-    // some stages) must not count as a change on every sweep
-    if (g_act) {
-      if (__double_as_longlong(bg.T) != __double_as_longlong(L.T[wg]) ||
-          __double_as_longlong(bg.gx) != __double_as_longlong(L.GX[wg]) ||
-          __double_as_longlong(bg.gz) != __double_as_longlong(L.GZ[wg])) {
-        L.T[wg] = bg.T; L.GX[wg] = bg.gx; L.GZ[wg] = bg.gz; ch = true; g_dirty = true;
+    for (int phase = 0; phase < 5; phase++) {
+      if (phase < 4 && g_act && g_color == phase) {
+        Best<double> b;
+        b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0;
+        update_grid_node<double>(v, gi, gj, b);
+        if (__double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
+            __double_as_longlong(b.gx) != __double_as_longlong(L.GX[wg]) ||
+            __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[wg])) {
+          L.T[wg] = b.T; L.GX[wg] = b.gx; L.GZ[wg] = b.gz; ch = true; g_dirty = true;
+        }
       }
-    }
-    if (i_act) {
-      if (__double_as_longlong(bi.T) != __double_as_longlong(L.iT[wi]) ||
-          __double_as_longlong(bi.gx) != __double_as_longlong(L.iGX[wi]) ||
-          __double_as_longlong(bi.gz) != __double_as_longlong(L.iGZ[wi])) {
-        L.iT[wi] = bi.T; L.iGX[wi] = bi.gx; L.iGZ[wi] = bi.gz; ch = true; i_dirty = true;
+      if (phase == 4 && i_act) {
+        Best<double> b;
+        b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0;
+        update_iface_node<double>(v, is, ic, b);
+        if (__double_as_longlong(b.T) != __double_as_longlong(L.iT[wi]) ||
+            __double_as_longlong(b.gx) != __double_as_longlong(L.iGX[wi]) ||
+            __double_as_longlong(b.gz) != __double_as_longlong(L.iGZ[wi])) {
+          L.iT[wi] = b.T; L.iGX[wi] = b.gx; L.iGZ[wi] = b.gz; ch = true; i_dirty = true;
+        }
       }
+      __syncthreads();
     }
     more = __syncthreads_or(ch);
     if (!more) break;
@@ -371,7 +431,6 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   int li = -1, lj = -1;
   if (g_dirty) { li = gi - tx0; lj = gj - tz0; }
   const int li2 = i_dirty ? ic - tx0 : -1, lj2 = i_dirty ? i_cc - tz0 : -1;
-  unsigned mask = 0;
   auto edge = [&](int a, int b) {
     unsigned m = 0;
     if (a < 0) return m;
@@ -380,17 +439,23 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
     if (L0 && T0) m |= 16; if (R0 && T0) m |= 32; if (L0 && B0) m |= 64; if (R0 && B0) m |= 128;
     return m;
   };
-  mask = edge(li, lj) | edge(li2, lj2);
+  unsigned mask = edge(li, lj) | edge(li2, lj2);
   if (more) mask |= 256;  // sweep cap reached: run this tile again
-  // wave-wide OR of the masks
-  for (int off = 32; off > 0; off >>= 1) mask |= (unsigned)__shfl_xor((int)mask, off, 64);
+  // activation key: earliest time among this tile's changed nodes
+  double kmin = 1.0e300;
+  if (g_dirty) kmin = L.T[wg];
+  if (i_dirty) kmin = fmin(kmin, L.iT[wi]);
+  for (int off = 32; off > 0; off >>= 1) {
+    mask |= (unsigned)__shfl_xor((int)mask, off, 64);
+    kmin = fmin(kmin, __shfl_xor(kmin, off, 64));
+  }
   if (lane < 9 && (mask & (1u << lane))) {
     const int tix = t % ntx, tiz = t / ntx;
     const int dxs[9] = {-1, 1, 0, 0, -1, 1, -1, 1, 0};
     const int dzs[9] = {0, 0, -1, 1, -1, -1, 1, 1, 0};
     const int nxi = tix + dxs[lane], nzi = tiz + dzs[lane];
     if (nxi >= 0 && nxi < ntx && nzi >= 0 && nzi < ntz)
-      activate(f, nzi * ntx + nxi, ntiles, flags_next, list_next, count_next);
+      add_tile(f, nzi * ntx + nxi, ntiles, key_of(kmin), in_set, tkey, list_next, count_next);
   }
 }
 
@@ -559,18 +624,29 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
 }
 
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
-                             uint8_t* flags_next, int2* list_next, int* count_next, hipStream_t st) {
+                             unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
+                             hipStream_t st) {
   const int total = nfield * nx;
   hipLaunchKernelGGL(seed_tiles_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, nfield, ntx, ntz,
-                     flags_next, list_next, count_next);
+                     in_set, tkey, list_next, count_next);
   return hipGetLastError();
 }
 
-hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nlist, int ntx, int ntz,
-                      uint8_t* flags_cur, uint8_t* flags_next, int2* list_next, int* count_next, int max_sweeps,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(fim_tiles_kernel, dim3(nlist), dim3(64), 0, st, descs, regs, list, ntx, ntz, flags_cur,
-                     flags_next, list_next, count_next, max_sweeps);
+hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int ntiles, unsigned long long* tkey,
+                           unsigned long long* front, double window, int color, unsigned int* in_set, int2* due,
+                           int* count_due, int2* carry, int* count_carry, hipStream_t st) {
+  hipLaunchKernelGGL(reset_front_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, front, nfield);
+  hipLaunchKernelGGL(front_kernel, dim3((n + 255) / 256), dim3(256), 0, st, list, n, ntiles, tkey, front);
+  hipLaunchKernelGGL(split_kernel, dim3((n + 255) / 256), dim3(256), 0, st, list, n, ntx, ntiles, tkey, front, window,
+                     color, in_set, due, count_due, carry, count_carry);
+  return hipGetLastError();
+}
+
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
+                      int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
+                      int* count_next, int max_sweeps, hipStream_t st) {
+  hipLaunchKernelGGL(fim_tiles_kernel, dim3(nmax), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
+                     tkey, list_next, count_next, max_sweeps);
   return hipGetLastError();
 }
 

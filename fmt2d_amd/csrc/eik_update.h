@@ -49,7 +49,10 @@ enum { CK_REGULAR = 1, CK_POINT = 2 };
 template <typename Real>
 struct MinVisitor {       // fixed-point iteration: keep the minimum
   Best<Real> b;
-  FMT_HD bool want(Real) const { return true; }
+  // every local update is >= its evaluation time (the clamp to the largest
+  // member time, T1 + dist/v, the acceptance atime >= T(node1)), so an update
+  // evaluated no earlier than the current minimum cannot lower it
+  FMT_HD bool want(Real evt) const { return evt < b.T; }
   FMT_HD void add(Real T, Real gx, Real gz, int kind, int susp) {
     if (T < b.T) { b.T = T; b.gx = gx; b.gz = gz; b.reg_set = (kind == CK_REGULAR); if (kind == CK_REGULAR) b.susp = susp; }
   }

@@ -28,7 +28,8 @@ class GridInfo(C.Structure):
 
 class ForwardStats(C.Structure):
     _fields_ = [("nsrc", C.c_int), ("nfield", C.c_int), ("launches", C.c_int),
-                ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double)]
+                ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double),
+                ("ms_fim", C.c_double)]
 
 
 class Fmt2dError(RuntimeError):
@@ -54,6 +55,7 @@ def load() -> C.CDLL:
     lib.fmt2d_forward.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_forward_stats_get.argtypes = [C.c_void_p, C.POINTER(ForwardStats)]
     lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
+    lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int]
     lib.fmt2d_tf_region.argtypes = [C.c_int]
     lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     _LIB = lib
@@ -63,6 +65,7 @@ def load() -> C.CDLL:
 # exported symbols declared in include/fmt2d_hip.h
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
+           "fmt2d_set_tuning",
            "fmt2d_tf_region",
            "fmt2d_get_field")
 
@@ -110,6 +113,9 @@ class Context:
     def forward(self, src_x: Sequence[float], headwave: bool = True):
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
+
+    def set_tuning(self, window_tiles: float = 2.0, max_sweeps: int = 64):
+        _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps)))
 
     def stats(self) -> ForwardStats:
         st = ForwardStats()

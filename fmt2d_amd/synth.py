@@ -193,10 +193,10 @@ def plain_pg_case(dx: float = 0.5):
 
 
 def multistage_case(dx: float = 0.1, nobs: int = 60):
-    """The full multistage forward: 0.1 km grid (2001x301), ``nobs`` OBS,
-    1000 shots, the five phases with the reference's offset windows."""
+    """The full multistage forward: 0.1 km grid (2001x301), ``nobs`` OBS every
+    3.3 km, 1000 shots every 0.2 km at 10 m depth."""
     m = make_model(dx=dx)
-    obs = 2.0 + 3.3 * np.arange(nobs)
-    shots = np.round(np.arange(1, 2000) * 0.1, 6)[:1000]
+    obs = np.round(2.0 + 3.3 * np.arange(nobs), 6)
+    shots = np.round(0.1 + 0.2 * np.arange(1000), 6)
     g = make_geometry(obs, shots, m, ray_types=(1,))
     return m, g

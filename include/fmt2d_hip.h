@@ -51,7 +51,8 @@ typedef struct {
   int launches;             /* tile-solver launches over all stage levels */
   long long tiles;          /* tiles processed over all launches */
   long long region_nodes;   /* sum of region node counts over the fields */
-  double ms_total;          /* wall time of the forward on the device stream */
+  double ms_total;          /* wall time of the forward */
+  double ms_fim;            /* summed device time of the tile-solver launches (HIP events) */
 } fmt2d_forward_stats;
 
 /* Context on HIP device `device` (owns all device memory). */
@@ -71,7 +72,10 @@ int fmt2d_grid_info_get(fmt2d_ctx* ctx, fmt2d_grid_info* info);
  * resident on the device.  flags: bit0 = skip the PmP head-wave post-pass. */
 int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags);
 int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
-/* Active-tile count of every solver launch of the last forward (diagnostics). */
+/* Solver tuning: arrival-time window of the tile schedule in tile crossing
+ * times (at 4 km/s) and the sweep cap of one tile visit.  Defaults 2.0, 64. */
+int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps);
+/* Due-tile count of every solver launch of the last forward (diagnostics). */
 int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int max, int* n);
 
 /* Region id (1..6) of time field itf (1..12). */

@@ -11,7 +11,7 @@
 ! by oracle/build_ref.sh and never ships with the product.
 !
 ! Input (ref_fields.in, one item per line):
-!   smesh file, Moho file, sediment-base file, output file,
+!   smesh file, Moho file, sediment-base file, output file ("none": no dump),
 !   number of sources, then one source x per line,
 !   number of stages to run per source (1 = tf1 only, 12 = all).
 ! Output (stream binary, little endian):
@@ -47,6 +47,8 @@ program ref_fields
   type(Ttime_field2d), target :: tfs(12)
   logical :: reflect
   integer :: m_top, m_below
+  integer(8) :: c0, c1, crate
+  logical :: dump
 
   open(10, file='ref_fields.in', status='old', action='read')
   read(10, '(A)') f_smesh
@@ -162,8 +164,10 @@ program ref_fields
   intersection(4)%regright => region(4)
   intersection(5)%regright => null()
 
-  ! ---- geometry dump
-  open(newunit=u, file=trim(f_out), access='stream', form='unformatted', status='replace')
+  ! ---- geometry dump (output file "none": timing run without dumps)
+  dump = trim(f_out) /= 'none'
+  if (dump) open(newunit=u, file=trim(f_out), access='stream', form='unformatted', status='replace')
+  if (dump) then
   write(u) pgrid%nx, pgrid%nz, pgrid%dx, pgrid%x0, pgrid%z0
   write(u) ((merge(1, 0, pgrid%fully_regular(ix, iz)), ix=1,pgrid%nx), iz=1,pgrid%nz)
   write(u) ((pgrid%node_region(ix, iz), ix=1,pgrid%nx), iz=1,pgrid%nz)
@@ -180,6 +184,7 @@ program ref_fields
     write(u) intersection(n)%x, intersection(n)%z, intersection(n)%normal, &
              intersection(n)%vel_top(VTYPE_P, :), intersection(n)%vel_bot(VTYPE_P, :)
   end do
+  endif
 
   ! ---- per-source stage schedule (run_forward2d, without ray tracing)
   do n = 1, 12
@@ -191,6 +196,7 @@ program ref_fields
     allocate(tfs(n)%arrivaltime(region(tf_region(n))%nnode))
     allocate(tfs(n)%time_gradient(2, region(tf_region(n))%nnode))
   end do
+  call system_clock(c0, crate)
   do is = 1, nsrc
     sx = src_x(is)
     sz = iface_z_at(intrface(2), sx)
@@ -243,12 +249,16 @@ program ref_fields
         endif
         tf%arrivaltime = reg%arrivaltime
         tf%time_gradient = reg%time_gradient
-        write(u) itf, reg%id
-        write(u) reg%arrivaltime(1:reg%nnode), reg%time_gradient(1:2, 1:reg%nnode)
+        if (dump) then
+          write(u) itf, reg%id
+          write(u) reg%arrivaltime(1:reg%nnode), reg%time_gradient(1:2, 1:reg%nnode)
+        endif
       end do
     end do
   end do
-  close(u)
+  call system_clock(c1)
+  if (dump) close(u)
+  write(*,'(A,F12.6)') 'stage loop seconds: ', real(c1 - c0, dp) / real(crate, dp)
 
 contains
 
