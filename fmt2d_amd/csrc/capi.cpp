@@ -91,7 +91,12 @@ struct fmt2d_ctx {
   double window_tiles = 2.0;          // window in tile crossing times
   int max_sweeps = 64;
   DBuf<int> sel_which, sel_up;
-  DBuf<double> hw_scratch, texit;
+  DBuf<double> hw_scratch;
+  // heap-exhaustion stop work arrays
+  DBuf<ExitField> x_ef;
+  DBuf<int> x_cnt, x_diff, x_off, x_total;
+  DBuf<unsigned long long> x_max, x_m;
+  DBuf<ExitEntry> x_ent;
   int* h_count = nullptr;              // pinned
   std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
   fmt2d_forward_stats stats{};
@@ -108,46 +113,6 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
   hipError_t e = b.alloc(v.size());
   if (e != hipSuccess) return e;
   return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
-}
-
-// Exit time of one field (see heap_exit.h): seeds are the nodes the
-// reference puts in the heap before marching -- the point-source cell corners
-// and the diagonal neighbours of an on-node source (:2337-2383), or every
-// start-interface node and the grid nodes the sweep seeds (:2141-2222).
-static double field_exit_time(const HostModel& m, const FieldDesc& d, const double* T, const double* maxnb) {
-  const int nx = m.nx, nz = m.nz;
-  const HostRegion& R = m.reg[d.region];
-  std::vector<double> tt, mx;
-  std::vector<char> sd, fx;
-  const int fi = d.fixed_key >= 0 ? d.fixed_key % nx : -10, fj = d.fixed_key >= 0 ? d.fixed_key / nx : -10;
-  const HostInterface* side_if = nullptr;
-  if (d.kind == SEED_IFACE) side_if = &m.isec[d.side == 0 ? R.itop : R.ibot];
-  for (int j = 0; j < nz + 2; j++)
-    for (int i = 0; i < nx; i++) {
-      const int q = j * nx + i;
-      bool seed = false;
-      if (j < nz) {
-        if (!(R.flags[q] & RF_IN)) continue;
-        if (d.kind == SEED_POINT) {
-          seed = (i >= d.ix0 && i <= d.ix0 + 1 && j >= d.iz0 && j <= d.iz0 + 1) ||
-                 (abs(i - fi) == 1 && abs(j - fj) == 1);
-        } else {
-          for (int n = i - 1; n <= i + 1 && !seed; n++) {
-            if (n < 0 || n >= nx) continue;
-            const int cc = side_if->cc[n];
-            if (cc < 0) continue;
-            seed = (d.side == 1) ? (j <= cc && j >= cc - 2) : (j >= cc + 1 && j <= cc + 3);
-          }
-        }
-      } else {
-        seed = d.kind == SEED_IFACE && (j - nz) == d.side;
-      }
-      tt.push_back(T[q]);
-      mx.push_back(maxnb[q]);
-      sd.push_back(seed);
-      fx.push_back(q == d.fixed_key);
-    }
-  return heap_exit_time(tt, mx, sd, fx);
 }
 
 extern "C" {
@@ -452,19 +417,32 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       launches++;
     }
     ctx->stats.launches += launches;
-    // heap-exhaustion stop of the reference (propagate2d.f90:80)
+    // heap-exhaustion stop of the reference (src/propagate2d.f90:74-80)
     {
       HIPCHK(launch_member_time(dl, ctx->regs.p, nf, nx, nz, st));
-      std::vector<double> hT((size_t)nf * nslot), hM((size_t)nf * nslot), te(nf);
-      HIPCHK(hipMemcpyAsync(hT.data(), ctx->fT.p + (size_t)f0 * nslot, hT.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(hM.data(), ctx->fOGX.p + (size_t)f0 * nslot, hM.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+      const size_t nb = (size_t)nf * EXIT_NBIN;
+      if (ctx->x_ef.n < (size_t)nf) HIPCHK(ctx->x_ef.alloc(nf));
+      if (ctx->x_cnt.n < nb) {
+        HIPCHK(ctx->x_cnt.alloc(nb));
+        HIPCHK(ctx->x_diff.alloc(nb));
+        HIPCHK(ctx->x_off.alloc(nb));
+        HIPCHK(ctx->x_max.alloc(nb));
+        HIPCHK(ctx->x_m.alloc(nb));
+      }
+      if (!ctx->x_total.p) HIPCHK(ctx->x_total.alloc(1));
+      ExitWork w{ctx->x_ef.p, ctx->x_cnt.p, ctx->x_diff.p, ctx->x_off.p, ctx->x_max.p, ctx->x_m.p,
+                 ctx->x_total.p, ctx->x_ent.p, (long long)ctx->x_ent.n};
+      HIPCHK(launch_exit_bins(dl, ctx->regs.p, nf, (int)nslot, w, st));
+      HIPCHK(hipMemcpyAsync(ctx->h_count + 2, ctx->x_total.p, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      for (int k = 0; k < nf; k++) te[k] = field_exit_time(m, fd[f0 + k], hT.data() + (size_t)k * nslot,
-                                                           hM.data() + (size_t)k * nslot);
-      HIPCHK(upload(ctx->texit, te, st));
-      HIPCHK(launch_truncate(dl, ctx->texit.p, nf, (int)nslot, st));
+      const long long ncand = ctx->h_count[2];
+      if ((long long)ctx->x_ent.n < ncand) {
+        HIPCHK(ctx->x_ent.alloc((size_t)ncand + ncand / 4));
+        w.ent = ctx->x_ent.p;
+        w.cap = (long long)ctx->x_ent.n;
+      }
+      HIPCHK(launch_exit_check(dl, ctx->regs.p, nf, (int)nslot, w, ncand, st));
+      ctx->stats.exit_nodes += ncand;
     }
     HIPCHK(launch_refit(dl, ctx->regs.p, nf, nx, nz, st));
     // water-layer top-row fixes and the PmP head-wave post-pass

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "eik_common.h"
+#include "heap_exit.h"
 
 namespace fmt2d {
 
@@ -67,7 +68,26 @@ hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2*
                       int* count_next, int max_sweeps, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
-hipError_t launch_truncate(const FieldDesc* descs, const double* texit, int nfield, int nslot, hipStream_t st);
+// device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays
+// hold nfield*EXIT_NBIN entries, `ent` holds `cap` entries
+struct ExitWork {
+  ExitField* ef;
+  int* bcnt;
+  int* bdiff;
+  int* boff;
+  unsigned long long* bmax;
+  unsigned long long* bm;
+  int* total;
+  ExitEntry* ent;
+  long long cap;
+};
+// init + stats + bins + scan; afterwards *total holds the number of nodes
+// in open bins
+hipError_t launch_exit_bins(const FieldDesc* descs, const RegionDev* regs, int nfield, int nslot, ExitWork& w,
+                            hipStream_t st);
+// gather + exact check (needs cap >= *total) + truncation of the fields
+hipError_t launch_exit_check(const FieldDesc* descs, const RegionDev* regs, int nfield, int nslot, ExitWork& w,
+                             long long ncand_bound, hipStream_t st);
 hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, hipStream_t st);
 hipError_t launch_water_fix(const FieldDesc* descs, const RegionDev* regs, const int* which, const int* up, int nsel,
                             int nx, hipStream_t st);

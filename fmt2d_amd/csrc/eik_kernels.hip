@@ -577,17 +577,255 @@ __global__ void member_time_kernel(const FieldDesc* __restrict__ descs, const Re
   }
 }
 
+// ---------------------------------------------------------------- heap-exhaustion stop
+// Binned evaluation of the stop rule stated in heap_exit.h.  Every kernel
+// below runs on a grid (ceil(nslot/256), nfield) so a block never straddles
+// two fields, or one block per field for the bin scan.
+
+// Node q of field d for the stop rule: false when q is no node of the
+// region.  `seed` marks the nodes the reference puts in its heap before
+// marching: the point-source cell corners and the diagonal neighbours of an
+// on-node source (:2337-2383), or the start-interface nodes and the grid
+// nodes 1-3 rows into the region that its sweep seeds (:2141-2222).
+__device__ inline bool exit_node(const FieldDesc& d, const RegionDev& R, int q, bool& seed, bool& fixed) {
+  const int nx = R.nx, nz = R.nz, i = q % nx, j = q / nx;
+  seed = false;
+  fixed = q == d.fixed_key;
+  if (j < nz) {
+    if (!(R.flags[q] & RF_IN)) return false;
+    if (d.kind == SEED_POINT) {
+      const int fi = d.fixed_key >= 0 ? d.fixed_key % nx : -10, fj = d.fixed_key >= 0 ? d.fixed_key / nx : -10;
+      seed = (i >= d.ix0 && i <= d.ix0 + 1 && j >= d.iz0 && j <= d.iz0 + 1) || (abs(i - fi) == 1 && abs(j - fj) == 1);
+    } else {
+      const int* cc = R.ifcc[d.side];
+      for (int n = i - 1; n <= i + 1 && !seed; n++) {
+        if (n < 0 || n >= nx) continue;
+        const int c = cc[n];
+        if (c < 0) continue;
+        seed = (d.side == 1) ? (j <= c && j >= c - 2) : (j >= c + 1 && j <= c + 3);
+      }
+    }
+  } else {
+    seed = d.kind == SEED_IFACE && (j - nz) == d.side;
+  }
+  return true;
+}
+
+__global__ void exit_init_kernel(ExitField* ef, int nfield, int* bcnt, int* bdiff, unsigned long long* bmax,
+                                 unsigned long long* bm, int* total) {
+  const long long nb = (long long)nfield * EXIT_NBIN;
+  const unsigned long long kneg = ord_key(-INFINITY);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (long long)gridDim.x * blockDim.x) {
+    bcnt[k] = 0; bdiff[k] = 0; bmax[k] = kneg; bm[k] = kneg;
+    if (k < nfield) {
+      ExitField e;
+      e.lo = ~0ull; e.hi = 0ull; e.smax = kneg; e.texit = ord_key(INFINITY);
+      e.count = 0; e.bad = 0; e.ncand = 0; e.pad = 0;
+      ef[k] = e;
+    }
+    if (k == 0) *total = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) exit_stats_kernel(const FieldDesc* __restrict__ descs,
+                                                         const RegionDev* __restrict__ regs, int nslot, ExitField* ef) {
+  __shared__ unsigned long long s_lo[256], s_hi[256], s_sm[256];
+  __shared__ int s_cnt[256], s_bad[256];
+  const int f = blockIdx.y, q = blockIdx.x * 256 + threadIdx.x;
+  const FieldDesc& d = descs[f];
+  unsigned long long lo = ~0ull, hi = 0ull, sm = ord_key(-INFINITY);
+  int cnt = 0, bad = 0;
+  bool seed, fixed;
+  if (q < nslot && exit_node(d, regs[d.region], q, seed, fixed) && !fixed) {
+    const double t = d.T[q];
+    const bool listed = t < FMT_HUGE_HALF;
+    if (seed) {
+      if (!listed) bad = 1;
+      else sm = ord_key(t);
+    }
+    if (listed) { lo = hi = ord_key(t); cnt = 1; }
+  }
+  const int x = threadIdx.x;
+  s_lo[x] = lo; s_hi[x] = hi; s_sm[x] = sm; s_cnt[x] = cnt; s_bad[x] = bad;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (x < o) {
+      s_lo[x] = min(s_lo[x], s_lo[x + o]);
+      s_hi[x] = max(s_hi[x], s_hi[x + o]);
+      s_sm[x] = max(s_sm[x], s_sm[x + o]);
+      s_cnt[x] += s_cnt[x + o];
+      s_bad[x] |= s_bad[x + o];
+    }
+    __syncthreads();
+  }
+  if (x == 0 && s_cnt[0] + s_bad[0] > 0) {
+    atomicMin(&ef[f].lo, s_lo[0]);
+    atomicMax(&ef[f].hi, s_hi[0]);
+    atomicMax(&ef[f].smax, s_sm[0]);
+    atomicAdd(&ef[f].count, s_cnt[0]);
+    if (s_bad[0]) atomicOr(&ef[f].bad, 1);
+  }
+}
+
+// per-bin node count, latest time, latest connection, and the difference
+// array of the bins one interval (T_i, maxnb_i) covers completely
+__global__ void __launch_bounds__(256) exit_bins_kernel(const FieldDesc* __restrict__ descs,
+                                                        const RegionDev* __restrict__ regs, int nslot,
+                                                        const ExitField* __restrict__ ef, int* bcnt, int* bdiff,
+                                                        unsigned long long* bmax, unsigned long long* bm) {
+  const int f = blockIdx.y, q = blockIdx.x * 256 + threadIdx.x;
+  const ExitField e = ef[f];
+  if (e.bad || e.count < 3 || q >= nslot) return;
+  const FieldDesc& d = descs[f];
+  bool seed, fixed;
+  if (!exit_node(d, regs[d.region], q, seed, fixed) || fixed) return;
+  const double t = d.T[q];
+  if (!(t < FMT_HUGE_HALF)) return;
+  const double m = d.OGX[q];
+  const double lo = ord_val(e.lo), hi = ord_val(e.hi), sc = exit_scale(lo, hi);
+  const int b = exit_bin(t, lo, sc);
+  const size_t base = (size_t)f * EXIT_NBIN;
+  atomicAdd(&bcnt[base + b], 1);
+  atomicMax(&bmax[base + b], ord_key(t));
+  atomicMax(&bm[base + b], ord_key(m));
+  const int end = exit_cover_end(m, lo, hi, sc);
+  if (end > b + 1) {
+    atomicAdd(&bdiff[base + b + 1], 1);
+    if (end < EXIT_NBIN) atomicAdd(&bdiff[base + end], -1);
+  }
+}
+
+// One block per field: running cover count and exclusive prefix maximum of
+// the per-bin latest connection; open bins get a slot range in the entry
+// list (bdiff becomes the gather fill counter, bm the prefix maximum).
+__global__ void __launch_bounds__(256) exit_scan_kernel(ExitField* ef, int* bcnt, int* bdiff,
+                                                        const unsigned long long* __restrict__ bmax,
+                                                        unsigned long long* bm, int* boff, int* total) {
+  constexpr int PER = EXIT_NBIN / 256;
+  __shared__ int s_a[256], s_c[256];
+  __shared__ double s_m[256];
+  __shared__ int s_base;
+  const int f = blockIdx.x, x = threadIdx.x;
+  const size_t base = (size_t)f * EXIT_NBIN + (size_t)x * PER;
+  const ExitField e = ef[f];
+  const bool skip = e.bad || e.count < 3;
+  const double smax = ord_val(e.smax);
+  // pass 1: per-thread sums of the difference array and max of connections
+  int dsum = 0;
+  double mmax = -INFINITY;
+  for (int k = 0; k < PER; k++) {
+    dsum += bdiff[base + k];
+    mmax = fmax(mmax, ord_val(bm[base + k]));
+  }
+  s_a[x] = dsum; s_m[x] = mmax;
+  __syncthreads();
+  // inclusive scans over the 256 thread totals
+  for (int o = 1; o < 256; o <<= 1) {
+    const int a = x >= o ? s_a[x - o] : 0;
+    const double m = x >= o ? s_m[x - o] : -INFINITY;
+    __syncthreads();
+    s_a[x] += a; s_m[x] = fmax(s_m[x], m);
+    __syncthreads();
+  }
+  int cover = x > 0 ? s_a[x - 1] : 0;
+  double run = x > 0 ? s_m[x - 1] : -INFINITY;
+  __syncthreads();
+  // pass 2: open bins and their node counts
+  int ncand = 0;
+  for (int k = 0; k < PER; k++) {
+    cover += bdiff[base + k];
+    const double bmk = ord_val(bm[base + k]);
+    const bool open = !skip && exit_bin_open(bcnt[base + k], cover, run, ord_val(bmax[base + k]), smax);
+    bm[base + k] = ord_key(run);
+    bdiff[base + k] = 0;
+    if (open) ncand += bcnt[base + k];
+    else bcnt[base + k] = 0;  // closed bin: no slot range
+    run = fmax(run, bmk);
+  }
+  s_c[x] = ncand;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int a = x >= o ? s_c[x - o] : 0;
+    __syncthreads();
+    s_c[x] += a;
+    __syncthreads();
+  }
+  if (x == 255) {
+    s_base = s_c[255] > 0 ? atomicAdd(total, s_c[255]) : 0;
+    ef[f].ncand = s_c[255];
+  }
+  __syncthreads();
+  int off = s_base + (x > 0 ? s_c[x - 1] : 0);
+  for (int k = 0; k < PER; k++) {
+    const int c = bcnt[base + k];
+    if (c > 0) { boff[base + k] = off; off += c; }
+    else boff[base + k] = -1;
+  }
+}
+
+__global__ void __launch_bounds__(256) exit_gather_kernel(const FieldDesc* __restrict__ descs,
+                                                          const RegionDev* __restrict__ regs, int nslot,
+                                                          const ExitField* __restrict__ ef,
+                                                          const int* __restrict__ boff, int* bfill, ExitEntry* ent,
+                                                          long long cap) {
+  const int f = blockIdx.y, q = blockIdx.x * 256 + threadIdx.x;
+  const ExitField e = ef[f];
+  if (e.ncand == 0 || q >= nslot) return;
+  const FieldDesc& d = descs[f];
+  bool seed, fixed;
+  if (!exit_node(d, regs[d.region], q, seed, fixed) || fixed) return;
+  const double t = d.T[q];
+  if (!(t < FMT_HUGE_HALF)) return;
+  const double lo = ord_val(e.lo), hi = ord_val(e.hi), sc = exit_scale(lo, hi);
+  const int b = exit_bin(t, lo, sc);
+  const size_t bb = (size_t)f * EXIT_NBIN + b;
+  const int o = boff[bb];
+  if (o < 0) return;
+  const long long p = (long long)o + atomicAdd(&bfill[bb], 1);
+  if (p >= cap) return;
+  ExitEntry en;
+  en.t = t; en.m = d.OGX[q]; en.f = f; en.b = b;
+  ent[p] = en;
+}
+
+// exact check of every node of an open bin against the nodes of its bin
+__global__ void __launch_bounds__(256) exit_check_kernel(const ExitEntry* __restrict__ ent,
+                                                         const int* __restrict__ total, long long cap,
+                                                         const int* __restrict__ bcnt, const int* __restrict__ boff,
+                                                         const unsigned long long* __restrict__ bm, ExitField* ef) {
+  const long long n = min((long long)*total, cap);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+    const ExitEntry c = ent[k];
+    const ExitField e = ef[c.f];
+    const double t = c.t;
+    if (!(t >= ord_val(e.smax) && t > ord_val(e.lo) && t < ord_val(e.hi))) continue;
+    const size_t bb = (size_t)c.f * EXIT_NBIN + c.b;
+    const long long j0 = boff[bb], j1 = min(j0 + bcnt[bb], cap);
+    double mx = ord_val(bm[bb]);
+    bool dup = false;
+    for (long long j = j0; j < j1 && !dup; j++) {
+      if (j == k) continue;
+      const double tj = ent[j].t;
+      dup = tj == t;
+      if (tj < t) mx = fmax(mx, ent[j].m);
+    }
+    if (!dup && mx <= t) atomicMin(&ef[c.f].texit, ord_key(t));
+  }
+}
+
 // Nodes the reference never reaches once its heap ran empty (times above the
-// exit time of their field) go back to huge_time with a zero gradient.
-__global__ void truncate_kernel(const FieldDesc* __restrict__ descs, const double* __restrict__ texit, int nfield,
+// stop time of their field) go back to huge_time with a zero gradient.
+__global__ void truncate_kernel(const FieldDesc* __restrict__ descs, const ExitField* __restrict__ ef, int nfield,
                                 int nslot) {
   const long long total = (long long)nfield * nslot;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (long long)gridDim.x * blockDim.x) {
     const int f = (int)(k / nslot), q = (int)(k % nslot);
+    const double te = ord_val(ef[f].texit);
+    if (te == INFINITY) continue;
     const FieldDesc& d = descs[f];
     const double t = d.T[q];
-    if (t > texit[f] && t < 0.5e20) { d.T[q] = FMT_HUGE_TIME; d.GX[q] = 0.0; d.GZ[q] = 0.0; }
+    if (t > te && t < FMT_HUGE_HALF) { d.T[q] = FMT_HUGE_TIME; d.GX[q] = 0.0; d.GZ[q] = 0.0; }
   }
 }
 
@@ -601,10 +839,31 @@ hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int
   return hipGetLastError();
 }
 
-hipError_t launch_truncate(const FieldDesc* descs, const double* texit, int nfield, int nslot, hipStream_t st) {
+hipError_t launch_exit_bins(const FieldDesc* descs, const RegionDev* regs, int nfield, int nslot, ExitWork& w,
+                            hipStream_t st) {
+  const long long nb = (long long)nfield * EXIT_NBIN;
+  hipLaunchKernelGGL(exit_init_kernel, dim3((unsigned)std::min<long long>((nb + 255) / 256, 65536)), dim3(256), 0, st,
+                     w.ef, nfield, w.bcnt, w.bdiff, w.bmax, w.bm, w.total);
+  const dim3 g((nslot + 255) / 256, nfield);
+  hipLaunchKernelGGL(exit_stats_kernel, g, dim3(256), 0, st, descs, regs, nslot, w.ef);
+  hipLaunchKernelGGL(exit_bins_kernel, g, dim3(256), 0, st, descs, regs, nslot, w.ef, w.bcnt, w.bdiff, w.bmax, w.bm);
+  hipLaunchKernelGGL(exit_scan_kernel, dim3(nfield), dim3(256), 0, st, w.ef, w.bcnt, w.bdiff, w.bmax, w.bm, w.boff,
+                     w.total);
+  return hipGetLastError();
+}
+
+hipError_t launch_exit_check(const FieldDesc* descs, const RegionDev* regs, int nfield, int nslot, ExitWork& w,
+                             long long ncand_bound, hipStream_t st) {
+  const dim3 g((nslot + 255) / 256, nfield);
+  hipLaunchKernelGGL(exit_gather_kernel, g, dim3(256), 0, st, descs, regs, nslot, w.ef, w.boff, w.bdiff, w.ent, w.cap);
+  if (ncand_bound > 0) {
+    const int blocks = (int)std::min<long long>((ncand_bound + 255) / 256, 16384);
+    hipLaunchKernelGGL(exit_check_kernel, dim3(blocks), dim3(256), 0, st, w.ent, w.total, w.cap, w.bcnt, w.boff, w.bm,
+                       w.ef);
+  }
   const long long total = (long long)nfield * nslot;
   int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(truncate_kernel, dim3(blocks), dim3(256), 0, st, descs, texit, nfield, nslot);
+  hipLaunchKernelGGL(truncate_kernel, dim3(blocks), dim3(256), 0, st, descs, w.ef, nfield, nslot);
   return hipGetLastError();
 }
 

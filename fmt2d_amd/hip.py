@@ -29,7 +29,7 @@ class GridInfo(C.Structure):
 class ForwardStats(C.Structure):
     _fields_ = [("nsrc", C.c_int), ("nfield", C.c_int), ("launches", C.c_int),
                 ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double),
-                ("ms_fim", C.c_double)]
+                ("ms_fim", C.c_double), ("exit_nodes", C.c_longlong)]
 
 
 class Fmt2dError(RuntimeError):

@@ -53,6 +53,7 @@ typedef struct {
   long long region_nodes;   /* sum of region node counts over the fields */
   double ms_total;          /* wall time of the forward */
   double ms_fim;            /* summed device time of the tile-solver launches (HIP events) */
+  long long exit_nodes;     /* nodes checked exactly by the heap-exhaustion stop */
 } fmt2d_forward_stats;
 
 /* Context on HIP device `device` (owns all device memory). */

@@ -13,7 +13,7 @@
 
 #include "../../fmt2d_amd/csrc/eik_update.h"
 #include "../../fmt2d_amd/csrc/model.h"
-#include "../../fmt2d_amd/csrc/heap_exit.h"
+#include "heap_exit_sorted.h"
 
 using namespace fmt2d;
 #ifndef GT
@@ -285,7 +285,7 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
         SD.push_back(sd);
         FX.push_back((int)g == F.fixed_key);
       }
-    const double te = heap_exit_time(TT, MX, SD, FX);
+    const double te = fmt2d_test::heap_exit_time(TT, MX, SD, FX);
     if (getenv("EMU_EXIT_DEBUG")) fprintf(stderr, "heap exit time %.9f\n", te);
     for (size_t k = 0; k < slot.size(); k++)
       if (TT[k] > te && TT[k] < FMT_HUGE_HALF) { F.T[slot[k]] = FMT_HUGE_TIME; F.GX[slot[k]] = 0; F.GZ[slot[k]] = 0; }
