@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -81,6 +82,7 @@ struct fmt2d_ctx {
   int nsrc = 0, nfield = 0;
   std::vector<int> field_of;           // [isrc*12 + itf-1] -> field index
   DBuf<double> fT, isT, fGX, fGZ, fOGX, fOGZ, isGX, isGZ;
+  DBuf<uint8_t> fTIE;
   DBuf<FieldDesc> descs;
   DBuf<StageLink> links;
   DBuf<unsigned int> tin;             // tile in active set
@@ -90,6 +92,8 @@ struct fmt2d_ctx {
   DBuf<int> tcount;                   // [due, next]
   double window_tiles = 2.0;          // window in tile crossing times
   int max_sweeps = 64;
+  int colors = 4;                     // 4: four-colour Gauss-Seidel in a tile, 1: Jacobi
+  DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;
   // heap-exhaustion stop work arrays
@@ -283,6 +287,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     HIPCHK(ctx->fGZ.alloc(nslot * nfield));
     HIPCHK(ctx->fOGX.alloc(nslot * nfield));
     HIPCHK(ctx->fOGZ.alloc(nslot * nfield));
+    HIPCHK(ctx->fTIE.alloc(nslot * nfield));
     HIPCHK(ctx->isT.alloc((size_t)nx * nfield));
     HIPCHK(ctx->isGX.alloc((size_t)nx * nfield));
     HIPCHK(ctx->isGZ.alloc((size_t)nx * nfield));
@@ -304,6 +309,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     d.GZ = ctx->fGZ.p + (size_t)f * nslot;
     d.OGX = ctx->fOGX.p + (size_t)f * nslot;
     d.OGZ = ctx->fOGZ.p + (size_t)f * nslot;
+    d.TIE = ctx->fTIE.p + (size_t)f * nslot;
     d.isT = ctx->isT.p + (size_t)f * nx;
     d.isGX = ctx->isGX.p + (size_t)f * nx;
     d.isGZ = ctx->isGZ.p + (size_t)f * nx;
@@ -366,6 +372,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
   if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
+  if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3));
+  HIPCHK(hipMemsetAsync(ctx->acc.p, 0, 3 * sizeof(unsigned long long), st));
   const int max_launch = 64 * (ntx + ntz) + 4000;
   const bool headwave = !(flags & 1);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
@@ -406,7 +414,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       }
       HIPCHK(hipEventRecord(ctx->ev[ei], st));
       HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, n, ctx->tcount.p, ntx, ntz, ctx->tin.p, ctx->tkey.p,
-                        ctx->tlist[nxt].p, ctx->tcount.p + 1, ctx->max_sweeps, st));
+                        ctx->tlist[nxt].p, ctx->tcount.p + 1, ctx->max_sweeps, ctx->colors, ctx->acc.p, st));
       HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
       HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -475,11 +483,18 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     fim_ms += ms;
   }
   ctx->stats.ms_fim = fim_ms;
+  unsigned long long acc[3];
+  HIPCHK(hipMemcpy(acc, ctx->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
+  ctx->stats.sweeps = (long long)acc[0];
+  ctx->stats.max_sweeps = (int)acc[1];
+  ctx->stats.capped = (long long)acc[2];
   return FMT2D_OK;
 }
 
-int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps) {
-  if (!ctx || !(window_tiles > 0.0) || max_sweeps < 1) return fail(FMT2D_EINVAL, "bad tuning parameters");
+int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors) {
+  if (!ctx || !(window_tiles > 0.0) || max_sweeps < 1 || (colors != 1 && colors != 4))
+    return fail(FMT2D_EINVAL, "bad tuning parameters");
+  ctx->colors = colors;
   ctx->window_tiles = window_tiles;
   ctx->max_sweeps = max_sweeps;
   return FMT2D_OK;
@@ -507,12 +522,15 @@ int fmt2d_get_field(fmt2d_ctx* ctx, int isrc, int itf, double* T, double* dTdx, 
   const size_t nslot = (size_t)(m.nz + 2) * m.nx;
   const int f = ctx->field_of[isrc * FMT2D_NTF + itf - 1];
   std::vector<double> t(nslot), gx(nslot), gz(nslot);
+  // FMT2D_RAW_GRADIENTS=1 (diagnostics): the marching gradients before the
+  // one-sided refit and the water-layer fixes
+  const bool raw = getenv("FMT2D_RAW_GRADIENTS") != nullptr;
   HIPCHK(hipMemcpyAsync(t.data(), ctx->fT.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
                         ctx->stream));
-  HIPCHK(hipMemcpyAsync(gx.data(), ctx->fOGX.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
-                        ctx->stream));
-  HIPCHK(hipMemcpyAsync(gz.data(), ctx->fOGZ.p + (size_t)f * nslot, nslot * sizeof(double), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(gx.data(), (raw ? ctx->fGX.p : ctx->fOGX.p) + (size_t)f * nslot, nslot * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(gz.data(), (raw ? ctx->fGZ.p : ctx->fOGZ.p) + (size_t)f * nslot, nslot * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const HostRegion& R = m.reg[TF_REGION[itf]];
   for (int k = 0; k < R.nnode; k++) {

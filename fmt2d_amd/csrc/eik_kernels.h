@@ -26,6 +26,7 @@ struct FieldDesc {
   double* GZ;
   double* OGX;       // gradient after the refit / fix post-passes (output)
   double* OGZ;
+  uint8_t* TIE;      // equal-time support mask of the grid nodes (TIE_* bits)
 };
 
 // Interface stage: where the start-interface values come from.
@@ -65,7 +66,7 @@ hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int nti
                            int* count_due, int2* carry, int* count_carry, hipStream_t st);
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, hipStream_t st);
+                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

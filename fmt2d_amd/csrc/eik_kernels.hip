@@ -82,6 +82,7 @@ struct GView {
   __device__ double T(int i, int j) const { return D->T[j * R->nx + i]; }
   __device__ double GX(int i, int j) const { return D->GX[j * R->nx + i]; }
   __device__ double GZ(int i, int j) const { return D->GZ[j * R->nx + i]; }
+  __device__ int tie(int i, int j) const { return D->TIE[j * R->nx + i]; }
   __device__ double vel(int i, int j) const { return R->vel[j * R->nx + i]; }
   __device__ double xg(int i) const { return R->xg[i]; }
   __device__ double zg(int j) const { return R->zg[j]; }
@@ -113,6 +114,7 @@ struct TileLds {
   double T[WIN2];
   double GX[WIN2], GZ[WIN2], vel[WIN2];
   uint16_t flags[WIN2];
+  uint8_t tie[WIN2];
   double xg[WIN], zg[WIN];
   double iT[2 * WIN];
   double iGX[2 * WIN], iGZ[2 * WIN], ivel[2 * WIN], ifx[2 * WIN], ifz[2 * WIN];
@@ -144,6 +146,7 @@ struct LView {
   __device__ double T(int i, int j) const { return L->T[w(i, j)]; }
   __device__ double GX(int i, int j) const { return L->GX[w(i, j)]; }
   __device__ double GZ(int i, int j) const { return L->GZ[w(i, j)]; }
+  __device__ int tie(int i, int j) const { return L->tie[w(i, j)]; }
   __device__ double vel(int i, int j) const { return L->vel[w(i, j)]; }
   __device__ double xg(int i) const { int a = i - wx0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->xg[a]; }
   __device__ double zg(int j) const { int a = j - wz0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->zg[a]; }
@@ -177,6 +180,7 @@ __global__ void fill_fields_kernel(const FieldDesc* __restrict__ descs, int nfie
     d.T[q] = FMT_HUGE_TIME;
     d.GX[q] = 0.0; d.GZ[q] = 0.0;
     d.OGX[q] = 0.0; d.OGZ[q] = 0.0;
+    d.TIE[q] = 0;
   }
 }
 
@@ -329,7 +333,8 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
                                                        const RegionDev* __restrict__ regs, const int2* __restrict__ list,
                                                        const int* __restrict__ count_due, int ntx, int ntz,
                                                        unsigned int* in_set, unsigned long long* tkey,
-                                                       int2* list_next, int* count_next, int max_sweeps) {
+                                                       int2* list_next, int* count_next, int max_sweeps,
+                                                       int colors, unsigned long long* acc) {
   __shared__ TileLds L;
   // the grid is sized for the whole active set; only the due tiles run
   if ((int)blockIdx.x >= *count_due) return;
@@ -347,9 +352,9 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
     if (gi >= 0 && gi < nx && gj >= 0 && gj < nz) {
       const int q = gj * nx + gi;
       L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
-      L.vel[k] = R.vel[q]; L.flags[k] = R.flags[q];
+      L.vel[k] = R.vel[q]; L.flags[k] = R.flags[q]; L.tie[k] = d.TIE[q];
     } else {
-      L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0;
+      L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0; L.tie[k] = 0;
     }
   }
   if (lane < WIN) { const int gi = wx0 + lane; L.xg[lane] = (gi >= 0 && gi < nx) ? R.xg[gi] : 0.0; }
@@ -379,33 +384,44 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   bool i_act = false;
   double i_t0 = FMT_HUGE_TIME, i_gx0 = 0.0, i_gz0 = 0.0;
   int i_cc = -1;
+  int i_home = -1;  // row the interface node is solved in
   if (lane < 16 && ic < nx - 1) {
     i_cc = L.ifcc[is * WIN + ic - wx0];
-    i_act = i_cc >= tz0 && i_cc < tz0 + TILE && i_cc >= 0;
+    if (i_cc >= tz0 - 1 && i_cc < tz0 + TILE) i_home = iface_home_row(v, is, ic, i_cc);
+    i_act = i_home >= tz0 && i_home < tz0 + TILE;
     if (i_act) seed_iface(d, is, ic, i_t0, i_gx0, i_gz0);
   }
   const int wi = is * WIN + (ic - wx0);
   bool g_dirty = false, i_dirty = false;
   bool more = true;
-  // Gauss-Seidel sweeps in five phases: the four parity colours of the grid
-  // nodes (a node's x/z neighbours and cell diagonals have other colours),
-  // then the interface nodes
-  for (int sweep = 0; sweep < max_sweeps; sweep++) {
+  // Sweeps.  colors == 4: Gauss-Seidel in five phases, the four parity
+  // colours of the grid nodes (a node's x/z neighbours and cell diagonals
+  // have other colours), then the interface nodes.  colors == 1: Jacobi, all
+  // grid nodes from the same state, then the interface nodes.
+  const int nphase = colors == 4 ? 5 : 2;
+  int sweep = 0;
+  for (; sweep < max_sweeps; sweep++) {
     bool ch = false;
-    for (int phase = 0; phase < 5; phase++) {
-      if (phase < 4 && g_act && g_color == phase) {
+    for (int phase = 0; phase < nphase; phase++) {
+      const bool grid_phase = phase < nphase - 1;
+      if (grid_phase) {
+        bool upd = false;
         Best<double> b;
-        b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0;
-        update_grid_node<double>(v, gi, gj, b);
-        if (__double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
-            __double_as_longlong(b.gx) != __double_as_longlong(L.GX[wg]) ||
-            __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[wg])) {
-          L.T[wg] = b.T; L.GX[wg] = b.gx; L.GZ[wg] = b.gz; ch = true; g_dirty = true;
+        if (g_act && (colors == 1 || g_color == phase)) {
+          b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
+          update_grid_node<double>(v, gi, gj, b);
+          upd = __double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
+                __double_as_longlong(b.gx) != __double_as_longlong(L.GX[wg]) ||
+                __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[wg]) || b.tie != (int)L.tie[wg];
         }
-      }
-      if (phase == 4 && i_act) {
+        if (colors == 1) __syncthreads();
+        if (upd) {
+          L.T[wg] = b.T; L.GX[wg] = b.gx; L.GZ[wg] = b.gz; L.tie[wg] = (uint8_t)b.tie;
+          ch = true; g_dirty = true;
+        }
+      } else if (i_act) {
         Best<double> b;
-        b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0;
+        b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
         update_iface_node<double>(v, is, ic, b);
         if (__double_as_longlong(b.T) != __double_as_longlong(L.iT[wi]) ||
             __double_as_longlong(b.gx) != __double_as_longlong(L.iGX[wi]) ||
@@ -418,19 +434,24 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
     more = __syncthreads_or(ch);
     if (!more) break;
   }
+  if (lane == 0 && acc) {
+    atomicAdd(acc, (unsigned long long)(sweep + 1));
+    atomicMax(acc + 1, (unsigned long long)(sweep + 1));
+    if (more) atomicAdd(acc + 2, 1ull);
+  }
   // ---- write back and activate neighbours whose halo changed
   if (g_dirty) {
     const int q = gj * nx + gi;
-    d.T[q] = L.T[wg]; d.GX[q] = L.GX[wg]; d.GZ[q] = L.GZ[wg];
+    d.T[q] = L.T[wg]; d.GX[q] = L.GX[wg]; d.GZ[q] = L.GZ[wg]; d.TIE[q] = L.tie[wg];
   }
   if (i_dirty) {
     const int q = (nz + is) * nx + ic;
     d.T[q] = L.iT[wi]; d.GX[q] = L.iGX[wi]; d.GZ[q] = L.iGZ[wi];
   }
-  // border position of each dirty node (interface nodes at their cell corner)
+  // border position of each dirty node (interface nodes at their home row)
   int li = -1, lj = -1;
   if (g_dirty) { li = gi - tx0; lj = gj - tz0; }
-  const int li2 = i_dirty ? ic - tx0 : -1, lj2 = i_dirty ? i_cc - tz0 : -1;
+  const int li2 = i_dirty ? ic - tx0 : -1, lj2 = i_dirty ? i_home - tz0 : -1;
   auto edge = [&](int a, int b) {
     unsigned m = 0;
     if (a < 0) return m;
@@ -903,9 +924,9 @@ hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int nti
 
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, hipStream_t st) {
+                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, hipStream_t st) {
   hipLaunchKernelGGL(fim_tiles_kernel, dim3(nmax), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
-                     tkey, list_next, count_next, max_sweeps);
+                     tkey, list_next, count_next, max_sweeps, colors, acc);
   return hipGetLastError();
 }
 

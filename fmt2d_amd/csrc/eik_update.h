@@ -39,7 +39,21 @@ struct Best {
   Real T, gx, gz;
   int susp;       // gradient is one-sided: refit after convergence (:784)
   int reg_set;    // a regular update produced the current value
+  int tie;        // directions of equal-time members the value rests on (TIE_*)
+  int indep;      // some candidate reaches the value from an earlier time
 };
+
+// Equal-time support.  A regular update clamped to its trigger member's time
+// (t1 = tmax, src/propagate2d.f90:770) gives a node the same time as that
+// member; in the FMM the member popped first, so two nodes can never rest on
+// each other this way.  A node records the directions of the members whose
+// clamped updates alone give its value; a member that rests only on the node
+// itself is not used by it.  This keeps an iteration from locking a pair of
+// equal-time neighbours at a transient value.
+enum { TIE_XP = 1, TIE_XM = 2, TIE_ZP = 4, TIE_ZM = 8 };
+FMT_HD int tie_dir(int di, int dj) {
+  return di > 0 ? TIE_XP : (di < 0 ? TIE_XM : (dj > 0 ? TIE_ZP : (dj < 0 ? TIE_ZM : 0)));
+}
 
 // Candidate visitors.  Every local update is announced with its evaluation
 // time (the time of the member whose pop triggers it = the largest member
@@ -53,9 +67,16 @@ struct MinVisitor {       // fixed-point iteration: keep the minimum
   // member time, T1 + dist/v, the acceptance atime >= T(node1)), so an update
   // evaluated no earlier than the current minimum cannot lower it
   FMT_HD bool want(Real evt) const { return evt < b.T; }
-  FMT_HD void add(Real T, Real gx, Real gz, int kind, int susp) {
-    if (T < b.T) { b.T = T; b.gx = gx; b.gz = gz; b.reg_set = (kind == CK_REGULAR); if (kind == CK_REGULAR) b.susp = susp; }
+  FMT_HD void add(Real T, Real gx, Real gz, int kind, int susp, int tie) {
+    if (T < b.T) {
+      b.T = T; b.gx = gx; b.gz = gz; b.reg_set = (kind == CK_REGULAR); if (kind == CK_REGULAR) b.susp = susp;
+      b.tie = tie; b.indep = tie == 0;
+    } else if (T == b.T) {
+      if (tie) b.tie |= tie; else b.indep = 1;
+    }
   }
+  // support mask to store with the node
+  FMT_HD int tie_mask() const { return b.indep ? 0 : b.tie; }
 };
 
 // ---------------------------------------------------------------- regular
@@ -142,9 +163,9 @@ FMT_HD void cand1(Real xu, Real zu, Real vu, Real x1, Real z1, Real t1, Real g1x
                   Real tol, Vis& vis) {
   const Real dx = xu - x1, dz = zu - z1;
   const Real dist = sqrt(dx * dx + dz * dz);
-  if (dist < (Real)0.01 * tol) { vis.add(t1, g1x, g1z, CK_POINT, 0); return; }
+  if (dist < (Real)0.01 * tol) { vis.add(t1, g1x, g1z, CK_POINT, 0, 0); return; }
   const Real at = t1 + dist / ((Real)0.5 * (vu + v1));
-  vis.add(at, dx / (dist * vu), dz / (dist * vu), CK_POINT, 0);
+  vis.add(at, dx / (dist * vu), dz / (dist * vu), CK_POINT, 0, 0);
 }
 
 // time_from_2_nodes2d (src/propagate2d.f90:831-946)
@@ -160,8 +181,8 @@ FMT_HD void cand2(Real xu, Real zu, Real vu,
   }
   const Real ctol = (Real)0.01 * tol;
   const Real r10 = xa - xu, r11 = za - zu, r20 = xb - xu, r21 = zb - zu;
-  if (sqrt(r10 * r10 + r11 * r11) < ctol) { vis.add(ta, gax, gaz, CK_POINT, 0); return; }
-  if (sqrt(r20 * r20 + r21 * r21) < ctol) { vis.add(tb, gbx, gbz, CK_POINT, 0); return; }
+  if (sqrt(r10 * r10 + r11 * r11) < ctol) { vis.add(ta, gax, gaz, CK_POINT, 0, 0); return; }
+  if (sqrt(r20 * r20 + r21 * r21) < ctol) { vis.add(tb, gbx, gbz, CK_POINT, 0, 0); return; }
   const Real e0 = xb - xa, e1 = zb - za;
   if (sqrt(e0 * e0 + e1 * e1) < ctol) {
     cand1<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, tol, vis);
@@ -196,7 +217,7 @@ FMT_HD void cand2(Real xu, Real zu, Real vu,
       const Real at = ta - (w0 * r10 + w1 * r11) / vm;
       if (at >= ta) {
         const Real wl = sqrt(w0 * w0 + w1 * w1);
-        vis.add(at, w0 / (wl * va), w1 / (wl * va), CK_POINT, 0);
+        vis.add(at, w0 / (wl * va), w1 / (wl * va), CK_POINT, 0, 0);
       }
     }
   }
@@ -266,6 +287,21 @@ FMT_HD void irregular_cell(const V& v, int cx, int cz, const Member& self, const
   }
 }
 
+// Equal-time bookkeeping of a regular update of value T: `tie` gets the
+// directions of the members whose time equals T; returns false when one of
+// them rests only on node (iu,ju) (it popped after the node, so the FMM
+// never evaluates this update).
+template <typename Real, class V>
+FMT_HD bool regular_tie(const V& v, int iu, int ju, const int* mx, const int* mz, int nm, Real T, int& tie) {
+  tie = 0;
+  for (int n = 0; n < nm; n++) {
+    if ((Real)v.T(mx[n], mz[n]) != T) continue;
+    if (v.tie(mx[n], mz[n]) == tie_dir(iu - mx[n], ju - mz[n])) return false;
+    tie |= tie_dir(mx[n] - iu, mz[n] - ju);
+  }
+  return true;
+}
+
 // quadrant / regular cell (cx,cz) of grid node (iu,ju)
 template <typename Real, class V, class Vis>
 FMT_HD void regular_cell(const V& v, int iu, int ju, int cx, int cz, Vis& vis) {
@@ -283,7 +319,10 @@ FMT_HD void regular_cell(const V& v, int iu, int ju, int cx, int cz, Vis& vis) {
     Member fm; fm.kind = 0; fm.i = fi; fm.j = fj;
     if (!v.fixed(fm) && vis.want((Real)v.T(fi, fj))) {
       mx[0] = fi; mz[0] = fj;
-      if (regular_cand<Real>(v, iu, ju, mx, mz, 1, T, gx, gz, susp)) vis.add(T, gx, gz, CK_REGULAR, susp);
+      if (regular_cand<Real>(v, iu, ju, mx, mz, 1, T, gx, gz, susp)) {
+        int tie;
+        if (regular_tie<Real>(v, iu, ju, mx, mz, 1, T, tie)) vis.add(T, gx, gz, CK_REGULAR, susp, tie);
+      }
     }
   }
   if (hx && hz) {
@@ -291,8 +330,10 @@ FMT_HD void regular_cell(const V& v, int iu, int ju, int cx, int cz, Vis& vis) {
     const bool zlater = v.T(iu, zn) >= v.T(xn, ju);
     mx[0] = zlater ? iu : xn; mz[0] = zlater ? zn : ju;
     mx[1] = zlater ? xn : iu; mz[1] = zlater ? ju : zn;
-    if (vis.want((Real)v.T(mx[0], mz[0])) && regular_cand<Real>(v, iu, ju, mx, mz, 2, T, gx, gz, susp))
-      vis.add(T, gx, gz, CK_REGULAR, susp);
+    if (vis.want((Real)v.T(mx[0], mz[0])) && regular_cand<Real>(v, iu, ju, mx, mz, 2, T, gx, gz, susp)) {
+      int tie;
+      if (regular_tie<Real>(v, iu, ju, mx, mz, 2, T, tie)) vis.add(T, gx, gz, CK_REGULAR, susp, tie);
+    }
   }
 }
 
@@ -343,11 +384,23 @@ FMT_HD int iface_partner_row(const V& v, int s, int c) {
   return -1;
 }
 
+// Row of the tile schedule that interface node (s,c) with cut-cell row cz is
+// solved in: cz, or cz+1 when it coincides with the grid node below.  An
+// interface node reads its cell (rows cz, cz+1) and the candidates of its
+// coincident grid node, which reach two rows beyond that node; homing the
+// interface node at its partner's row keeps every read and every dependency
+// within two rows of the node's home.
+template <class V>
+FMT_HD int iface_home_row(const V& v, int s, int c, int cz) {
+  const int pj = iface_partner_row(v, s, c);
+  return pj == cz + 1 ? pj : cz;
+}
+
 // own minimum of interface node (s,c) leaving out grid node (pi,pj)
 template <typename Real, class V>
 FMT_HD MinVisitor<Real> iface_own_min(const V& v, int s, int c, int pi, int pj) {
   MinVisitor<Real> m;
-  m.b.T = v.iT0(s, c); m.b.gx = v.iGX0(s, c); m.b.gz = v.iGZ0(s, c); m.b.susp = 0; m.b.reg_set = 0;
+  m.b.T = v.iT0(s, c); m.b.gx = v.iGX0(s, c); m.b.gz = v.iGZ0(s, c); m.b.susp = 0; m.b.reg_set = 0; m.b.tie = 0; m.b.indep = 1;
   Member ex; ex.kind = 0; ex.i = pi; ex.j = pj;
   own_iface_candidates<Real>(v, s, c, ex, m);
   return m;
@@ -357,7 +410,7 @@ FMT_HD MinVisitor<Real> iface_own_min(const V& v, int s, int c, int pi, int pj) 
 template <typename Real, class V>
 FMT_HD MinVisitor<Real> grid_own_min(const V& v, int c, int j, int s) {
   MinVisitor<Real> m;
-  m.b.T = v.T0(c, j); m.b.gx = v.GX0(c, j); m.b.gz = v.GZ0(c, j); m.b.susp = 0; m.b.reg_set = 0;
+  m.b.T = v.T0(c, j); m.b.gx = v.GX0(c, j); m.b.gz = v.GZ0(c, j); m.b.susp = 0; m.b.reg_set = 0; m.b.tie = 0; m.b.indep = 1;
   Member ex; ex.kind = 1 + s; ex.i = c; ex.j = 0;
   Member gm; gm.kind = 0; gm.i = c; gm.j = j;
   if (!v.fixed(gm)) own_grid_candidates<Real>(v, c, j, ex, m);
@@ -373,7 +426,7 @@ FMT_HD void grid_candidates(const V& v, int iu, int ju, Vis& vis) {
   own_grid_candidates<Real>(v, iu, ju, excl, vis);
   if (ps >= 0) {
     MinVisitor<Real> p = iface_own_min<Real>(v, ps, iu, iu, ju);
-    if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0);
+    if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0, 0);
   }
 }
 
@@ -389,16 +442,18 @@ FMT_HD void iface_candidates(const V& v, int s, int c, Vis& vis) {
   own_iface_candidates<Real>(v, s, c, excl, vis);
   if (pj >= 0) {
     MinVisitor<Real> p = grid_own_min<Real>(v, c, pj, s);
-    if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0);
+    if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0, 0);
   }
 }
 
-// Fixed-point update of grid node (iu,ju); b holds the node's seed on entry.
+// Fixed-point update of grid node (iu,ju); b holds the node's seed on entry,
+// on return b.tie is the node's equal-time support mask.
 template <typename Real, class V>
 FMT_HD void update_grid_node(const V& v, int iu, int ju, Best<Real>& b) {
   MinVisitor<Real> m; m.b = b;
   grid_candidates<Real>(v, iu, ju, m);
   b = m.b;
+  b.tie = m.tie_mask();
 }
 
 template <typename Real, class V>
@@ -480,7 +535,7 @@ struct EvtCollector {
     e[k] = t; n++;
     return false;
   }
-  FMT_HD void add(Real, Real, Real, int, int) {}
+  FMT_HD void add(Real, Real, Real, int, int, int) {}
 };
 
 // Replays the updates of one evaluation time in the FMM's order.
@@ -489,7 +544,7 @@ struct ReplayVisitor {
   Real evt, T;
   int susp;
   FMT_HD bool want(Real t) const { return t == evt; }
-  FMT_HD void add(Real t, Real, Real, int kind, int s) {
+  FMT_HD void add(Real t, Real, Real, int kind, int s, int) {
     if (t < T) { T = t; if (kind == CK_REGULAR) susp = s; }
   }
 };
@@ -504,7 +559,7 @@ FMT_HD int grid_suspect(const V& v, int iu, int ju, Real seedT) {
   const uint16_t fl = v.flags(iu, ju);
   if (!(fl & (RF_IRR0 * 15)) && grid_partner_side(v, iu, ju) < 0) {
     MinVisitor<Real> m;
-    m.b.T = seedT; m.b.gx = 0; m.b.gz = 0; m.b.susp = 0; m.b.reg_set = 0;
+    m.b.T = seedT; m.b.gx = 0; m.b.gz = 0; m.b.susp = 0; m.b.reg_set = 0; m.b.tie = 0; m.b.indep = 1;
     grid_candidates<Real>(v, iu, ju, m);
     return m.b.susp;
   }

@@ -55,10 +55,10 @@ static void intersection_velocities(const HostModel& m, HostInterface& is) {
   const double vdx = m.dx, dz = m.dz;
   for (int c = 0; c < is.n; c++) {
     const double px = is.x[c], pz = is.z[c];
-    int ivx = (int)floor((px - m.xg[0]) / vdx) + 1;
+    int ivx = (int)floor((px - m.vx[0]) / vdx) + 1;
     if (ivx < 1) ivx = 1;
     if (ivx >= m.nx) ivx = m.nx - 1;
-    double wx = (px - m.xg[ivx - 1]) / vdx;
+    double wx = (px - m.vx[ivx - 1]) / vdx;
     wx = std::min(1.0, std::max(0.0, wx));
     for (int side = 0; side < 2; side++) {
       const double po = side == 0 ? pz - dz : pz + dz;
@@ -107,6 +107,7 @@ std::string build_model(HostModel& m, int nx_s, int nz_s, const double* xpos, co
   m.xg.resize(m.nx);
   m.zg.resize(m.nz);
   for (int i = 0; i < m.nx; i++) m.xg[i] = m.x0 + i * m.dx;
+  m.vx.assign(xpos, xpos + nx_s);
   for (int j = 0; j < m.nz; j++) m.zg[j] = m.z0 + j * m.dz;
   const int nx = m.nx, nz = m.nz;
   // interfaces must sit on the grid columns (the cut-cell bookkeeping uses

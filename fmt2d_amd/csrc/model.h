@@ -31,6 +31,7 @@ struct HostModel {
   double dx = 0, dz = 0, x0 = 0, z0 = 0, tol = 0;
   double vwater = 1.5;
   std::vector<double> xg, zg;
+  std::vector<double> vx;  // velocity-grid x: the smesh node positions (fmt2d_main.f90:181)
   std::vector<double> vgrid;               // (nz, nx) z-resampled velocity
   std::vector<int> node_region;            // last writer (reference semantics)
   std::vector<uint8_t> assoc;              // (nz, nx) cell cut by any interface

@@ -29,7 +29,8 @@ class GridInfo(C.Structure):
 class ForwardStats(C.Structure):
     _fields_ = [("nsrc", C.c_int), ("nfield", C.c_int), ("launches", C.c_int),
                 ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double),
-                ("ms_fim", C.c_double), ("exit_nodes", C.c_longlong)]
+                ("ms_fim", C.c_double), ("exit_nodes", C.c_longlong), ("sweeps", C.c_longlong),
+                ("capped", C.c_longlong), ("max_sweeps", C.c_int)]
 
 
 class Fmt2dError(RuntimeError):
@@ -55,7 +56,7 @@ def load() -> C.CDLL:
     lib.fmt2d_forward.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_forward_stats_get.argtypes = [C.c_void_p, C.POINTER(ForwardStats)]
     lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
-    lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     lib.fmt2d_tf_region.argtypes = [C.c_int]
     lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     _LIB = lib
@@ -114,8 +115,8 @@ class Context:
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
 
-    def set_tuning(self, window_tiles: float = 2.0, max_sweeps: int = 64):
-        _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps)))
+    def set_tuning(self, window_tiles: float = 2.0, max_sweeps: int = 64, colors: int = 4):
+        _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
 
     def stats(self) -> ForwardStats:
         st = ForwardStats()

@@ -54,6 +54,9 @@ typedef struct {
   double ms_total;          /* wall time of the forward */
   double ms_fim;            /* summed device time of the tile-solver launches (HIP events) */
   long long exit_nodes;     /* nodes checked exactly by the heap-exhaustion stop */
+  long long sweeps;         /* in-tile sweeps over all tile visits */
+  long long capped;         /* tile visits that reached the sweep cap */
+  int max_sweeps;           /* most sweeps of one tile visit */
 } fmt2d_forward_stats;
 
 /* Context on HIP device `device` (owns all device memory). */
@@ -74,8 +77,9 @@ int fmt2d_grid_info_get(fmt2d_ctx* ctx, fmt2d_grid_info* info);
 int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags);
 int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
 /* Solver tuning: arrival-time window of the tile schedule in tile crossing
- * times (at 4 km/s) and the sweep cap of one tile visit.  Defaults 2.0, 64. */
-int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps);
+ * times (at 4 km/s), the sweep cap of one tile visit, and the in-tile sweep
+ * order (4: four-colour Gauss-Seidel, 1: Jacobi).  Defaults 2.0, 64, 4. */
+int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors);
 /* Due-tile count of every solver launch of the last forward (diagnostics). */
 int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int max, int* n);
 

@@ -776,7 +776,19 @@ static void time_from_2(FMM *F, int u, int a1, int a2) {
   }
 }
 
+static int trace_node = -2;
+static void update_time_impl(FMM *F, int u);
 static void update_time(FMM *F, int u) {
+  /* ORC_TRACE=<region node id> (diagnostics): report every change of that node */
+  if (trace_node == -2) { const char *e = getenv("ORC_TRACE"); trace_node = e ? atoi(e) : -1; }
+  double old = F->R->T[u];
+  update_time_impl(F, u);
+  if (u == trace_node && F->R->T[u] != old)
+    fprintf(stderr, "trace node %d: newalive %d (i1=%d i2=%d i3=%d T=%.17g) T %.17g -> %.17g g=(%.17g,%.17g)\n", u,
+            F->newalive, F->R->i1[F->newalive], F->R->i2[F->newalive], F->R->i3[F->newalive], F->R->T[F->newalive],
+            old, F->R->T[u], F->R->gx[u], F->R->gz[u]);
+}
+static void update_time_impl(FMM *F, int u) {
   Model *M = F->M;
   Region *R = F->R;
   int i1 = R->i1[u], i2 = R->i2[u];
@@ -946,8 +958,10 @@ void orc_propagate(Model *M, Region *R) {
       }
     }
   }
-  for (int n = 1; n <= R->nnode; n++)
-    if (F.suspect[n]) orc_fit_gradient(M, R, n, &R->gx[n], &R->gz[n]);
+  /* ORC_NO_REFIT (diagnostics): keep the raw marching gradients */
+  if (!getenv("ORC_NO_REFIT"))
+    for (int n = 1; n <= R->nnode; n++)
+      if (F.suspect[n]) orc_fit_gradient(M, R, n, &R->gx[n], &R->gz[n]);
   free(F.tree); free(F.counted); free(F.suspect);
 }
 

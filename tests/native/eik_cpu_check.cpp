@@ -27,6 +27,7 @@ struct CpuField {
   int nx, nz;
   std::vector<RT> T, T0;
   std::vector<GT> GX, GZ, GX0, GZ0;
+  std::vector<uint8_t> TIE;
   int fixed_key = -1;
 };
 
@@ -39,6 +40,7 @@ struct CpuView {
   RT T(int i, int j) const { return F->T[j * R->nx + i]; }
   GT GX(int i, int j) const { return F->GX[j * R->nx + i]; }
   GT GZ(int i, int j) const { return F->GZ[j * R->nx + i]; }
+  int tie(int i, int j) const { return F->TIE.empty() ? 0 : F->TIE[j * R->nx + i]; }
   double vel(int i, int j) const { return R->vel[j * R->nx + i]; }
   double xg(int i) const { return R->xg[i]; }
   double zg(int j) const { return R->zg[j]; }
@@ -108,7 +110,9 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
       for (int s = 0; s < 2; s++)
         for (int c = tx0; c < std::min(nx - 1, tx0 + TILE); c++) {
           const int cc = R.ifcc[s][c];
-          if (cc >= tz0 && cc < tz0 + TILE) keys.push_back((nz + s) * nx + c);
+          if (cc < 0) continue;
+          const int home = iface_home_row(vv, s, c, cc);
+          if (home >= tz0 && home < tz0 + TILE) keys.push_back((nz + s) * nx + c);
         }
       std::vector<char> dirty(keys.size(), 0);
       int sweep;
@@ -119,10 +123,11 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
         bool ch = false;
         for (size_t k = 0; k < keys.size(); k++) {
           const int g = keys[k], j = g / nx, i = g % nx;
-          Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0};
+          Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0, 0, 1};
           if (j < nz) update_grid_node<RT>(sv, i, j, b); else update_iface_node<RT>(sv, j - nz, i, b);
-          if (b.T != loc.T[g] || (GT)b.gx != loc.GX[g] || (GT)b.gz != loc.GZ[g]) {
-            loc.T[g] = b.T; loc.GX[g] = b.gx; loc.GZ[g] = b.gz; ch = true; dirty[k] = 1;
+          const int tie = j < nz ? b.tie : 0;
+          if (b.T != loc.T[g] || (GT)b.gx != loc.GX[g] || (GT)b.gz != loc.GZ[g] || tie != loc.TIE[g]) {
+            loc.T[g] = b.T; loc.GX[g] = b.gx; loc.GZ[g] = b.gz; loc.TIE[g] = tie; ch = true; dirty[k] = 1;
           }
         }
         if (!ch) { more = false; break; }
@@ -131,9 +136,9 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
       for (size_t k = 0; k < keys.size(); k++) {
         if (!dirty[k]) continue;
         const int g = keys[k];
-        F.T[g] = loc.T[g]; F.GX[g] = loc.GX[g]; F.GZ[g] = loc.GZ[g];
+        F.T[g] = loc.T[g]; F.GX[g] = loc.GX[g]; F.GZ[g] = loc.GZ[g]; F.TIE[g] = loc.TIE[g];
         int j = g / nx, i = g % nx;
-        if (j >= nz) j = R.ifcc[j - nz][i];
+        if (j >= nz) j = iface_home_row(vv, j - nz, i, R.ifcc[j - nz][i]);
         const int a = i - tx0, b = j - tz0;
         const bool L0 = a < 2, R0 = a >= TILE - 2, T0 = b < 2, B0 = b >= TILE - 2;
         if (L0) mask |= 1; if (R0) mask |= 2; if (T0) mask |= 4; if (B0) mask |= 8;
@@ -226,6 +231,7 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
       else { F.T0[g] = (RT)(dist / va); F.GX0[g] = (RT)(dx / (dist * va)); F.GZ0[g] = (RT)(dz / (dist * va)); }
     }
   F.T = F.T0; F.GX = F.GX0; F.GZ = F.GZ0;
+  F.TIE.assign(N, 0);
   CpuView v{&R, &F};
   int it;
   long long tiles_done = 0;
@@ -245,15 +251,16 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
     for (int j = 0; j < nz + 2; j++)
       for (int i = 0; i < nx; i++) {
         const size_t g = (size_t)j * nx + i;
-        Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0};
+        Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0, 0, 1};
         if (j < nz) {
           if (!(R.flags[g] & RF_IN) || (int)g == F.fixed_key) continue;
           update_grid_node<RT>(v, i, j, b);
         } else {
           update_iface_node<RT>(v, j - nz, i, b);
         }
-        if (b.T != F.T[g] || (GT)b.gx != F.GX[g] || (GT)b.gz != F.GZ[g]) {
-          F.T[g] = b.T; F.GX[g] = b.gx; F.GZ[g] = b.gz; changed++;
+        const int tie = j < nz ? b.tie : 0;
+        if (b.T != F.T[g] || (GT)b.gx != F.GX[g] || (GT)b.gz != F.GZ[g] || tie != F.TIE[g]) {
+          F.T[g] = b.T; F.GX[g] = b.gx; F.GZ[g] = b.gz; F.TIE[g] = tie; changed++;
         }
       }
     v.F = &F;
@@ -304,6 +311,93 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
     outT[k] = F.T[g]; outGX[k] = OX[g]; outGZ[k] = OZ[g];
   }
   return it;
+}
+
+// One Jacobi pass of the operator over a given field (reference node order):
+// where the result differs from the input, the field is no fixed point of
+// the operator.  Diagnostics of operator-vs-FMM differences.
+int cpu_point_eval(void* hp, int r, double sx, const double* inT, const double* inGX, const double* inGZ,
+                   double* outT, double* outGX, double* outGZ) {
+  Holder* h = (Holder*)hp;
+  HostModel& m = h->m;
+  const RegionDev& R = h->rd[r];
+  const int nx = m.nx, nz = m.nz;
+  const size_t N = (size_t)(nz + 2) * nx;
+  CpuField F;
+  F.nx = nx; F.nz = nz;
+  F.T0.assign(N, FMT_HUGE_TIME); F.GX0.assign(N, 0.f); F.GZ0.assign(N, 0.f);
+  const double sz = m.iface_z_at(2, sx);
+  int ix0 = (int)floor((sx - m.x0) / m.dx), iz0 = (int)floor((sz - m.z0) / m.dz);
+  ix0 = std::max(0, std::min(ix0, nx - 2));
+  iz0 = std::max(0, std::min(iz0, nz - 2));
+  for (int i = ix0; i <= ix0 + 1; i++)
+    for (int j = iz0; j <= iz0 + 1; j++) {
+      const size_t g = (size_t)j * nx + i;
+      if (!(R.flags[g] & RF_IN)) continue;
+      const double dx = m.xg[i] - sx, dz = m.zg[j] - sz, dist = sqrt(dx * dx + dz * dz);
+      double va = m.reg[r].vel[g];
+      if (va < 0.01) va = 0.01;
+      if (dist < 0.01 * m.tol) { F.T0[g] = 0; F.fixed_key = (int)g; }
+      else { F.T0[g] = (RT)(dist / va); F.GX0[g] = (RT)(dx / (dist * va)); F.GZ0[g] = (RT)(dz / (dist * va)); }
+    }
+  F.T.assign(N, FMT_HUGE_TIME); F.GX.assign(N, 0.0); F.GZ.assign(N, 0.0);
+  const HostRegion& HR = m.reg[r];
+  for (int k = 0; k < HR.nnode; k++) {
+    const int g = HR.ref_order[k];
+    F.T[g] = inT[k]; F.GX[g] = inGX[k]; F.GZ[g] = inGZ[k];
+  }
+  CpuView v{&R, &F};
+  // pass 1: equal-time support masks of the given field; pass 2: evaluate
+  F.TIE.assign(N, 0);
+  std::vector<uint8_t> tie(N, 0);
+  for (int pass = 0; pass < 2; pass++) {
+    for (int k = 0; k < HR.nnode; k++) {
+      const int g = HR.ref_order[k], j = g / nx, i = g % nx;
+      Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0, 0, 1};
+      if (g == F.fixed_key) { outT[k] = F.T[g]; outGX[k] = F.GX[g]; outGZ[k] = F.GZ[g]; continue; }
+      if (j < nz) update_grid_node<RT>(v, i, j, b); else update_iface_node<RT>(v, j - nz, i, b);
+      if (pass == 0) { if (j < nz) tie[g] = (uint8_t)b.tie; continue; }
+      outT[k] = b.T; outGX[k] = b.gx; outGZ[k] = b.gz;
+    }
+    F.TIE = tie;
+  }
+  return 0;
+}
+
+struct PrintVisitor {
+  Best<double> b;
+  double last_evt = -1;
+  bool want(double evt) { last_evt = evt; return true; }
+  void add(double T, double gx, double gz, int kind, int susp, int tie) {
+    fprintf(stderr, "    cand kind=%d evt=%.17g T=%.17g g=(%.17g,%.17g) susp=%d tie=%d%s\n", kind, last_evt, T, gx, gz, susp, tie,
+            T < b.T ? "  <-- new min" : "");
+    if (T < b.T) { b.T = T; b.gx = gx; b.gz = gz; }
+  }
+};
+
+// Prints every candidate of node k (reference order) of the given field.
+int cpu_node_debug(void* hp, int r, double sx, const double* inT, const double* inGX, const double* inGZ, int k) {
+  Holder* h = (Holder*)hp;
+  HostModel& m = h->m;
+  const RegionDev& R = h->rd[r];
+  const int nx = m.nx, nz = m.nz;
+  const size_t N = (size_t)(nz + 2) * nx;
+  CpuField F;
+  F.nx = nx; F.nz = nz;
+  F.T0.assign(N, FMT_HUGE_TIME); F.GX0.assign(N, 0.f); F.GZ0.assign(N, 0.f);
+  F.T.assign(N, FMT_HUGE_TIME); F.GX.assign(N, 0.0); F.GZ.assign(N, 0.0);
+  const HostRegion& HR = m.reg[r];
+  for (int q = 0; q < HR.nnode; q++) {
+    const int g = HR.ref_order[q];
+    F.T[g] = inT[q]; F.GX[g] = inGX[q]; F.GZ[g] = inGZ[q];
+  }
+  CpuView v{&R, &F};
+  const int g = HR.ref_order[k], j = g / nx, i = g % nx;
+  fprintf(stderr, "node %d slot (%d,%d) T=%.17g flags=%x\n", k, i, j, F.T[g], j < nz ? R.flags[g] : 0);
+  PrintVisitor pv;
+  pv.b.T = FMT_HUGE_TIME; pv.b.gx = 0; pv.b.gz = 0; pv.b.susp = 0; pv.b.reg_set = 0; pv.b.tie = 0; pv.b.indep = 1;
+  if (j < nz) grid_candidates<double>(v, i, j, pv); else iface_candidates<double>(v, j - nz, i, pv);
+  return 0;
 }
 
 }  // extern "C"
