@@ -3,7 +3,7 @@
  * a checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
  * It is never linked into, called by, or shipped with the product library.
  *
- * Restated (FP64, same operation order) from /root/reference:
+ * Restated (FP64, same operation order) from the reference sources:
  *   geometry       src/propagate2d.f90:1335-1979  (find_intersection2d,
  *                  define_region2d, velocities_to_*, tag_regular_nodes2d)
  *   model build    src/fmt2d_main.f90:91-418 (interfaces, resampling, regions)
@@ -20,7 +20,7 @@
  *
  * Pinned against the reference's own routines (oracle/ref_fields.f90 driving
  * the reference modules built by oracle/build_ref.sh): geometry, velocities and
- * every stage field are compared node by node in tests/test_oracle_fields.py.
+ * every stage field are compared node by node in tests/test_oracle_pin.py.
  *
  * Indexing follows the reference: node ids and grid indices are 1-based.
  */

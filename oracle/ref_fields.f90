@@ -7,7 +7,7 @@
 ! per-source stage schedule of run_forward2d (src/inversion2d.f90:4184-4416)
 ! for every listed source and writes the raw region fields after each stage.
 !
-! It is linked against the reference modules compiled from /root/reference
+! It is linked against the reference modules compiled from the reference sources
 ! by oracle/build_ref.sh and never ships with the product.
 !
 ! Input (ref_fields.in, one item per line):
