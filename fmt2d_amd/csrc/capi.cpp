@@ -11,112 +11,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fmt2d_hip.h"
-#include "eik_kernels.h"
-#include "heap_exit.h"
-#include "model.h"
-
-using namespace fmt2d;
+#include "ctx.h"
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int fmt2d_fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
-}
-
-#define HIPCHK(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess) return fail(FMT2D_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-  } while (0)
-
-namespace {
-
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DBuf() { release(); }
-};
-
-// stage table of run_forward2d (src/invert2d_main.f90:564-811,
-// src/inversion2d.f90:4184-4416): region, start interface, previous stage,
-// water-layer top-row fix (+1 up-going, -1 down-going)
-const int TF_REGION[13] = {0, 5, 2, 2, 6, 6, 1, 1, 1, 1, 1, 5, 1};
-const int TF_ISTART[13] = {0, 2, 2, 3, 2, 4, 2, 1, 2, 2, 1, 2, 2};
-const int TF_PREV[13] = {0, 0, 0, 2, 0, 4, 1, 6, 7, 0, 9, 10, 11};
-const int TF_WATERFIX[13] = {0, 0, 0, 0, 0, 0, 1, -1, 1, 1, -1, 0, 1};
-const int LEVELS[4][4] = {{1, 2, 4, 9}, {3, 5, 6, 10}, {7, 11, 0, 0}, {8, 12, 0, 0}};
-const int LEVEL_N[4] = {4, 4, 2, 2};
-
-}  // namespace
-
-struct fmt2d_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool have_model = false;
-  HostModel m;
-  // model on the device
-  DBuf<double> xg, zg, rvel[NREG + 1], ifx[NREG + 1][2], ifz[NREG + 1][2];
-  DBuf<uint16_t> rflags[NREG + 1];
-  DBuf<int> ifcc[NREG + 1][2];
-  DBuf<uint8_t> ifok[NREG + 1][2];
-  DBuf<RegionDev> regs;
-  DBuf<double> isn[NIF + 1][4];  // nx, nz, vel_top, vel_bot
-  DBuf<IsecDev> isecs;
-  DBuf<int> hw_slot;
-  DBuf<double> hw_vhw;
-  DBuf<HeadwaveMap> hw_map;
-  // fields of the last forward
-  int nsrc = 0, nfield = 0;
-  std::vector<int> field_of;           // [isrc*12 + itf-1] -> field index
-  DBuf<double> fT, isT, fGX, fGZ, fOGX, fOGZ, isGX, isGZ;
-  DBuf<uint8_t> fTIE;
-  DBuf<FieldDesc> descs;
-  DBuf<StageLink> links;
-  DBuf<unsigned int> tin;             // tile in active set
-  DBuf<unsigned long long> tkey;      // tile arrival key
-  DBuf<unsigned long long> tfront;    // per-field front key
-  DBuf<int2> tlist[3];                // active set, its next state, due tiles
-  DBuf<int> tcount;                   // [due, next]
-  double window_tiles = 2.0;          // window in tile crossing times
-  int max_sweeps = 64;
-  int colors = 4;                     // 4: four-colour Gauss-Seidel in a tile, 1: Jacobi
-  DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
-  DBuf<int> sel_which, sel_up;
-  DBuf<double> hw_scratch;
-  // heap-exhaustion stop work arrays
-  DBuf<ExitField> x_ef;
-  DBuf<int> x_cnt, x_diff, x_off, x_total;
-  DBuf<unsigned long long> x_max, x_m;
-  DBuf<ExitEntry> x_ent;
-  int* h_count = nullptr;              // pinned
-  std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
-  fmt2d_forward_stats stats{};
-  std::vector<int> trace;              // active tiles per launch (all levels)
-  ~fmt2d_ctx() {
-    for (auto e : ev) (void)hipEventDestroy(e);
-    if (h_count) (void)hipHostFree(h_count);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-template <typename T>
-static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
-  hipError_t e = b.alloc(v.size());
-  if (e != hipSuccess) return e;
-  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
 }
 
 extern "C" {
@@ -158,6 +59,9 @@ int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, cons
     return fail(FMT2D_EINVAL, "NULL argument");
   HIPCHK(hipSetDevice(ctx->device));
   ctx->have_model = false;
+  ctx->rgeom_ok = false;
+  ctx->nsrc = 0;
+  ctx->nray = 0;
   ctx->m = HostModel();
   std::string e = build_model(ctx->m, nx_s, nz_s, xpos, topo, zrel, vel, vwater, nsed, sed_x, sed_z, nmoho, moho_x,
                               moho_z);
@@ -294,6 +198,10 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   }
   ctx->nsrc = nsrc;
   ctx->nfield = nfield;
+  ctx->src_x.assign(src_x, src_x + nsrc);
+  ctx->src_z.resize(nsrc);
+  for (int s = 0; s < nsrc; s++) ctx->src_z[s] = m.iface_z_at(2, src_x[s]);
+  ctx->nray = 0;
   std::vector<FieldDesc> fd(nfield);
   std::vector<StageLink> lk(nfield);
   long long region_nodes = 0;

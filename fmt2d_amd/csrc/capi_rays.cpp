@@ -1,0 +1,264 @@
+// C ABI of the ray tracer and the Frechet rows: the parameterisation of the
+// inversion (frechet.in), ray backtracing on the resident fields of the last
+// forward, CSR assembly.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "csr.h"
+#include "ctx.h"
+
+namespace {
+
+const int PATH_NSEC[6] = {0, 1, 2, 2, 4, 4};
+
+// Inversion parameters of initialize_inversion2d (frechet2d.f90:14-118):
+// the resampled velocity grid's active nodes (below the seafloor,
+// invert2d_main.f90:346-363 and :461-475), numbered iz-outer / ix-inner, then
+// each inverted interface's input nodes as one block.
+int build_ray_geom(fmt2d_ctx* ctx) {
+  HostModel& m = ctx->m;
+  hipStream_t st = ctx->stream;
+  RayGeomDev& G = ctx->hgeom;
+  memset(&G, 0, sizeof(G));
+  G.nx = m.nx; G.nz = m.nz; G.dx = m.dx; G.dz = m.dz; G.x0 = m.x0; G.z0 = m.z0;
+  G.xg = ctx->xg.p; G.zg = ctx->zg.p;
+  for (int k = 1; k <= NIF; k++) {
+    HIPCHK(upload(ctx->risx[k], m.isec[k].x, st));
+    HIPCHK(upload(ctx->risz[k], m.isec[k].z, st));
+    G.isx[k] = ctx->risx[k].p; G.isz[k] = ctx->risz[k].p; G.isn[k] = (int)m.isec[k].x.size();
+  }
+  for (int r = 1; r <= NREG; r++) {
+    const HostRegion& R = m.reg[r];
+    HIPCHK(upload(ctx->rorder[r], R.ref_order, st));
+    RayRegionDev& d = G.reg[r];
+    d.id = r; d.nnode = R.nnode; d.itop = R.itop; d.ibot = R.ibot;
+    d.order = ctx->rorder[r].p; d.flags = ctx->rflags[r].p; d.vel = ctx->rvel[r].p;
+  }
+  G.reg[0] = G.reg[1];
+  HIPCHK(upload(ctx->rsfx, m.if_x[2], st));
+  HIPCHK(upload(ctx->rsfz, m.if_z[2], st));
+  G.sfx = ctx->rsfx.p; G.sfz = ctx->rsfz.p; G.nsf = (int)m.if_x[2].size();
+  G.sfdx = m.if_x[2][1] - m.if_x[2][0];
+  G.vwater = m.vwater;
+  HIPCHK(upload(ctx->rpvel, m.vgrid, st));
+  G.pvel = ctx->rpvel.p;
+  // inversion velocity grid
+  const int vnx = (int)m.vx.size(), vnz = m.nz;
+  G.vnx = vnx; G.vnz = vnz; G.vdx = m.vx[1] - m.vx[0]; G.vdz = m.dz;
+  std::vector<double> vz(vnz);
+  for (int j = 0; j < vnz; j++) vz[j] = (double)j * m.dz;
+  HIPCHK(upload(ctx->rvx, m.vx, st));
+  HIPCHK(upload(ctx->rvz, vz, st));
+  G.vx = ctx->rvx.p; G.vz = ctx->rvz.p;
+  std::vector<int> inv((size_t)vnx * vnz, 0);
+  int np = 0;
+  const std::vector<double>& topo = m.if_z[2];
+  if (ctx->inv_vel) {
+    for (int iz = 1; iz <= vnz; iz++)
+      for (int ix = 1; ix <= vnx; ix++) {
+        int ivx = ix;
+        if (ivx >= vnx) ivx = vnx - 1;
+        double wx = (double)(ix - ivx);
+        wx = std::min(1.0, std::max(0.0, wx));
+        const double zsf = (1.0 - wx) * topo[ivx - 1] + wx * topo[ivx];
+        if (vz[iz - 1] - zsf >= 0.0) inv[(size_t)(iz - 1) * vnx + (ix - 1)] = ++np;
+      }
+  }
+  HIPCHK(upload(ctx->rinv, inv, st));
+  G.inv_idx = ctx->rinv.p;
+  G.n_vel_par = np;
+  int group = ctx->inv_vel ? 1 : 0, last = np;
+  for (int k = 0; k <= NIF; k++) G.if_start[k] = -1;
+  for (int k : ctx->inv_ifaces) {
+    group++;
+    const int start = group == 1 ? 0 : last;
+    const int nn = (int)m.if_x[k].size();
+    HIPCHK(upload(ctx->rifx_in[k], m.if_x[k], st));
+    G.if_start[k] = start; G.if_nnode[k] = nn; G.ifx_in[k] = ctx->rifx_in[k].p;
+    G.ifdx_in[k] = m.if_x[k][1] - m.if_x[k][0];
+    last = start + nn;
+    np += nn;
+  }
+  G.n_par = np;
+  ctx->n_vel_par = G.n_vel_par;
+  ctx->n_par = np;
+  std::vector<RayGeomDev> one(1, G);
+  HIPCHK(upload(ctx->rgeom, one, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->rgeom_ok = true;
+  return FMT2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids) {
+  if (!ctx || (n_iface > 0 && !iface_ids) || n_iface < 0 || n_iface > 3)
+    return fail(FMT2D_EINVAL, "bad inversion parameterisation");
+  std::vector<int> ids;
+  for (int k = 0; k < n_iface; k++) {
+    if (iface_ids[k] < 2 || iface_ids[k] > 4) return fail(FMT2D_EINVAL, "invertible interfaces are 2, 3, 4");
+    ids.push_back(iface_ids[k]);
+  }
+  ctx->inv_vel = invert_velocity ? 1 : 0;
+  ctx->inv_ifaces = ids;
+  ctx->rgeom_ok = false;
+  return FMT2D_OK;
+}
+
+int fmt2d_inversion_info(fmt2d_ctx* ctx, int* n_vel_par, int* n_par) {
+  if (!ctx || !n_vel_par || !n_par) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
+  *n_vel_par = ctx->n_vel_par;
+  *n_par = ctx->n_par;
+  return FMT2D_OK;
+}
+
+int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const double* rx, const double* rz,
+                double* t_rcv, int* valid) {
+  if (!ctx || nray < 0 || (nray > 0 && (!src || !path || !rx || !rz || !t_rcv || !valid)))
+    return fail(FMT2D_EINVAL, "NULL argument");
+  if (ctx->nsrc < 1) return fail(FMT2D_ESTATE, "no forward has run");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
+  HostModel& m = ctx->m;
+  hipStream_t st = ctx->stream;
+  const size_t nslot = (size_t)(m.nz + 2) * m.nx;
+  // stage fields of every source (final gradients after the post-passes)
+  std::vector<RayField> fl((size_t)ctx->nsrc * FMT2D_NTF);
+  for (int s = 0; s < ctx->nsrc; s++)
+    for (int itf = 1; itf <= FMT2D_NTF; itf++) {
+      const int f = ctx->field_of[s * FMT2D_NTF + itf - 1];
+      RayField& r = fl[(size_t)s * FMT2D_NTF + itf - 1];
+      r.T = ctx->fT.p + (size_t)f * nslot;
+      r.GX = ctx->fOGX.p + (size_t)f * nslot;
+      r.GZ = ctx->fOGZ.p + (size_t)f * nslot;
+      r.region = TF_REGION[itf];
+    }
+  HIPCHK(upload(ctx->rfields, fl, st));
+  HIPCHK(upload(ctx->d_srcx, ctx->src_x, st));
+  HIPCHK(upload(ctx->d_srcz, ctx->src_z, st));
+  ctx->hjobs.resize(nray);
+  ctx->hpoff.resize((size_t)nray + 1);
+  long long off = 0;
+  for (int r = 0; r < nray; r++) {
+    if (src[r] < 0 || src[r] >= ctx->nsrc) return fail(FMT2D_EINVAL, "ray source index out of range");
+    if (path[r] < 1 || path[r] > 5) return fail(FMT2D_EINVAL, "ray path must be 1..5");
+    ctx->hjobs[r] = RayJob{src[r], path[r], rx[r], rz[r]};
+    ctx->hpoff[r] = off;
+    off += (long long)PATH_NSEC[path[r]] * RAY_MAXP;
+  }
+  ctx->hpoff[nray] = off;
+  ctx->nray = nray;
+  if (nray == 0) return FMT2D_OK;
+  HIPCHK(upload(ctx->rjobs, ctx->hjobs, st));
+  HIPCHK(upload(ctx->rpoff, ctx->hpoff, st));
+  if (ctx->rpts.n < (size_t)off) HIPCHK(ctx->rpts.alloc((size_t)off));
+  if (ctx->rout.n < (size_t)nray) HIPCHK(ctx->rout.alloc((size_t)nray));
+  const int nsf = ctx->hgeom.nsf;
+  if (ctx->rtsf.n < (size_t)ctx->nsrc * 3 * nsf) HIPCHK(ctx->rtsf.alloc((size_t)ctx->nsrc * 3 * nsf));
+  HIPCHK(launch_seafloor_times(ctx->rgeom.p, ctx->rfields.p, ctx->nsrc, nsf, ctx->rtsf.p, st));
+  HIPCHK(launch_trace(ctx->rgeom.p, ctx->rfields.p, ctx->d_srcx.p, ctx->d_srcz.p, ctx->rtsf.p, ctx->rjobs.p, nray,
+                      ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, st));
+  std::vector<RayOut> out(nray);
+  HIPCHK(hipMemcpyAsync(out.data(), ctx->rout.p, sizeof(RayOut) * nray, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int r = 0; r < nray; r++) { t_rcv[r] = out[r].t; valid[r] = out[r].valid; }
+  return FMT2D_OK;
+}
+
+int fmt2d_ray_info(fmt2d_ctx* ctx, int iray, double* entry_xz, double* water_time, int* npts, int* flags) {
+  if (!ctx || !entry_xz || !water_time || !npts || !flags) return fail(FMT2D_EINVAL, "NULL argument");
+  if (iray < 0 || iray >= ctx->nray) return fail(FMT2D_EINVAL, "ray index out of range");
+  HIPCHK(hipSetDevice(ctx->device));
+  RayOut o;
+  HIPCHK(hipMemcpy(&o, ctx->rout.p + iray, sizeof(o), hipMemcpyDeviceToHost));
+  entry_xz[0] = o.ex; entry_xz[1] = o.ez;
+  *water_time = o.water_time;
+  for (int k = 0; k < RAY_MAXSEC; k++) npts[k] = o.npts[k];
+  flags[0] = o.headwave; flags[1] = o.diffracted;
+  return FMT2D_OK;
+}
+
+int fmt2d_ray_points(fmt2d_ctx* ctx, int iray, int section, double* x, double* z, int cap, int* npts) {
+  if (!ctx || !npts) return fail(FMT2D_EINVAL, "NULL argument");
+  if (iray < 0 || iray >= ctx->nray) return fail(FMT2D_EINVAL, "ray index out of range");
+  const int nsec = PATH_NSEC[ctx->hjobs[iray].path];
+  if (section < 1 || section > nsec) return fail(FMT2D_EINVAL, "section out of range");
+  HIPCHK(hipSetDevice(ctx->device));
+  RayOut o;
+  HIPCHK(hipMemcpy(&o, ctx->rout.p + iray, sizeof(o), hipMemcpyDeviceToHost));
+  const int n = o.npts[section - 1];
+  *npts = n;
+  if (!x || !z || cap <= 0) return FMT2D_OK;
+  if (cap < n) return fail(FMT2D_EINVAL, "point buffer too small");
+  std::vector<double2> p(n);
+  HIPCHK(hipMemcpy(p.data(), ctx->rpts.p + ctx->hpoff[iray] + (long long)(section - 1) * RAY_MAXP,
+                   sizeof(double2) * n, hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; k++) { x[k] = p[k].x; z[k] = p[k].y; }
+  return FMT2D_OK;
+}
+
+int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, double* val, int cap, int* nnz) {
+  if (!ctx || !nnz) return fail(FMT2D_EINVAL, "NULL argument");
+  if (index_base != 0 && index_base != 1) return fail(FMT2D_EINVAL, "index_base must be 0 or 1");
+  if (ctx->nray < 1) return fail(FMT2D_ESTATE, "no rays traced");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int nray = ctx->nray;
+  // pass 1: contributions per ray
+  DBuf<long long> cnt, coff;
+  HIPCHK(cnt.alloc(nray));
+  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
+                                ctx->rout.p, nullptr, cnt.p, nullptr, nullptr, st));
+  std::vector<long long> hc(nray), ho(nray);
+  HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  long long total = 0;
+  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
+  HIPCHK(upload(coff, ho, st));
+  DBuf<unsigned long long> keys;
+  DBuf<double> vals;
+  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
+                                ctx->rout.p, coff.p, nullptr, keys.p, vals.p, st));
+  int rbits = 1;
+  while ((1LL << rbits) < nray) rbits++;
+  CsrOut co;
+  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, index_base, co, st);
+  if (e != hipSuccess) {
+    if (co.rowptr) (void)hipFree(co.rowptr);
+    if (co.col) (void)hipFree(co.col);
+    if (co.val) (void)hipFree(co.val);
+    return fail(FMT2D_EHIP, std::string("CSR assembly: ") + hipGetErrorString(e));
+  }
+  *nnz = co.nnz;
+  int rc = FMT2D_OK;
+  if (row_ptr) {
+    std::vector<int> rp((size_t)nray + 1);
+    HIPCHK(hipMemcpy(rp.data(), co.rowptr, sizeof(int) * (nray + 1), hipMemcpyDeviceToHost));
+    for (int r = 0; r <= nray; r++) row_ptr[r] = rp[r] + index_base;
+  }
+  if (col && val) {
+    if (cap < co.nnz) rc = fail(FMT2D_EINVAL, "CSR buffers too small");
+    else {
+      HIPCHK(hipMemcpy(col, co.col, sizeof(int) * co.nnz, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(val, co.val, sizeof(double) * co.nnz, hipMemcpyDeviceToHost));
+    }
+  }
+  (void)hipFree(co.rowptr);
+  (void)hipFree(co.col);
+  (void)hipFree(co.val);
+  return rc;
+}
+
+}  // extern "C"

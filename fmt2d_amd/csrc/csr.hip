@@ -1,0 +1,147 @@
+// CSR assembly of the Frechet rows from per-ray contributions.
+//
+// The reference accumulates every ray into a dense scratch row and compacts
+// it by parameter index (compute_frechet_for_ray2d, inversion2d.f90:4573-4577,
+// 4761-4768; read_frechet_data2d packs the rows, :3965-4052).  Here each
+// contribution is a (ray << 32 | parameter, value) pair written in the
+// reference's accumulation order; a stable radix sort groups equal keys
+// without reordering them, so summing each run front to back reproduces the
+// reference's sums bit for bit.  Entries with |sum| <= 1e-30 are dropped as
+// the reference drops them.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "csr.h"
+
+namespace fmt2d {
+
+__global__ void run_heads_kernel(const unsigned long long* __restrict__ k, long long n, int* head) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  head[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+}
+
+__global__ void run_start_kernel(const int* __restrict__ head, const int* __restrict__ rid, long long n,
+                                 long long* start) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (head[i]) start[rid[i]] = i;
+}
+
+// one thread per run: ordered sum, keep flag, per-ray count
+__global__ void run_sum_kernel(const unsigned long long* __restrict__ k, const double* __restrict__ v,
+                               const long long* __restrict__ start, int nrun, long long n, double* sum, int* keep,
+                               int* rowcnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrun) return;
+  const long long a = start[r], b = (r + 1 < nrun) ? start[r + 1] : n;
+  double s = 0.0;
+  for (long long i = a; i < b; i++) s = s + v[i];
+  const int kp = fabs(s) > 1.0e-30 ? 1 : 0;
+  sum[r] = s;
+  keep[r] = kp;
+  if (kp) atomicAdd(&rowcnt[(int)(k[a] >> 32)], 1);
+}
+
+__global__ void run_scatter_kernel(const unsigned long long* __restrict__ k, const long long* __restrict__ start,
+                                   const double* __restrict__ sum, const int* __restrict__ keep,
+                                   const int* __restrict__ pos, int nrun, int base, int* col, double* val) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrun || !keep[r]) return;
+  col[pos[r]] = (int)(k[start[r]] & 0xffffffffull) + base;
+  val[pos[r]] = sum[r];
+}
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, (n ? n : 1) * sizeof(T)); }
+
+#define CK(x) do { hipError_t _e = (x); if (_e != hipSuccess) { err = _e; goto done; } } while (0)
+
+hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nray, int key_bits, int base,
+                     CsrOut& out, hipStream_t st) {
+  hipError_t err = hipSuccess;
+  unsigned long long* k2 = nullptr;
+  double* v2 = nullptr;
+  int *head = nullptr, *rid = nullptr, *keep = nullptr, *pos = nullptr;
+  long long* start = nullptr;
+  double* sum = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0, tb2 = 0;
+  int nrun = 0, nnz = 0;
+  int hl[2];
+  out.nnz = 0;
+  CK(dalloc(&out.rowptr, (size_t)nray + 1));
+  CK(hipMemsetAsync(out.rowptr, 0, sizeof(int) * ((size_t)nray + 1), st));
+  if (n == 0) {
+    CK(dalloc(&out.col, 1));
+    CK(dalloc(&out.val, 1));
+    goto done;
+  }
+  CK(dalloc(&k2, n));
+  CK(dalloc(&v2, n));
+  CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, k2, vals, v2, n, 0, key_bits, st));
+  CK(dalloc(&head, n));
+  CK(dalloc(&rid, n));
+  CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, head, rid, n, st));
+  CK(hipMalloc(&tmp, std::max(tb, tb2)));
+  CK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, k2, vals, v2, n, 0, key_bits, st));
+  hipLaunchKernelGGL(run_heads_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, k2, n, head);
+  CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, head, rid, n, st));
+  CK(hipMemcpyAsync(&hl[0], rid + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(&hl[1], head + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  nrun = hl[0] + hl[1];
+  CK(dalloc(&start, nrun));
+  CK(dalloc(&sum, nrun));
+  CK(dalloc(&keep, nrun));
+  CK(dalloc(&pos, nrun));
+  hipLaunchKernelGGL(run_start_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, head, rid, n, start);
+  // row counts in rowptr[0..nray), scanned below
+  hipLaunchKernelGGL(run_sum_kernel, dim3((nrun + 255) / 256), dim3(256), 0, st, k2, v2, start, nrun, n, sum, keep,
+                     out.rowptr);
+  {
+    size_t tb3 = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, keep, pos, nrun, st));
+    if (tb3 > std::max(tb, tb2)) {
+      CK(hipFree(tmp));
+      tmp = nullptr;
+      CK(hipMalloc(&tmp, tb3));
+      tb = tb3;
+    }
+    CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb3, keep, pos, nrun, st));
+    CK(hipMemcpyAsync(&hl[0], pos + nrun - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&hl[1], keep + nrun - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    nnz = hl[0] + hl[1];
+    CK(dalloc(&out.col, nnz));
+    CK(dalloc(&out.val, nnz));
+    hipLaunchKernelGGL(run_scatter_kernel, dim3((nrun + 255) / 256), dim3(256), 0, st, k2, start, sum, keep, pos, nrun,
+                       base, out.col, out.val);
+    // rowptr: exclusive scan of the counts (nray+1 entries, last count 0)
+    int* rp = nullptr;
+    size_t tb4 = 0;
+    CK(dalloc(&rp, (size_t)nray + 1));
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, out.rowptr, rp, nray + 1, st));
+    void* tmp4 = nullptr;
+    CK(hipMalloc(&tmp4, tb4 ? tb4 : 1));
+    CK(hipcub::DeviceScan::ExclusiveSum(tmp4, tb4, out.rowptr, rp, nray + 1, st));
+    CK(hipStreamSynchronize(st));
+    (void)hipFree(tmp4);
+    (void)hipFree(out.rowptr);
+    out.rowptr = rp;
+  }
+  out.nnz = nnz;
+done:
+  if (k2) (void)hipFree(k2);
+  if (v2) (void)hipFree(v2);
+  if (head) (void)hipFree(head);
+  if (rid) (void)hipFree(rid);
+  if (start) (void)hipFree(start);
+  if (sum) (void)hipFree(sum);
+  if (keep) (void)hipFree(keep);
+  if (pos) (void)hipFree(pos);
+  if (tmp) (void)hipFree(tmp);
+  return err;
+}
+
+}  // namespace fmt2d

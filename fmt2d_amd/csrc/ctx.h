@@ -1,0 +1,138 @@
+// Context of the C ABI (private to the library): device buffers of the
+// model, the resident stage fields of the last forward and the rays of the
+// last trace.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/fmt2d_hip.h"
+#include "eik_kernels.h"
+#include "heap_exit.h"
+#include "model.h"
+#include "rays.h"
+
+using namespace fmt2d;
+
+int fmt2d_fail(int code, const std::string& msg);
+#define fail fmt2d_fail
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(FMT2D_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+// stage table of run_forward2d (src/invert2d_main.f90:564-811,
+// src/inversion2d.f90:4184-4416): region, start interface, previous stage,
+// water-layer top-row fix (+1 up-going, -1 down-going)
+const int TF_REGION[13] = {0, 5, 2, 2, 6, 6, 1, 1, 1, 1, 1, 5, 1};
+const int TF_ISTART[13] = {0, 2, 2, 3, 2, 4, 2, 1, 2, 2, 1, 2, 2};
+const int TF_PREV[13] = {0, 0, 0, 2, 0, 4, 1, 6, 7, 0, 9, 10, 11};
+const int TF_WATERFIX[13] = {0, 0, 0, 0, 0, 0, 1, -1, 1, 1, -1, 0, 1};
+const int LEVELS[4][4] = {{1, 2, 4, 9}, {3, 5, 6, 10}, {7, 11, 0, 0}, {8, 12, 0, 0}};
+const int LEVEL_N[4] = {4, 4, 2, 2};
+
+}  // namespace
+
+struct fmt2d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool have_model = false;
+  HostModel m;
+  // model on the device
+  DBuf<double> xg, zg, rvel[NREG + 1], ifx[NREG + 1][2], ifz[NREG + 1][2];
+  DBuf<uint16_t> rflags[NREG + 1];
+  DBuf<int> ifcc[NREG + 1][2];
+  DBuf<uint8_t> ifok[NREG + 1][2];
+  DBuf<RegionDev> regs;
+  DBuf<double> isn[NIF + 1][4];  // nx, nz, vel_top, vel_bot
+  DBuf<IsecDev> isecs;
+  DBuf<int> hw_slot;
+  DBuf<double> hw_vhw;
+  DBuf<HeadwaveMap> hw_map;
+  // fields of the last forward
+  int nsrc = 0, nfield = 0;
+  std::vector<int> field_of;           // [isrc*12 + itf-1] -> field index
+  DBuf<double> fT, isT, fGX, fGZ, fOGX, fOGZ, isGX, isGZ;
+  DBuf<uint8_t> fTIE;
+  DBuf<FieldDesc> descs;
+  DBuf<StageLink> links;
+  DBuf<unsigned int> tin;             // tile in active set
+  DBuf<unsigned long long> tkey;      // tile arrival key
+  DBuf<unsigned long long> tfront;    // per-field front key
+  DBuf<int2> tlist[3];                // active set, its next state, due tiles
+  DBuf<int> tcount;                   // [due, next]
+  double window_tiles = 2.0;          // window in tile crossing times
+  int max_sweeps = 64;
+  int colors = 4;                     // 4: four-colour Gauss-Seidel in a tile, 1: Jacobi
+  DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
+  DBuf<int> sel_which, sel_up;
+  DBuf<double> hw_scratch;
+  // heap-exhaustion stop work arrays
+  DBuf<ExitField> x_ef;
+  DBuf<int> x_cnt, x_diff, x_off, x_total;
+  DBuf<unsigned long long> x_max, x_m;
+  DBuf<ExitEntry> x_ent;
+  // ---- rays and Frechet rows
+  std::vector<double> src_x, src_z;    // sources of the last forward (on the seafloor)
+  DBuf<double> d_srcx, d_srcz;
+  bool rgeom_ok = false;               // ray geometry built for the current model
+  RayGeomDev hgeom{};
+  DBuf<RayGeomDev> rgeom;
+  DBuf<int> rorder[NREG + 1];
+  DBuf<double> risx[NIF + 1], risz[NIF + 1], rifx_in[NIF + 1];
+  DBuf<double> rsfx, rsfz, rpvel, rvx, rvz;
+  DBuf<int> rinv;
+  DBuf<RayField> rfields;
+  int nray = 0;
+  std::vector<RayJob> hjobs;
+  std::vector<long long> hpoff;
+  DBuf<RayJob> rjobs;
+  DBuf<RayOut> rout;
+  DBuf<long long> rpoff;
+  DBuf<double2> rpts;
+  DBuf<double> rtsf;
+  // inversion parameterisation (frechet.in): velocity grid and interfaces
+  int inv_vel = 1;
+  std::vector<int> inv_ifaces;
+  int n_vel_par = 0, n_par = 0;
+  int* h_count = nullptr;              // pinned
+  std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
+  fmt2d_forward_stats stats{};
+  std::vector<int> trace;              // active tiles per launch (all levels)
+  ~fmt2d_ctx() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (h_count) (void)hipHostFree(h_count);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+template <typename T>
+static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.alloc(v.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+

@@ -1,0 +1,802 @@
+// Ray backtracing and Frechet contributions, one thread per ray.
+//
+// Every routine follows its reference counterpart operation for operation in
+// FP64 (this file is compiled with -ffp-contract=off), so that the discrete
+// decisions of the tracer (interface hits, step halvings, source snapping)
+// are taken exactly as the reference takes them.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "eik_common.h"
+#include "rays.h"
+
+namespace fmt2d {
+
+#define RAY_HUGE 1.0e20
+
+// path tables of invert2d_main.f90:564-681: stage fields per section and the
+// intersection sequence (istart of section n = seq[2n-1], iend = seq[2n-2])
+__constant__ int c_path_ntf[6] = {0, 1, 2, 2, 4, 4};
+__constant__ int c_path_tf[6][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {2, 3, 0, 0}, {4, 5, 0, 0}, {1, 6, 7, 8}, {9, 10, 11, 12}};
+__constant__ int c_path_seq[6][8] = {{0}, {0, 2}, {0, 3, 3, 0}, {0, 4, 4, 0}, {0, 2, 2, 1, 1, 2, 2, 2},
+                                     {0, 1, 1, 2, 2, 2, 2, 2}};
+__constant__ int c_tf_region[13] = {0, 5, 2, 2, 6, 6, 1, 1, 1, 1, 1, 5, 1};
+
+__device__ inline int ifloor(double v) {
+  const double f = floor(v);
+  return f < -1.0e9 ? -1000000000 : (f > 1.0e9 ? 1000000000 : (int)f);
+}
+__device__ inline double clamp01(double w) { return w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w); }
+
+// ---------------------------------------------------------------- region lookups
+
+// region_rnode_id(ix,iz) (1-based): field slot of the grid node, -1 outside
+__device__ inline int rnode(const RayGeomDev& G, const RayRegionDev& R, int ix, int iz) {
+  if (ix < 1 || ix > G.nx || iz < 1 || iz > G.nz) return -1;
+  const int q = (iz - 1) * G.nx + (ix - 1);
+  return (R.flags[q] & RF_IN) ? q : -1;
+}
+
+// find_nearby_region_node (rays2d.f90:1010-1034)
+__device__ inline int rnode_nearby(const RayGeomDev& G, const RayRegionDev& R, int ix, int iz) {
+  for (int d = 1; d <= 5; d++) {
+    int n = rnode(G, R, ix, iz + d);
+    if (n >= 0) return n;
+    n = rnode(G, R, ix, iz - d);
+    if (n >= 0) return n;
+  }
+  return -1;
+}
+
+__device__ inline void slot_xz(const RayGeomDev& G, const RayRegionDev& R, int q, double& x, double& z) {
+  if (q < G.nz * G.nx) { x = G.xg[q % G.nx]; z = G.zg[q / G.nx]; return; }
+  const int s = q / G.nx - G.nz, c = q % G.nx, k = s == 0 ? R.itop : R.ibot;
+  x = G.isx[k][c]; z = G.isz[k][c];
+}
+
+// nearest region node over the whole node list, first minimum in node order
+__device__ int nearest_any(const RayGeomDev& G, const RayRegionDev& R, double px, double pz, double dmin0) {
+  double dmin = dmin0;
+  int best = -1;
+  for (int k = 0; k < R.nnode; k++) {
+    const int q = R.order[k];
+    double x, z;
+    slot_xz(G, R, q, x, z);
+    const double d = (px - x) * (px - x) + (pz - z) * (pz - z);
+    if (d < dmin) { dmin = d; best = q; }
+  }
+  return best;
+}
+
+// find_position_in_region2d (rays2d.f90:1076-1131)
+__device__ int position_in_region(const RayGeomDev& G, const RayRegionDev& R, double px, double pz) {
+  int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= G.nx) ix = G.nx - 1;
+  if (iz < 1) iz = 1;
+  if (iz >= G.nz) iz = G.nz - 1;
+  double distmin = 1.0e100;
+  int inode = -1;
+  for (int ii = ix; ii <= ix + 1; ii++)
+    for (int jj = iz; jj <= iz + 1; jj++) {
+      const int m = rnode(G, R, ii, jj);
+      if (m < 0) continue;
+      double x, z;
+      slot_xz(G, R, m, x, z);
+      const double dist = (px - x) * (px - x) + (pz - z) * (pz - z);
+      if (dist < distmin) { inode = m; distmin = dist; }
+    }
+  if (inode < 0) inode = nearest_any(G, R, px, pz, 1.0e100);
+  return inode;
+}
+
+// interpolate_time_gradient2d (rays2d.f90:825-894)
+__device__ void interp_grad(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz,
+                            double& dtdx, double& dtdz) {
+  int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= G.nx) ix = G.nx - 1;
+  if (iz < 1) iz = 1;
+  if (iz >= G.nz) iz = G.nz - 1;
+  const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
+  int n1 = rnode(G, R, ix, iz), n2 = rnode(G, R, ix + 1, iz), n3 = rnode(G, R, ix, iz + 1),
+      n4 = rnode(G, R, ix + 1, iz + 1);
+  if (n1 < 0) n1 = rnode_nearby(G, R, ix, iz);
+  if (n2 < 0) n2 = rnode_nearby(G, R, ix + 1, iz);
+  if (n3 < 0) n3 = rnode_nearby(G, R, ix, iz + 1);
+  if (n4 < 0) n4 = rnode_nearby(G, R, ix + 1, iz + 1);
+  const double g1x = n1 >= 0 ? F.GX[n1] : 0.0, g1z = n1 >= 0 ? F.GZ[n1] : 0.0;
+  const double g2x = n2 >= 0 ? F.GX[n2] : 0.0, g2z = n2 >= 0 ? F.GZ[n2] : 0.0;
+  const double g3x = n3 >= 0 ? F.GX[n3] : 0.0, g3z = n3 >= 0 ? F.GZ[n3] : 0.0;
+  const double g4x = n4 >= 0 ? F.GX[n4] : 0.0, g4z = n4 >= 0 ? F.GZ[n4] : 0.0;
+  double gx = (1.0 - w1) * (1.0 - w2) * g1x + w1 * (1.0 - w2) * g2x + (1.0 - w1) * w2 * g3x + w1 * w2 * g4x;
+  double gz = (1.0 - w1) * (1.0 - w2) * g1z + w1 * (1.0 - w2) * g2z + (1.0 - w1) * w2 * g3z + w1 * w2 * g4z;
+  if (sqrt(gx * gx + gz * gz) < 0.01) {
+    // interpolate_gradient_reg (rays2d.f90:1036-1073): the nearest node's gradient
+    const int m = position_in_region(G, R, px, pz);
+    gx = m >= 0 ? F.GX[m] : 0.0;
+    gz = m >= 0 ? F.GZ[m] : 0.0;
+  }
+  dtdx = gx;
+  dtdz = gz;
+}
+
+// interpolate_arrivaltime2d (rays2d.f90:898-958)
+__device__ double interp_time(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz) {
+  int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= G.nx) ix = G.nx - 1;
+  if (iz < 1) iz = 1;
+  if (iz >= G.nz) iz = G.nz - 1;
+  const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
+  const int n1 = rnode(G, R, ix, iz), n2 = rnode(G, R, ix + 1, iz), n3 = rnode(G, R, ix, iz + 1),
+            n4 = rnode(G, R, ix + 1, iz + 1);
+  double t1 = RAY_HUGE, t2 = RAY_HUGE, t3 = RAY_HUGE, t4 = RAY_HUGE;
+  if (n1 >= 0) t1 = F.T[n1];
+  if (n2 >= 0) t2 = F.T[n2];
+  if (n3 >= 0) t3 = F.T[n3];
+  if (n4 >= 0) t4 = F.T[n4];
+  if (n1 < 0) { const int m = rnode_nearby(G, R, ix, iz); if (m >= 0) t1 = F.T[m]; }
+  if (n2 < 0) { const int m = rnode_nearby(G, R, ix + 1, iz); if (m >= 0) t2 = F.T[m]; }
+  if (n3 < 0) { const int m = rnode_nearby(G, R, ix, iz + 1); if (m >= 0) t3 = F.T[m]; }
+  if (n4 < 0) { const int m = rnode_nearby(G, R, ix + 1, iz + 1); if (m >= 0) t4 = F.T[m]; }
+  const double half = RAY_HUGE * 0.5;
+  if (t1 < half && t2 < half && t3 < half && t4 < half)
+    return (1.0 - w1) * (1.0 - w2) * t1 + w1 * (1.0 - w2) * t2 + (1.0 - w1) * w2 * t3 + w1 * w2 * t4;
+  // interpolate_arrivaltime_reg (rays2d.f90:960-976): nearest node's time
+  const int m = nearest_any(G, R, px, pz, RAY_HUGE);
+  return m >= 0 ? F.T[m] : RAY_HUGE;
+}
+
+// interpolate_interface2d (rays2d.f90:1228-1262) on intersection k
+__device__ double iface_z(const RayGeomDev& G, int k, double px) {
+  const int n = G.isn[k];
+  const double* x = G.isx[k];
+  const double* z = G.isz[k];
+  if (n < 1) return 0.0;
+  if (n < 2) return z[0];
+  const int ix = ifloor((px - x[0]) / G.dx) + 1;
+  if (ix < 1) return z[0];
+  if (ix >= n) return z[n - 1];
+  const double w = clamp01((px - x[ix - 1]) / G.dx);
+  return (1.0 - w) * z[ix - 1] + w * z[ix];
+}
+
+// interface_normal2d (rays2d.f90:1266-1308)
+__device__ void iface_normal(const RayGeomDev& G, int k, double px, double& nx_, double& nz_, double& h) {
+  const int n = G.isn[k];
+  const double* x = G.isx[k];
+  const double* z = G.isz[k];
+  if (n < 2) { nx_ = 0.0; nz_ = 1.0; h = z[0]; return; }
+  int ix = ifloor((px - x[0]) / G.dx) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= n) ix = n - 1;
+  const double w = clamp01((px - x[ix - 1]) / G.dx);
+  h = (1.0 - w) * z[ix - 1] + w * z[ix];
+  double dzdx;
+  if (ix > 1 && ix < n - 1) dzdx = (z[ix] - z[ix - 2]) / (2.0 * G.dx);
+  else if (ix == 1 && n >= 3) dzdx = (z[ix] - z[ix - 1]) / G.dx;
+  else if (ix == n - 1 && n >= 3) dzdx = (z[ix] - z[ix - 2]) / (2.0 * G.dx);
+  else dzdx = 0.0;
+  const double nl = sqrt(1.0 + dzdx * dzdx);
+  nx_ = -dzdx / nl;
+  nz_ = 1.0 / nl;
+}
+
+// interpolate_iface_z_inv (inversion2d.f90:6315-6341) on the seafloor as read
+__device__ double seafloor_z(const RayGeomDev& G, double px) {
+  if (G.nsf < 2) return G.sfz[0];
+  const int i = ifloor((px - G.sfx[0]) / G.sfdx) + 1;
+  if (i < 1) return G.sfz[0];
+  if (i >= G.nsf) return G.sfz[G.nsf - 1];
+  const double w = clamp01((px - G.sfx[i - 1]) / G.sfdx);
+  return (1.0 - w) * G.sfz[i - 1] + w * G.sfz[i];
+}
+
+// get_velocity_at2d (rays2d.f90:1312-1348) on the propagation grid
+__device__ double velocity_at(const RayGeomDev& G, double px, double pz) {
+  int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= G.nx) ix = G.nx - 1;
+  if (iz < 1) iz = 1;
+  if (iz >= G.nz) iz = G.nz - 1;
+  const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
+  const double* V = G.pvel;
+  const int q = (iz - 1) * G.nx + (ix - 1);
+  return (1.0 - w1) * (1.0 - w2) * V[q] + w1 * (1.0 - w2) * V[q + 1] + (1.0 - w1) * w2 * V[q + G.nx] +
+         w1 * w2 * V[q + G.nx + 1];
+}
+
+// best_diving_return_node2d (rays2d.f90:725-760)
+__device__ int diving_return_node(const RayGeomDev& G, const RayRegionDev& R, const RayField& F,
+                                  const double* rvel, double x_if, double z_if, double min_depth) {
+  int best = -1;
+  double best_t = RAY_HUGE;
+  for (int k = 0; k < R.nnode; k++) {
+    const int q = R.order[k];
+    if (F.T[q] >= RAY_HUGE * 0.5) continue;
+    double x, z;
+    slot_xz(G, R, q, x, z);
+    if (z <= z_if + min_depth) continue;
+    if (fabs(x - x_if) > 3.0 * min_depth) continue;
+    const double dx = x - x_if, dz = z - z_if, dist = sqrt(dx * dx + dz * dz);
+    if (dist <= 1.0e-10) continue;
+    double vloc = rvel[q];
+    if (vloc < 0.01) vloc = 0.01;
+    const double cand = F.T[q] + dist / vloc;
+    if (cand < best_t) { best_t = cand; best = q; }
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------- sections
+
+struct RayState {
+  bool valid, headwave, diffracted;
+};
+
+struct SecInfo {
+  int seqnum, nsec, path;
+  const RayField* F;        // this section's field
+  const RayRegionDev* R;
+  const RayField* Fn;       // field of section seqnum-1 (toward the source), or null
+  const RayRegionDev* Rn;
+  int istart, iend;         // intersection ids, 0 none
+  int next_reg;             // region id of section seqnum+1 (0 none)
+  int prev_reg;             // region id of section seqnum-1 (0 none)
+  const double* rvel;       // region node velocities of this section's region
+};
+
+// find_interface_crossing2d (rays2d.f90:764-821); false = no crossing
+__device__ bool interface_crossing(const RayGeomDev& G, double2* pt, int n, const double pp[2], int ifend,
+                                   const double grad0[2], double dl) {
+  const double tol = G.dx < G.dz ? G.dx * 0.01 : G.dz * 0.01;  // grid%tolerance
+  const double zi = iface_z(G, ifend, pp[0]);
+  const double x1 = 0.0, x2 = dl;
+  const double f = pt[n - 2].y - iface_z(G, ifend, pt[n - 2].x);
+  double fmid = pp[1] - zi;
+  if (f * fmid > 0.01 * tol) return false;
+  if (fabs(f * fmid) <= 0.01 * tol) {
+    pt[n - 1] = pt[n - 2];
+    pt[n - 1].y = iface_z(G, ifend, pt[n - 1].x);
+    return true;
+  }
+  if (f * fmid < -0.01 * tol) {
+    double rtbis, dxb;
+    if (f < 0.0) { rtbis = x1; dxb = x2 - x1; } else { rtbis = x2; dxb = x1 - x2; }
+    for (int it = 0; it < 10; it++) {
+      dxb = dxb * 0.5;
+      const double xmid = rtbis + dxb;
+      pt[n - 1].x = pt[n - 2].x - xmid * grad0[0];
+      pt[n - 1].y = pt[n - 2].y - xmid * grad0[1];
+      fmid = pt[n - 1].y - iface_z(G, ifend, pt[n - 1].x);
+      if (fmid <= 0.0) rtbis = xmid;
+    }
+    pt[n - 1].x = pt[n - 2].x - rtbis * grad0[0];
+    pt[n - 1].y = pt[n - 2].y - rtbis * grad0[1];
+    pt[n - 1].y = iface_z(G, ifend, pt[n - 1].x);
+  }
+  return true;
+}
+
+// Headwave / refraction decision at the section's ending interface
+// (rays2d.f90:451-455 and repeats): true = the ray refracts (section ends).
+__device__ bool refracts(const RayGeomDev& G, const SecInfo& S, int ifend, double px, double zi, double grad0[2],
+                         RayState& st) {
+  double nx_, nz_, h, g1[2];
+  iface_normal(G, ifend, px, nx_, nz_, h);
+  interp_grad(G, *S.R, *S.F, px, zi, grad0[0], grad0[1]);
+  interp_grad(G, *S.Rn, *S.Fn, px, zi, g1[0], g1[1]);
+  const double dn = nx_ * g1[0] + nz_ * g1[1];
+  const double a = g1[0] - dn * nx_, b = g1[1] - dn * nz_;
+  return sqrt(a * a + b * b) <= sqrt(grad0[0] * grad0[0] + grad0[1] * grad0[1]);
+}
+
+// trace_ray_section2d (rays2d.f90:157-720).  pt holds RAY_MAXP points; on
+// return *npts is the section's point count (0 when invalid before any).
+__device__ void trace_section(const RayGeomDev& G, const SecInfo& S, double sx, double sz, double x_in, double z_in,
+                              double2* pt, int* npts, double& x_out, double& z_out, RayState& st) {
+  const RayRegionDev& R = *S.R;
+  const int iftop = R.itop, ifbot = R.ibot;
+  const int seqnum = S.seqnum;
+  const double v_average = 5.0;
+  int ifend = 0, start_direction = 0;
+  bool section_is_refraction = false, same_internal_ifend = false, left_start_interface = true;
+  if (seqnum < S.nsec && seqnum > 1) {
+    ifend = S.iend;
+    start_direction = R.id - S.next_reg;
+    section_is_refraction = S.prev_reg != R.id;
+  }
+  if (seqnum == S.nsec && seqnum > 1) {
+    ifend = S.iend;
+    start_direction = 0;
+    section_is_refraction = S.prev_reg != R.id;
+  }
+  if (seqnum == 1) {
+    ifend = 0;
+    start_direction = (seqnum + 1 <= S.nsec) ? R.id - S.next_reg : 0;
+  }
+  if (ifend && S.istart) {
+    same_internal_ifend = ifend == S.istart && ifend != iftop && ifend != ifbot;
+    if (same_internal_ifend) left_start_interface = false;
+  }
+  const double dlmax = 0.1 * G.dx * v_average;
+  const double p_tol = 5.0e-4 * G.dx;
+  const double s_tol = 2.5 * G.dx;
+  int n = 1;
+  pt[0] = make_double2(x_in, z_in);
+  double dist_headwave = 0.0;
+  double zi = iface_z(G, ifbot, x_in);
+  const bool start_on_bot = fabs(z_in - zi) < 0.1 * G.dz;
+  zi = iface_z(G, iftop, x_in);
+  const bool start_on_top = fabs(z_in - zi) < 0.1 * G.dz;
+  // converted diving phases start the deep section from the interior node
+  // that seeded the return (rays2d.f90:234-264)
+  if (seqnum == 2 && start_on_top && S.istart && S.iend && S.istart == iftop && S.iend == iftop &&
+      (S.path == 5 || S.path == 6 || S.path == 8)) {
+    const int seed = diving_return_node(G, R, *S.F, S.rvel, x_in, z_in, 2.0 * G.dz);
+    if (seed >= 0) {
+      double qx, qz;
+      slot_xz(G, R, seed, qx, qz);
+      const double seed_dist = sqrt((qx - x_in) * (qx - x_in) + (qz - z_in) * (qz - z_in));
+      if (seed_dist > 1.0e-10) {
+        int nseed = (int)ceil(seed_dist / fmax(dlmax, 1.0e-10));
+        if (nseed < 1) nseed = 1;
+        if (nseed > RAY_MAXP - 2) nseed = RAY_MAXP - 2;
+        for (int k = 1; k <= nseed; k++) {
+          const double frac = (double)k / (double)nseed;
+          pt[k] = make_double2(x_in + frac * (qx - x_in), z_in + frac * (qz - z_in));
+        }
+        n = 1 + nseed;
+      }
+    }
+  }
+  double grad0[2], grad1[2], pp[2];
+  double dl = dlmax;
+  auto snap_source = [&](int m) -> bool {  // point m (1-based) within s_tol of the source
+    const double dd = sqrt((pt[m - 1].x - sx) * (pt[m - 1].x - sx) + (pt[m - 1].y - sz) * (pt[m - 1].y - sz));
+    if (dd < s_tol) { pt[m - 1] = make_double2(sx, sz); return true; }
+    return false;
+  };
+  bool exited = false;  // left ploop through an ending interface
+  while (true) {  // ploop
+    n = n + 1;
+    if (n > RAY_MAXP) { st.valid = false; *npts = 0; return; }
+    if (seqnum == 1 && n > 2) {
+      const double dd = sqrt((pt[n - 2].x - sx) * (pt[n - 2].x - sx) + (pt[n - 2].y - sz) * (pt[n - 2].y - sz));
+      if (dd < s_tol) { pt[n - 1] = make_double2(sx, sz); *npts = n; return; }
+    }
+    if (st.headwave || st.diffracted) {
+      if (n > 2) {
+        const double ax = pt[n - 2].x - pt[n - 3].x, az = pt[n - 2].y - pt[n - 3].y;
+        dist_headwave = dist_headwave + sqrt(ax * ax + az * az);
+      }
+      if (dist_headwave > 200.0) { st.valid = false; *npts = 0; return; }
+    }
+    dl = dlmax;
+    if (n == 2 && (start_on_bot || start_on_top)) dl = 10.0 * dlmax;
+    interp_grad(G, R, *S.F, pt[n - 2].x, pt[n - 2].y, grad0[0], grad0[1]);
+    pp[0] = pt[n - 2].x - dl * grad0[0];
+    pp[1] = pt[n - 2].y - dl * grad0[1];
+    if (start_on_bot && pp[1] > pt[n - 2].y) pp[1] = 2.0 * pt[n - 2].y - pp[1];
+    if (start_on_top && pp[1] < pt[n - 2].y) pp[1] = 2.0 * pt[n - 2].y - pp[1];
+    if (n == 2 && start_direction != 0 && !section_is_refraction && seqnum != 1) {
+      double nx_, nz_, h;
+      iface_normal(G, S.istart, x_in, nx_, nz_, h);
+      if (grad0[0] * (start_direction * nx_) + grad0[1] * (start_direction * nz_) > 0.0) {
+        st.valid = false; *npts = 0; return;
+      }
+    }
+    // ending interface inside the region (rays2d.f90:400-434)
+    if (ifend && ifend != iftop && ifend != ifbot) {
+      const double zp = iface_z(G, ifend, pt[n - 2].x), zn = iface_z(G, ifend, pp[0]);
+      const double fp = pt[n - 2].y - zp, fn = pp[1] - zn;
+      if (same_internal_ifend && !left_start_interface) {
+        if (fabs(fn) > 0.5 * G.dz) left_start_interface = true;
+      } else if (fabs(fp) > 1.0e-3) {
+        if (fp * fn <= 0.0) {
+          double frac = fabs(fp - fn) > 1.0e-12 ? fp / (fp - fn) : 1.0;
+          frac = clamp01(frac);
+          pt[n - 1].x = pt[n - 2].x + frac * (pp[0] - pt[n - 2].x);
+          pt[n - 1].y = iface_z(G, ifend, pt[n - 1].x);
+          if (seqnum == 1) snap_source(n);
+          *npts = n;
+          return;
+        }
+      }
+    }
+    // top interface reached by the predictor (rays2d.f90:436-496)
+    zi = iface_z(G, iftop, pp[0]);
+    if (pp[1] <= zi) {
+      if (iftop == ifend) {
+        if (section_is_refraction) {
+          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited = true; break; }
+          pt[n - 1] = make_double2(pp[0], zi);
+          st.headwave = true;
+          if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+          continue;
+        }
+        exited = true;
+        break;
+      } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
+        // leave the start interface
+      } else {
+        pt[n - 1] = make_double2(pp[0], zi);
+        st.diffracted = true;
+        if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+        continue;
+      }
+    }
+    // bottom interface reached by the predictor (rays2d.f90:498-562)
+    zi = iface_z(G, ifbot, pp[0]);
+    if (pp[1] >= zi) {
+      if (ifbot == ifend) {
+        if (section_is_refraction) {
+          if (S.path == 4 && seqnum == 2) { exited = true; break; }
+          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited = true; break; }
+          pt[n - 1] = make_double2(pp[0], zi);
+          st.headwave = true;
+          if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+          continue;
+        }
+        exited = true;
+        break;
+      } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
+      } else {
+        pt[n - 1] = make_double2(pp[0], zi);
+        st.diffracted = true;
+        if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+        continue;
+      }
+    }
+    // corrector with step halving (rays2d.f90:564-668)
+    int iter = 1;
+    bool next_ploop = false;
+    while (true) {
+      pp[0] = pt[n - 2].x - dl * grad0[0];
+      pp[1] = pt[n - 2].y - dl * grad0[1];
+      interp_grad(G, R, *S.F, pp[0], pp[1], grad1[0], grad1[1]);
+      pt[n - 1].x = pt[n - 2].x - dl * (grad0[0] + grad1[0]) * 0.5;
+      pt[n - 1].y = pt[n - 2].y - dl * (grad0[1] + grad1[1]) * 0.5;
+      zi = iface_z(G, iftop, pt[n - 1].x);
+      if (pt[n - 1].y <= zi) {
+        if (iftop == ifend) {
+          if (section_is_refraction) {
+            if (refracts(G, S, ifend, pt[n - 1].x, zi, grad0, st)) {
+              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
+              exited = true;
+              break;
+            }
+            pt[n - 1].y = zi;
+            st.headwave = true;
+            next_ploop = true;
+            break;
+          }
+          pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
+          exited = true;
+          break;
+        } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
+        } else {
+          pt[n - 1].y = zi;
+          st.diffracted = true;
+          next_ploop = true;
+          break;
+        }
+      }
+      zi = iface_z(G, ifbot, pt[n - 1].x);
+      if (pt[n - 1].y >= zi) {
+        if (ifbot == ifend) {
+          if (section_is_refraction) {
+            if (S.path == 4 && seqnum == 2) {
+              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
+              exited = true;
+              break;
+            }
+            if (refracts(G, S, ifend, pt[n - 1].x, zi, grad0, st)) {
+              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
+              exited = true;
+              break;
+            }
+            pt[n - 1].y = zi;
+            st.headwave = true;
+            next_ploop = true;
+            break;
+          }
+          pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
+          exited = true;
+          break;
+        } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
+        } else {
+          pt[n - 1].y = zi;
+          st.diffracted = true;
+          next_ploop = true;
+          break;
+        }
+      }
+      const double ex = pt[n - 1].x - pp[0], ez = pt[n - 1].y - pp[1];
+      if (sqrt(ex * ex + ez * ez) < p_tol) break;
+      iter = iter + 1;
+      if (iter > 10) { st.valid = false; *npts = 0; return; }
+      dl = 0.5 * dl;
+    }
+    if (exited) break;
+    if (next_ploop) continue;
+    if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+  }
+  // left ploop through the ending interface
+  if (seqnum == 1) { st.valid = false; *npts = 0; return; }
+  if (ifend) {
+    if (!interface_crossing(G, pt, n, pp, ifend, grad0, dl)) { st.valid = false; *npts = 0; return; }
+  }
+  x_out = pt[n - 1].x;
+  z_out = pt[n - 1].y;
+  *npts = n;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Arrival time of each path's last stage field at every seafloor node, per
+// source (the first sweep of find_optimal_entry_inv): [src][3][nsf] for
+// tf1, tf3, tf5.
+__global__ void seafloor_times_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
+                                      int nsrc, double* tsf) {
+  const RayGeomDev& G = *Gp;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nsrc * 3 * G.nsf;
+  if (k >= total) return;
+  const int i = (int)(k % G.nsf), w = (int)((k / G.nsf) % 3), s = (int)(k / G.nsf / 3);
+  const int tf = w == 0 ? 1 : (w == 1 ? 3 : 5);
+  const RayField& F = fields[s * 12 + tf - 1];
+  tsf[k] = interp_time(G, G.reg[F.region], F, G.sfx[i], G.sfz[i]);
+}
+
+__global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
+                             const double* __restrict__ src_x, const double* __restrict__ src_z,
+                             const double* __restrict__ tsf, const RayJob* __restrict__ jobs, int nray,
+                             const long long* __restrict__ pt_off, double2* pts, RayOut* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nray) return;
+  const RayGeomDev& G = *Gp;
+  const RayJob J = jobs[r];
+  RayOut o;
+  o.t = 0.0; o.water_time = 0.0; o.valid = 1; o.nsec = c_path_ntf[J.path];
+  for (int k = 0; k < RAY_MAXSEC; k++) o.npts[k] = 0;
+  o.headwave = 0; o.diffracted = 0;
+  const int nsec = o.nsec, s = J.src;
+  const double sx = src_x[s], sz = src_z[s];
+  // receiver placement (invert2d_main.f90:517-531, inversion2d.f90:4502-4519)
+  double rx = J.rx, rz = J.rz, wt = 0.0;
+  const double zsf = seafloor_z(G, J.rx);
+  if (J.rz < zsf) {
+    if (J.path == 4 || J.path == 5) {
+      rx = J.rx; rz = J.rz; wt = 0.0;
+    } else {
+      // find_optimal_entry_inv (inversion2d.f90:6344-6438)
+      const int itf = c_path_tf[J.path][nsec - 1];
+      const RayField& F = fields[s * 12 + itf - 1];
+      const RayRegionDev& R = G.reg[F.region];
+      const double* ts = tsf + ((long long)s * 3 + (itf == 1 ? 0 : (itf == 3 ? 1 : 2))) * G.nsf;
+      double t_best = RAY_HUGE;
+      const double z0 = seafloor_z(G, J.rx);
+      double xb = J.rx, zb = z0, wb = (z0 - J.rz) / G.vwater;
+      for (int i = 0; i < G.nsf; i++) {
+        const double tf_ = ts[i];
+        if (tf_ > RAY_HUGE * 0.5) continue;
+        const double ddx = G.sfx[i] - J.rx, ddz = G.sfz[i] - J.rz;
+        const double tw = sqrt(ddx * ddx + ddz * ddz) / G.vwater;
+        const double tt = tf_ + tw;
+        if (tt < t_best) { t_best = tt; xb = G.sfx[i]; zb = G.sfz[i]; wb = tw; }
+      }
+      if (!(xb <= G.sfx[0] || xb >= G.sfx[G.nsf - 1])) {
+        double xl = 0.0, xr = 0.0;
+        for (int i = 0; i < G.nsf - 1; i++)
+          if (xb >= G.sfx[i] && xb <= G.sfx[i + 1]) { xl = G.sfx[i]; xr = G.sfx[i + 1]; break; }
+        const double dxr = (xr - xl) / 10.0;
+        for (int k = 1; k <= 9; k++) {
+          const double xt = xl + (double)k * dxr;
+          const double zt = seafloor_z(G, xt);
+          const double tf_ = interp_time(G, R, F, xt, zt);
+          if (tf_ > RAY_HUGE * 0.5) continue;
+          const double ddx = xt - J.rx, ddz = zt - J.rz;
+          const double tw = sqrt(ddx * ddx + ddz * ddz) / G.vwater;
+          const double tt = tf_ + tw;
+          if (tt < t_best) { t_best = tt; xb = xt; zb = zt; wb = tw; }
+        }
+      }
+      rx = xb; rz = zb; wt = wb;
+    }
+  }
+  o.ex = rx; o.ez = rz; o.water_time = wt;
+  // trace_ray_from_receiver2d (rays2d.f90:7-153)
+  RayState st{true, false, false};
+  double x_in = rx, z_in = rz, x_out = 0.0, z_out = 0.0;
+  double2* P = pts + pt_off[r];
+  for (int n = nsec; n >= 1; n--) {
+    const int tf = c_path_tf[J.path][n - 1];
+    SecInfo S;
+    S.seqnum = n; S.nsec = nsec; S.path = J.path;
+    S.F = &fields[s * 12 + tf - 1];
+    S.R = &G.reg[S.F->region];
+    S.rvel = G.reg[S.F->region].vel;
+    S.istart = n < nsec ? c_path_seq[J.path][2 * n - 1] : 0;
+    S.iend = n > 1 ? c_path_seq[J.path][2 * n - 2] : 0;
+    S.next_reg = n < nsec ? c_tf_region[c_path_tf[J.path][n]] : 0;
+    if (n > 1) {
+      const int tfn = c_path_tf[J.path][n - 2];
+      S.Fn = &fields[s * 12 + tfn - 1];
+      S.Rn = &G.reg[S.Fn->region];
+      S.prev_reg = c_tf_region[tfn];
+    } else {
+      S.Fn = nullptr; S.Rn = nullptr; S.prev_reg = 0;
+    }
+    if (n == nsec) {
+      const double tr = interp_time(G, *S.R, *S.F, rx, rz);
+      double gx, gz;
+      interp_grad(G, *S.R, *S.F, rx, rz, gx, gz);
+      o.t = tr;
+      if (tr > 1.0e10 || (gx == 0.0 && gz == 0.0)) { st.valid = false; break; }
+    }
+    int np = 0;
+    trace_section(G, S, sx, sz, x_in, z_in, P + (long long)(n - 1) * RAY_MAXP, &np, x_out, z_out, st);
+    o.npts[n - 1] = np;
+    if (!st.valid) break;
+    if (n > 1) { x_in = x_out; z_in = z_out; }
+  }
+  o.valid = st.valid ? 1 : 0;
+  if (o.valid) o.t = o.t + wt;
+  o.headwave = st.headwave; o.diffracted = st.diffracted;
+  out[r] = o;
+}
+
+// Frechet contributions of one ray in the order compute_frechet_for_ray2d
+// accumulates them (inversion2d.f90:4580-4757): velocity weights of every
+// segment midpoint outside the water, sections 1..n, then the interface
+// partials at the section starts, sections n..1.
+template <bool WRITE>
+__device__ long long ray_contrib(const RayGeomDev& G, const RayField* fields, const RayJob& J, int rloc,
+                                 const double2* P, const RayOut& o, unsigned long long* keys, double* vals) {
+  long long cnt = 0;
+  auto emit = [&](int idx1, double v) {  // idx1: 1-based parameter
+    if (WRITE) { keys[cnt] = ((unsigned long long)rloc << 32) | (unsigned)(idx1 - 1); vals[cnt] = v; }
+    cnt++;
+  };
+  const int nsec = o.nsec;
+  for (int si = 1; si <= nsec; si++) {
+    const int np = o.npts[si - 1];
+    if (np <= 1) continue;
+    const int tf = c_path_tf[J.path][si - 1];
+    if (c_tf_region[tf] == 1) continue;  // water sections carry no velocity sensitivity
+    const double2* p = P + (long long)(si - 1) * RAY_MAXP;
+    for (int k = 0; k < np - 1; k++) {
+      const double px = 0.5 * (p[k].x + p[k + 1].x), pz = 0.5 * (p[k].y + p[k + 1].y);
+      const double ax = p[k + 1].x - p[k].x, az = p[k + 1].y - p[k].y;
+      const double seg = sqrt(ax * ax + az * az);
+      if (seg < 1.0e-10) continue;
+      int ivx = ifloor((px - G.vx[0]) / G.vdx) + 1;
+      if (ivx < 1) ivx = 1;
+      if (ivx >= G.vnx) ivx = G.vnx - 1;
+      int ivz;
+      double wz;
+      if (pz < G.vz[0]) continue;
+      if (pz >= G.vz[G.vnz - 1]) { ivz = G.vnz - 1; wz = 1.0; }
+      else {
+        ivz = ifloor((pz - G.vz[0]) / G.vdz) + 1;
+        if (ivz < 1) ivz = 1;
+        if (ivz >= G.vnz) ivz = G.vnz - 1;
+        wz = (pz - G.vz[ivz - 1]) / G.vdz;
+      }
+      const double wx = clamp01((px - G.vx[ivx - 1]) / G.vdx);
+      wz = clamp01(wz);
+      const int* I = G.inv_idx + (long long)(ivz - 1) * G.vnx + (ivx - 1);
+      if (I[0] > 0) emit(I[0], (1.0 - wx) * (1.0 - wz) * seg);
+      if (I[1] > 0) emit(I[1], wx * (1.0 - wz) * seg);
+      if (I[G.vnx] > 0) emit(I[G.vnx], (1.0 - wx) * wz * seg);
+      if (I[G.vnx + 1] > 0) emit(I[G.vnx + 1], wx * wz * seg);
+    }
+  }
+  // interface partials
+  int prev = 0;  // section index of raysec_prev
+  bool rec_done = false, do_if = false;
+  for (int si = nsec; si >= 1; si--) {
+    const int np = o.npts[si - 1];
+    if (np <= 2) continue;
+    if (!rec_done) { prev = si; rec_done = true; } else do_if = true;
+    if (do_if) {
+      const int ist = si < nsec ? c_path_seq[J.path][2 * si - 1] : 0;
+      if (ist == 0) { prev = si; continue; }
+      if (G.if_start[ist] >= 0) {
+        const double2* p = P + (long long)(si - 1) * RAY_MAXP;
+        const double px = p[0].x, pz = p[0].y;
+        const int tf = c_path_tf[J.path][si - 1];
+        const int reg = c_tf_region[tf];
+        // the region holds the field of the first section (toward the
+        // source) that uses it when the Frechet row is built
+        int tl = tf;
+        for (int m = 1; m <= nsec; m++)
+          if (c_tf_region[c_path_tf[J.path][m - 1]] == reg) { tl = c_path_tf[J.path][m - 1]; break; }
+        const RayField& F = fields[J.src * 12 + tl - 1];
+        double gi[2];
+        interp_grad(G, G.reg[reg], F, px, pz, gi[0], gi[1]);
+        const double vin = velocity_at(G, px, pz);
+        const double nl = sqrt(gi[0] * gi[0] + gi[1] * gi[1]);
+        if (nl > 0.0) { gi[0] = gi[0] / (vin * nl); gi[1] = gi[1] / (vin * nl); }
+        double nx_, nz_, h;
+        iface_normal(G, ist, px, nx_, nz_, h);
+        const double perp = gi[0] * nx_ + gi[1] * nz_;
+        double out_perp;
+        const int prev_reg = c_tf_region[c_path_tf[J.path][prev - 1]];
+        if (reg == prev_reg) out_perp = -perp;
+        else {
+          const double a = gi[0] - perp * nx_, b = gi[1] - perp * nz_;
+          const double par = sqrt(a * a + b * b);
+          const double vout = velocity_at(G, px, pz);
+          double det = 1.0 / (vout * vout) - par * par;
+          if (det < 0.0) det = 0.0;
+          out_perp = copysign(sqrt(det), perp);
+        }
+        const double geo = (perp - out_perp) * nz_;
+        const int nif = G.if_nnode[ist];
+        int ivx = ifloor((px - G.ifx_in[ist][0]) / G.ifdx_in[ist]) + 1;
+        if (ivx < 1) ivx = 1;
+        if (ivx >= nif) ivx = nif - 1;
+        const double wx = clamp01((px - G.ifx_in[ist][ivx - 1]) / G.ifdx_in[ist]);
+        int idx = G.if_start[ist] + ivx - 1;
+        if (idx >= 1 && idx <= G.n_par) emit(idx + 1, (1.0 - wx) * geo);
+        idx = G.if_start[ist] + ivx;
+        if (idx >= 1 && idx <= G.n_par) emit(idx + 1, wx * geo);
+      }
+    }
+    prev = si;
+  }
+  return cnt;
+}
+
+__global__ void frechet_contrib_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
+                                       const RayJob* __restrict__ jobs, int nray, int ray0,
+                                       const long long* __restrict__ pt_off, const double2* __restrict__ pts,
+                                       const RayOut* __restrict__ out, const long long* __restrict__ coff,
+                                       long long* count, unsigned long long* keys, double* vals) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nray) return;
+  const int g = ray0 + r;
+  const RayOut o = out[g];
+  if (!o.valid) { if (!keys) count[r] = 0; return; }
+  const RayGeomDev& G = *Gp;
+  if (!keys) {
+    count[r] = ray_contrib<false>(G, fields, jobs[g], r, pts + pt_off[g], o, nullptr, nullptr);
+  } else {
+    ray_contrib<true>(G, fields, jobs[g], r, pts + pt_off[g], o, keys + coff[r], vals + coff[r]);
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, int nsrc, int nsf, double* tsf,
+                                 hipStream_t st) {
+  const long long total = (long long)nsrc * 3 * nsf;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(seafloor_times_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, G, fields, nsrc,
+                     tsf);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
+                        const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
+                        RayOut* out, hipStream_t st) {
+  if (nray == 0) return hipSuccess;
+  hipLaunchKernelGGL(trace_kernel, dim3((nray + 63) / 64), dim3(64), 0, st, G, fields, src_x, src_z, tsf, jobs, nray,
+                     pt_off, pts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_frechet_contrib(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, int nray,
+                                  int ray0, const long long* pt_off, const double2* pts, const RayOut* out,
+                                  const long long* coff, long long* count, unsigned long long* keys, double* vals,
+                                  hipStream_t st) {
+  if (nray == 0) return hipSuccess;
+  hipLaunchKernelGGL(frechet_contrib_kernel, dim3((nray + 63) / 64), dim3(64), 0, st, G, fields, jobs, nray, ray0,
+                     pt_off, pts, out, coff, count, keys, vals);
+  return hipGetLastError();
+}
+
+}  // namespace fmt2d

@@ -59,6 +59,12 @@ def load() -> C.CDLL:
     lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     lib.fmt2d_tf_region.argtypes = [C.c_int]
     lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+    lib.fmt2d_set_inversion.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
+    lib.fmt2d_inversion_info.argtypes = [C.c_void_p, _ip, _ip]
+    lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
+    lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
+    lib.fmt2d_ray_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip]
+    lib.fmt2d_frechet_csr.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int, _ip]
     _LIB = lib
     return lib
 
@@ -68,7 +74,9 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_set_tuning",
            "fmt2d_tf_region",
-           "fmt2d_get_field")
+           "fmt2d_get_field",
+           "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
+           "fmt2d_frechet_csr")
 
 
 def _check(rc: int):
@@ -129,6 +137,62 @@ class Context:
         out = np.zeros(max(n.value, 1), dtype=np.int32)
         _check(self._lib.fmt2d_forward_trace(self._h, out.ctypes.data_as(_ip), n.value, C.byref(n)))
         return out[:n.value]
+
+    # ---- rays and Frechet rows
+    def set_inversion(self, invert_velocity: bool = True, ifaces: Sequence[int] = ()):
+        ids = np.ascontiguousarray(ifaces, dtype=np.int32)
+        _check(self._lib.fmt2d_set_inversion(self._h, int(bool(invert_velocity)), len(ids),
+                                             ids.ctypes.data_as(_ip) if len(ids) else None))
+
+    def inversion_info(self):
+        a, b = C.c_int(), C.c_int()
+        _check(self._lib.fmt2d_inversion_info(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def trace(self, src, path, rx, rz):
+        """Rays of the last forward: returns (receiver times, valid flags)."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        path = np.ascontiguousarray(path, dtype=np.int32)
+        rx = np.ascontiguousarray(rx, dtype=np.float64)
+        rz = np.ascontiguousarray(rz, dtype=np.float64)
+        n = len(src)
+        self._nray = n
+        t = np.zeros(n)
+        v = np.zeros(n, dtype=np.int32)
+        _check(self._lib.fmt2d_trace(self._h, n, src.ctypes.data_as(_ip), path.ctypes.data_as(_ip), _p(rx), _p(rz),
+                                     _p(t), v.ctypes.data_as(_ip)))
+        return t, v.astype(bool)
+
+    def ray_info(self, iray: int):
+        e = np.zeros(2)
+        w = C.c_double()
+        npts = np.zeros(4, dtype=np.int32)
+        fl = np.zeros(2, dtype=np.int32)
+        _check(self._lib.fmt2d_ray_info(self._h, iray, _p(e), C.byref(w), npts.ctypes.data_as(_ip),
+                                        fl.ctypes.data_as(_ip)))
+        return {"entry": e, "water_time": w.value, "npts": npts, "headwave": bool(fl[0]), "diffracted": bool(fl[1])}
+
+    def ray_points(self, iray: int, section: int):
+        n = C.c_int()
+        _check(self._lib.fmt2d_ray_points(self._h, iray, section, None, None, 0, C.byref(n)))
+        x, z = np.zeros(max(n.value, 1)), np.zeros(max(n.value, 1))
+        _check(self._lib.fmt2d_ray_points(self._h, iray, section, _p(x), _p(z), len(x), C.byref(n)))
+        return x[:n.value], z[:n.value]
+
+    def frechet_csr(self, index_base: int = 0):
+        """(row_ptr, col, val) of the Frechet rows of the last trace."""
+        nnz = C.c_int()
+        nr = self._nray_last()
+        rp = np.zeros(nr + 1, dtype=np.int32)
+        _check(self._lib.fmt2d_frechet_csr(self._h, index_base, rp.ctypes.data_as(_ip), None, None, 0, C.byref(nnz)))
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        val = np.zeros(max(nnz.value, 1))
+        _check(self._lib.fmt2d_frechet_csr(self._h, index_base, rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
+                                           _p(val), len(col), C.byref(nnz)))
+        return rp, col[:nnz.value], val[:nnz.value]
+
+    def _nray_last(self):
+        return self._nray
 
     def field(self, isrc: int, itf: int):
         """(T, dT/dx, dT/dz) of field itf (1..12) of source isrc, reference node order."""

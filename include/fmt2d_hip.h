@@ -19,6 +19,19 @@
  *   fmt2d_get_field   read-back of one time field (Ttime_field2d arrivaltime /
  *                     time_gradient, mod_fmt2d.f90:262-279) in the reference's
  *                     region node order
+ *   fmt2d_set_inversion / fmt2d_inversion_info
+ *                     the parameterisation read from frechet.in
+ *                     (initialize_inversion2d, frechet2d.f90:14-118) over the
+ *                     resampled velocity grid (invert2d_main.f90:346-475)
+ *   fmt2d_trace       the ray loop of run_forward2d (inversion2d.f90:4491-4537):
+ *                     find_optimal_entry_inv (:6344-6438) for receivers in the
+ *                     water, trace_ray_from_receiver2d / trace_ray_section2d
+ *                     (rays2d.f90:7-762), receiver time + water leg
+ *   fmt2d_ray_info / fmt2d_ray_points
+ *                     Tray2d / Tray_section2d point lists (mod_fmt2d.f90)
+ *   fmt2d_frechet_csr compute_frechet_for_ray2d (inversion2d.f90:4548-4787) for
+ *                     every valid ray, packed as CSR rows (read_frechet_data2d,
+ *                     :3965-4052)
  */
 #ifndef FMT2D_HIP_H
 #define FMT2D_HIP_H
@@ -90,6 +103,30 @@ int fmt2d_tf_region(int itf);
  * order (grid nodes ix-outer/iz-inner, then top, then bottom interface
  * nodes); arrays hold region_nnode[fmt2d_tf_region(itf)] values. */
 int fmt2d_get_field(fmt2d_ctx* ctx, int isrc, int itf, double* T, double* dTdx, double* dTdz);
+
+/* Inversion parameterisation (frechet.in): invert the velocity grid and/or
+ * the listed interfaces (2 seafloor, 3 sediment base, 4 Moho).  Default:
+ * velocity only. */
+int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids);
+/* Number of velocity parameters and of all parameters (n_inv_parms). */
+int fmt2d_inversion_info(fmt2d_ctx* ctx, int* n_vel_par, int* n_par);
+/* Traces nray rays on the fields of the last fmt2d_forward.  Ray r runs
+ * from receiver (rx[r], rz[r]) back to source src[r] (0-based index into the
+ * last forward's sources) along path path[r] (1 Pg, 2 PsedP, 3 PmP, 4 Pg_ss,
+ * 5 Pg_ss2).  t_rcv: receiver time including the water leg; valid: 1/0. */
+int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const double* rx, const double* rz,
+                double* t_rcv, int* valid);
+/* Ray r of the last trace: entry point (x, z) the ray starts from, water-leg
+ * time, points per section (4), flags (headwave, diffracted). */
+int fmt2d_ray_info(fmt2d_ctx* ctx, int iray, double* entry_xz, double* water_time, int* npts, int* flags);
+/* Points of section `section` (1 = at the source) of ray r; with x == NULL
+ * only *npts is returned. */
+int fmt2d_ray_points(fmt2d_ctx* ctx, int iray, int section, double* x, double* z, int cap, int* npts);
+/* Frechet rows of the last trace as CSR (one row per ray, empty for invalid
+ * rays; columns are parameter numbers).  Call with col == NULL for *nnz and
+ * row_ptr, then with buffers of cap >= nnz.  index_base 1 gives Fortran
+ * indexing for row_ptr and col. */
+int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, double* val, int cap, int* nnz);
 
 #ifdef __cplusplus
 }

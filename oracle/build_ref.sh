@@ -47,6 +47,22 @@ $FC $FFLAGS -c "$S/fmt2d_main.f90"
 $FC $FFLAGS -c "$S/invert2d_main.f90"
 $FC $FFLAGS -o "$OUT/fmt2d" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o fmt2d_main.o
 $FC $FFLAGS -o "$OUT/invert2d" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o frechet2d.o inversion2d.o invert2d_main.o
+# checker build with the ray / Frechet-row dump (oracle/raydump.inc): a temp
+# copy of inversion2d.f90 includes the dump routine before
+# compute_frechet_for_ray2d and calls it after every ray of run_forward2d
+cp "$S/inversion2d.f90" "$TMP/inversion2d_dump.f90"
+chmod u+w "$TMP/inversion2d_dump.f90"
+cp "$HERE/raydump.inc" "$TMP/raydump.inc"
+perl -0pi -e 's/\n(subroutine compute_frechet_for_ray2d\(ray\))/\ninclude "raydump.inc"\n\n$1/' "$TMP/inversion2d_dump.f90"
+perl -0pi -e 's/(          t_fr_acc = t_fr_acc \+ real\(t_c3 - t_c2, kind=dp\) \/ real\(t_rate, kind=dp\)\n        end if\n)/$1        call fmt2d_dump_ray(isrc, ircv, receiver(ircv)%ray(iray))\n/' "$TMP/inversion2d_dump.f90"
+nd=$(grep -c "call fmt2d_dump_ray" "$TMP/inversion2d_dump.f90")
+ni=$(grep -c 'include "raydump.inc"' "$TMP/inversion2d_dump.f90")
+if [ "$nd" != 1 ] || [ "$ni" != 1 ]; then echo "ray dump patch applied at $nd/$ni sites, expected 1/1" >&2; exit 1; fi
+mkdir -p "$TMP/dump" && cd "$TMP/dump"
+$FC $FFLAGS -I.. -c "$TMP/inversion2d_dump.f90" -o inversion2d.o
+$FC $FFLAGS -I.. -c "$S/invert2d_main.f90" -o invert2d_main.o
+$FC $FFLAGS -o "$OUT/invert2d_dump" ../mod_fmt2d.o ../propagate2d.o ../rays2d.o ../read_tomo2d.o ../frechet2d.o inversion2d.o invert2d_main.o
+cd "$TMP"
 if [ -f "$HERE/ref_fields.f90" ]; then
   $FC $FFLAGS -c "$HERE/ref_fields.f90"
   $FC $FFLAGS -o "$OUT/ref_fields" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o ref_fields.o
