@@ -200,3 +200,36 @@ def multistage_case(dx: float = 0.1, nobs: int = 60):
     shots = np.round(0.1 + 0.2 * np.arange(1000), 6)
     g = make_geometry(obs, shots, m, ray_types=(1,))
     return m, g
+
+
+def phase_windows_geometry(model: Model, obs_x: Sequence[float], shot_x: Sequence[float],
+                           shot_z: float = 0.010) -> Geometry:
+    """Receiver lines for all five phases with the reference's windows:
+    Pg at every offset, PsedP at 3-6 mean sediment thicknesses, PmP at 3-30 km,
+    Pg_ss and Pg_ss2 up to 50 km (ray types 1..5, ``fmt2d_main.f90:686-735``)."""
+    obs_x = np.asarray(obs_x, dtype=float)
+    hsed = float(np.mean(model.sed_z - model.topo))
+    recs = []
+    for ox in obs_x:
+        lst = []
+        for sx in shot_x:
+            off = abs(sx - ox)
+            lst.append((float(sx), shot_z, 1, 1.0, 0.05))
+            if 3.0 * hsed <= off <= 6.0 * hsed:
+                lst.append((float(sx), shot_z, 2, 1.0, 0.05))
+            if 3.0 <= off <= 30.0:
+                lst.append((float(sx), shot_z, 3, 1.0, 0.05))
+            if off <= 50.0:
+                lst.append((float(sx), shot_z, 4, 1.0, 0.05))
+                lst.append((float(sx), shot_z, 5, 1.0, 0.05))
+        recs.append(lst)
+    return Geometry(obs_x=obs_x, obs_z=np.interp(obs_x, model.xpos, model.topo), receivers=recs)
+
+
+def multiphase_case(dx: float = 0.5):
+    """The plain case's model and sources with receiver lines for all five
+    phases (see ``phase_windows_geometry``)."""
+    m = make_model(dx=dx)
+    obs = np.arange(5.0, 200.0, 10.0)
+    shots = np.round(np.arange(0.5, 200.0 - 0.25, 0.5), 6)
+    return m, phase_windows_geometry(m, obs, shots)
