@@ -55,6 +55,9 @@ chmod u+w "$TMP/inversion2d_dump.f90"
 cp "$HERE/raydump.inc" "$TMP/raydump.inc"
 perl -0pi -e 's/\n(subroutine compute_frechet_for_ray2d\(ray\))/\ninclude "raydump.inc"\n\n$1/' "$TMP/inversion2d_dump.f90"
 perl -0pi -e 's/(          t_fr_acc = t_fr_acc \+ real\(t_c3 - t_c2, kind=dp\) \/ real\(t_rate, kind=dp\)\n        end if\n)/$1        call fmt2d_dump_ray(isrc, ircv, receiver(ircv)%ray(iray))\n/' "$TMP/inversion2d_dump.f90"
+perl -0pi -e 's/\n(end subroutine run_lsqr)/\n  call fmt2d_dump_lsqr(ndata, nparm, nnz, n_vel_parms, smooth_type_in, max_lsqr_iter, lsqr_iter, \&\n       wdv, wdv_iface, w_smooth, w_smooth_iface, eps_min, rnorm_out, bnorm_out, test2_out, \&\n       row_ptr, gindx, gval, residual, gtg_diag, dm_sum, dm_out)\n$1/' "$TMP/inversion2d_dump.f90"
+nl=$(grep -c "call fmt2d_dump_lsqr" "$TMP/inversion2d_dump.f90")
+if [ "$nl" != 1 ]; then echo "lsqr dump patch applied at $nl sites, expected 1" >&2; exit 1; fi
 nd=$(grep -c "call fmt2d_dump_ray" "$TMP/inversion2d_dump.f90")
 ni=$(grep -c 'include "raydump.inc"' "$TMP/inversion2d_dump.f90")
 if [ "$nd" != 1 ] || [ "$ni" != 1 ]; then echo "ray dump patch applied at $nd/$ni sites, expected 1/1" >&2; exit 1; fi
