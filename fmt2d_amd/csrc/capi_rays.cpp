@@ -96,6 +96,46 @@ int build_ray_geom(fmt2d_ctx* ctx) {
 
 }  // namespace
 
+// Frechet rows of the last trace, assembled on the device once per trace.
+int frechet_build(fmt2d_ctx* ctx) {
+  if (ctx->g_nnz >= 0) return FMT2D_OK;
+  hipStream_t st = ctx->stream;
+  const int nray = ctx->nray;
+  // pass 1: contributions per ray
+  DBuf<long long> cnt, coff;
+  HIPCHK(cnt.alloc(nray));
+  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
+                                ctx->rout.p, nullptr, cnt.p, nullptr, nullptr, st));
+  std::vector<long long> hc(nray), ho(nray);
+  HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  long long total = 0;
+  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
+  HIPCHK(upload(coff, ho, st));
+  DBuf<unsigned long long> keys;
+  DBuf<double> vals;
+  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
+                                ctx->rout.p, coff.p, nullptr, keys.p, vals.p, st));
+  int rbits = 1;
+  while ((1LL << rbits) < nray) rbits++;
+  CsrOut co;
+  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, 0, co, st);
+  if (e != hipSuccess) {
+    if (co.rowptr) (void)hipFree(co.rowptr);
+    if (co.col) (void)hipFree(co.col);
+    if (co.val) (void)hipFree(co.val);
+    return fail(FMT2D_EHIP, std::string("CSR assembly: ") + hipGetErrorString(e));
+  }
+  ctx->g_rp.release(); ctx->g_col.release(); ctx->g_val.release();
+  ctx->g_rp.p = co.rowptr; ctx->g_rp.n = (size_t)nray + 1;
+  ctx->g_col.p = co.col; ctx->g_col.n = (size_t)std::max(co.nnz, 1);
+  ctx->g_val.p = co.val; ctx->g_val.n = (size_t)std::max(co.nnz, 1);
+  ctx->g_nnz = co.nnz;
+  return FMT2D_OK;
+}
+
 extern "C" {
 
 int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids) {
@@ -158,6 +198,8 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   }
   ctx->hpoff[nray] = off;
   ctx->nray = nray;
+  ctx->g_nnz = -1;
+  ctx->sys_ndata = -1;
   if (nray == 0) return FMT2D_OK;
   HIPCHK(upload(ctx->rjobs, ctx->hjobs, st));
   HIPCHK(upload(ctx->rpoff, ctx->hpoff, st));
@@ -171,7 +213,14 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   std::vector<RayOut> out(nray);
   HIPCHK(hipMemcpyAsync(out.data(), ctx->rout.p, sizeof(RayOut) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  for (int r = 0; r < nray; r++) { t_rcv[r] = out[r].t; valid[r] = out[r].valid; }
+  ctx->h_tcalc.resize(nray);
+  ctx->h_valid.resize(nray);
+  for (int r = 0; r < nray; r++) {
+    t_rcv[r] = out[r].t; valid[r] = out[r].valid;
+    ctx->h_tcalc[r] = out[r].t; ctx->h_valid[r] = out[r].valid;
+  }
+  ctx->g_nnz = -1;
+  ctx->sys_ndata = -1;
   return FMT2D_OK;
 }
 
@@ -212,53 +261,22 @@ int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, do
   if (index_base != 0 && index_base != 1) return fail(FMT2D_EINVAL, "index_base must be 0 or 1");
   if (ctx->nray < 1) return fail(FMT2D_ESTATE, "no rays traced");
   HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
+  const int rc0 = frechet_build(ctx);
+  if (rc0) return rc0;
   const int nray = ctx->nray;
-  // pass 1: contributions per ray
-  DBuf<long long> cnt, coff;
-  HIPCHK(cnt.alloc(nray));
-  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
-                                ctx->rout.p, nullptr, cnt.p, nullptr, nullptr, st));
-  std::vector<long long> hc(nray), ho(nray);
-  HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  long long total = 0;
-  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
-  HIPCHK(upload(coff, ho, st));
-  DBuf<unsigned long long> keys;
-  DBuf<double> vals;
-  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
-                                ctx->rout.p, coff.p, nullptr, keys.p, vals.p, st));
-  int rbits = 1;
-  while ((1LL << rbits) < nray) rbits++;
-  CsrOut co;
-  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, index_base, co, st);
-  if (e != hipSuccess) {
-    if (co.rowptr) (void)hipFree(co.rowptr);
-    if (co.col) (void)hipFree(co.col);
-    if (co.val) (void)hipFree(co.val);
-    return fail(FMT2D_EHIP, std::string("CSR assembly: ") + hipGetErrorString(e));
-  }
-  *nnz = co.nnz;
-  int rc = FMT2D_OK;
+  *nnz = ctx->g_nnz;
   if (row_ptr) {
     std::vector<int> rp((size_t)nray + 1);
-    HIPCHK(hipMemcpy(rp.data(), co.rowptr, sizeof(int) * (nray + 1), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rp.data(), ctx->g_rp.p, sizeof(int) * (nray + 1), hipMemcpyDeviceToHost));
     for (int r = 0; r <= nray; r++) row_ptr[r] = rp[r] + index_base;
   }
   if (col && val) {
-    if (cap < co.nnz) rc = fail(FMT2D_EINVAL, "CSR buffers too small");
-    else {
-      HIPCHK(hipMemcpy(col, co.col, sizeof(int) * co.nnz, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(val, co.val, sizeof(double) * co.nnz, hipMemcpyDeviceToHost));
-    }
+    if (cap < ctx->g_nnz) return fail(FMT2D_EINVAL, "CSR buffers too small");
+    HIPCHK(hipMemcpy(col, ctx->g_col.p, sizeof(int) * ctx->g_nnz, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(val, ctx->g_val.p, sizeof(double) * ctx->g_nnz, hipMemcpyDeviceToHost));
+    if (index_base) for (int k = 0; k < ctx->g_nnz; k++) col[k] += index_base;
   }
-  (void)hipFree(co.rowptr);
-  (void)hipFree(co.col);
-  (void)hipFree(co.val);
-  return rc;
+  return FMT2D_OK;
 }
 
 }  // extern "C"

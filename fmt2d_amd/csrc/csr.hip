@@ -12,6 +12,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "csr.h"
+#include "lsqr.h"
 
 namespace fmt2d {
 
@@ -140,6 +141,64 @@ done:
   if (sum) (void)hipFree(sum);
   if (keep) (void)hipFree(keep);
   if (pos) (void)hipFree(pos);
+  if (tmp) (void)hipFree(tmp);
+  return err;
+}
+
+}  // namespace fmt2d
+
+namespace fmt2d {
+
+__global__ void entry_rows_kernel(const int* __restrict__ rp, int nrow, int* erow, int* eidx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow) return;
+  for (int k = rp[i]; k < rp[i + 1]; k++) { erow[k] = i; eidx[k] = k; }
+}
+
+__global__ void col_count_kernel(const int* __restrict__ col, int nnz, int* cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  atomicAdd(&cnt[col[k]], 1);
+}
+
+__global__ void csc_gather_kernel(const int* __restrict__ perm, const int* __restrict__ erow,
+                                  const double* __restrict__ val, int nnz, int* crow, double* cval) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int e = perm[k];
+  crow[k] = erow[e];
+  cval[k] = val[e];
+}
+
+hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow, int ncol, int nnz, int* cp, int* crow,
+                     double* cval, hipStream_t st) {
+  hipError_t err = hipSuccess;
+  int *erow = nullptr, *eidx = nullptr, *kout = nullptr, *perm = nullptr, *cnt = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0, tb2 = 0;
+  int bits = 1;
+  while ((1 << bits) < ncol + 1) bits++;
+  CK(dalloc(&erow, nnz)); CK(dalloc(&eidx, nnz)); CK(dalloc(&kout, nnz)); CK(dalloc(&perm, nnz));
+  CK(dalloc(&cnt, (size_t)ncol + 1));
+  CK(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)ncol + 1), st));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(entry_rows_kernel, dim3((nrow + 127) / 128), dim3(128), 0, st, rp, nrow, erow, eidx);
+    hipLaunchKernelGGL(col_count_kernel, dim3((nnz + 255) / 256), dim3(256), 0, st, col, nnz, cnt);
+    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, col, kout, eidx, perm, nnz, 0, bits, st));
+  }
+  CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt, cp, ncol + 1, st));
+  CK(hipMalloc(&tmp, std::max<size_t>(std::max(tb, tb2), 1)));
+  if (nnz > 0) CK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, col, kout, eidx, perm, nnz, 0, bits, st));
+  CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt, cp, ncol + 1, st));
+  if (nnz > 0)
+    hipLaunchKernelGGL(csc_gather_kernel, dim3((nnz + 255) / 256), dim3(256), 0, st, perm, erow, val, nnz, crow, cval);
+  CK(hipStreamSynchronize(st));
+done:
+  if (erow) (void)hipFree(erow);
+  if (eidx) (void)hipFree(eidx);
+  if (kout) (void)hipFree(kout);
+  if (perm) (void)hipFree(perm);
+  if (cnt) (void)hipFree(cnt);
   if (tmp) (void)hipFree(tmp);
   return err;
 }

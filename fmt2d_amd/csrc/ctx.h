@@ -13,6 +13,7 @@
 #include "heap_exit.h"
 #include "model.h"
 #include "rays.h"
+#include "lsqr.h"
 
 using namespace fmt2d;
 
@@ -114,6 +115,24 @@ struct fmt2d_ctx {
   DBuf<long long> rpoff;
   DBuf<double2> rpts;
   DBuf<double> rtsf;
+  std::vector<double> h_tcalc;         // receiver times of the last trace
+  std::vector<int> h_valid;
+  // Frechet rows of the last trace on the device (0-based columns)
+  DBuf<int> g_rp, g_col;
+  DBuf<double> g_val;
+  int g_nnz = -1;                      // -1: not built for the current trace
+  // tomography system (selected, weighted, scaled rows) and its column copy
+  int sys_ndata = -1, sys_nnz = 0;
+  double sys_eps_min = 0.0;
+  DBuf<int> s_rp, s_col, s_cp, s_crow;
+  DBuf<double> s_val, s_cval, s_res, s_gtg, s_mvs;
+  std::vector<double> h_gtg;
+  // LSQR work arrays
+  DBuf<double> l_u, l_v, l_w, l_dm, l_gv, l_gtu, l_sm, l_dmsum, l_ws, l_wd;
+  DBuf<uint8_t> l_mask;
+  DBuf<int4> l_lapnb;
+  DBuf<int2> l_ifnb;
+  DBuf<LsqrScalars> l_sc;
   // inversion parameterisation (frechet.in): velocity grid and interfaces
   int inv_vel = 1;
   std::vector<int> inv_ifaces;
@@ -136,3 +155,5 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
   return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
 }
 
+// Frechet rows of the last trace on the device (capi_rays.cpp)
+int frechet_build(fmt2d_ctx* ctx);

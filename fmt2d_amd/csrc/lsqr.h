@@ -1,0 +1,47 @@
+// Device LSQR of the tomography system (see lsqr.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fmt2d {
+
+struct LsqrScalars {
+  double norm;  // last front-to-back sum of squares
+  double alpha, beta, bnorm, rnorm, test2, anorm2, rho, rhobar, phi, phibar, theta, c;
+  int done, iters, trivial, pad;
+};
+
+struct LsqrSys {
+  int ndata, nparm, n_vel;
+  const int* rp;          // G rows (0-based), ndata+1
+  const int* col;
+  const double* val;
+  const int* cp;          // G columns, nparm+1
+  const int* crow;
+  const double* cval;
+  const double* res;      // weighted residuals
+  const double* dm_sum;   // accumulated model update (jumping strategy)
+  const uint8_t* mask;    // velocity columns with gtg_diag < 0.01 eps_min
+  const double* ws;       // smoothing weight per parameter
+  const double* wd;       // damping weight per parameter
+  const int4* lap_nb;     // velocity parameters: neighbours x-1, x+1, z-1, z+1 (-1 none)
+  const int2* if_nb;      // interface parameters: (1 one-sided | 2 central | 0 none, neighbour)
+  int smooth_on, damp_on;
+};
+
+struct LsqrWork {
+  double *u, *v, *w, *dm, *gv, *gtu, *sm;
+  LsqrScalars* sc;
+};
+
+hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
+                    LsqrScalars* host_out);
+hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double* src_val, const int* keep_row,
+                             const double* wt, const int* rp, int ndata, int base, const double* mvs, int nparm,
+                             int* col, double* val, hipStream_t st);
+hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st);
+// column-major copy of a CSR matrix (stable: rows ascending within a column)
+hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow, int ncol, int nnz, int* cp, int* crow,
+                     double* cval, hipStream_t st);
+
+}  // namespace fmt2d

@@ -1,0 +1,298 @@
+// LSQR on the augmented, smoothed and damped tomography system, on the
+// device (run_lsqr, src/inversion2d.f90:6578-7167, smooth_type 1).
+//
+// Krylov iterations amplify rounding differences (reordering a single dot
+// product moves the reference's model update by ~1%), so every reduction
+// here keeps the reference's FP64 order: G*x one thread per row and G^T*u
+// one thread per column, both front to back over ascending indices; the
+// Laplacian adds its neighbours in the reference's order; the two norms of
+// an iteration are front-to-back sums.  Element-wise work is parallel (it
+// does not round differently), and the scalar recurrences run in one
+// thread.  The host launches a fixed sequence per iteration and polls the
+// stop flag every few iterations; kernels of iterations after the stop do
+// nothing, so the iteration count is the reference's.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "lsqr.h"
+
+namespace fmt2d {
+
+__device__ inline bool stopped(const LsqrScalars* s) { return s->done != 0; }
+
+// y[i] = sum_k G(i,k) x(col k), ascending k (sparse_Ax, :2997-3018)
+__global__ void spmv_rows_kernel(const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ val,
+                                 int nrow, const double* __restrict__ x, double* y, const LsqrScalars* s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow || stopped(s)) return;
+  double acc = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; k++) acc = acc + val[k] * x[col[k]];
+  y[i] = acc;
+}
+
+// x[j] = sum_i G(i,j) u(i), ascending i (sparse_ATx, :3021-3042), from the
+// column-major copy of G
+__global__ void spmv_cols_kernel(const int* __restrict__ cp, const int* __restrict__ row, const double* __restrict__ val,
+                                 int ncol, const double* __restrict__ u, double* x, const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol || stopped(s)) return;
+  double acc = 0.0;
+  for (int k = cp[j]; k < cp[j + 1]; k++) acc = acc + val[k] * u[row[k]];
+  x[j] = acc;
+}
+
+// apply_laplacian_smooth_2d (:1161-1249), then the mask and weight of the
+// smoothing block: out = masked ? 0 : w_j * (L vin)_j
+__global__ void smooth_kernel(const LsqrSys S, const double* __restrict__ vin, double* out, const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm || stopped(s)) return;
+  double acc = 0.0;
+  if (j < S.n_vel) {
+    const int4 nb = S.lap_nb[j];
+    int deg = 0;
+    if (nb.x >= 0) { acc = acc + vin[nb.x]; deg++; }
+    if (nb.y >= 0) { acc = acc + vin[nb.y]; deg++; }
+    if (nb.z >= 0) { acc = acc + vin[nb.z]; deg++; }
+    if (nb.w >= 0) { acc = acc + vin[nb.w]; deg++; }
+    acc = acc - (double)deg * vin[j];
+  } else {
+    const int2 e = S.if_nb[j - S.n_vel];  // (kind, neighbour): kind 1 one-sided, 2 central, 0 none
+    if (e.x == 1) acc = acc + vin[e.y] - vin[j];
+    else if (e.x == 2) {
+      acc = acc + vin[j - 1];
+      acc = acc + vin[j + 1];
+      acc = acc - 2.0 * vin[j];
+    }
+  }
+  out[j] = S.mask[j] ? 0.0 : S.ws[j] * acc;
+}
+
+// front-to-back sum of a[i]^2 into *dst (one thread)
+__global__ void seqnorm_kernel(const double* __restrict__ a, long long n, double* dst, const LsqrScalars* s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || stopped(s)) return;
+  double acc = 0.0;
+  for (long long i = 0; i < n; i++) acc = acc + a[i] * a[i];
+  *dst = acc;
+}
+
+// ---- element-wise steps
+
+// initial right-hand side: u = [r; -smooth(dm_sum); masked ? 0 : -wd dm_sum]
+__global__ void rhs_kernel(const LsqrSys S, const double* __restrict__ sm, double* u) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nd = S.ndata, np = S.nparm;
+  if (i >= nd + 2 * np) return;
+  if (i < nd) u[i] = S.res[i];
+  else if (i < nd + np) u[i] = S.smooth_on ? -sm[i - nd] : 0.0;
+  else {
+    const int j = (int)(i - nd - np);
+    u[i] = (S.damp_on && !S.mask[j]) ? -S.wd[j] * S.dm_sum[j] : 0.0;
+  }
+}
+
+// u = (what == 0) u / beta ; (what == 1) u := Av - alpha u, Av assembled from
+// G v (gv), the smoothing block (sm) and the damping block
+__global__ void u_update_kernel(const LsqrSys S, const double* __restrict__ gv, const double* __restrict__ sm,
+                                const double* __restrict__ v, double* u, const LsqrScalars* s) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nd = S.ndata, np = S.nparm;
+  if (i >= nd + 2 * np || stopped(s)) return;
+  double av;
+  if (i < nd) av = gv[i];
+  else if (i < nd + np) av = S.smooth_on ? sm[i - nd] : 0.0;
+  else {
+    const int j = (int)(i - nd - np);
+    av = (S.damp_on && !S.mask[j]) ? S.wd[j] * v[j] : 0.0;
+  }
+  u[i] = av - s->alpha * u[i];
+}
+
+__global__ void scale_kernel(double* a, long long n, const double* __restrict__ by, int only_positive,
+                             const LsqrScalars* s) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || stopped(s)) return;
+  const double b = *by;
+  if (only_positive && !(b > 0.0)) return;
+  a[i] = a[i] / b;
+}
+
+// v = (G^T u + smooth(u_L)) + damping(u_D) - beta_factor * v, masked to 0
+__global__ void v_update_kernel(const LsqrSys S, const double* __restrict__ gtu, const double* __restrict__ sm,
+                                const double* __restrict__ u, double* v, int first, const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm || stopped(s)) return;
+  double a = gtu[j];
+  if (S.smooth_on) a = a + sm[j];
+  if (S.damp_on && !S.mask[j]) a = a + S.wd[j] * u[(long long)S.ndata + S.nparm + j];
+  if (!first) a = a - s->beta * v[j];
+  v[j] = S.mask[j] ? 0.0 : a;
+}
+
+// v is masked before G v and L v at the top of every iteration
+__global__ void mask_kernel(const LsqrSys S, double* v, const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm || stopped(s)) return;
+  if (S.mask[j]) v[j] = 0.0;
+}
+
+__global__ void w_init_kernel(const LsqrSys S, const double* __restrict__ v, double* w, double* dm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm) return;
+  w[j] = v[j];
+  dm[j] = 0.0;
+}
+
+// z = masked w; w = masked (v - theta/rho w); dm = dm + phi/rho z
+__global__ void w_update_kernel(const LsqrSys S, const double* __restrict__ v, double* w, double* dm,
+                                const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm || stopped(s)) return;
+  const double z = S.mask[j] ? 0.0 : w[j];
+  const double wn = v[j] - (s->theta / s->rho) * w[j];
+  w[j] = S.mask[j] ? 0.0 : wn;
+  dm[j] = dm[j] + (s->phi / s->rho) * z;
+}
+
+// ---- scalar steps (one thread)
+
+// step 0: beta = bnorm = sqrt(norm)   step 1: alpha = sqrt(norm) (initial)
+// step 2: beta = sqrt(norm)           step 3: alpha = sqrt(norm) + recurrences
+__global__ void scalar_kernel(LsqrScalars* s, int step, int iter, int max_iter, double atol) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || s->done) return;
+  if (step == 0) {
+    s->bnorm = sqrt(s->norm);
+    s->beta = s->bnorm;
+    if (s->bnorm < 1.0e-30) { s->done = 1; s->iters = 0; s->rnorm = 0.0; s->test2 = 0.0; s->trivial = 1; }
+  } else if (step == 1) {
+    s->alpha = sqrt(s->norm);
+    if (s->alpha < 1.0e-30) { s->done = 1; s->iters = 0; s->rnorm = s->bnorm; s->test2 = 0.0; s->trivial = 1; return; }
+    s->phibar = s->beta;
+    s->rhobar = s->alpha;
+    s->anorm2 = s->alpha * s->alpha + s->beta * s->beta;
+  } else if (step == 2) {
+    s->beta = sqrt(s->norm);
+  } else {
+    s->alpha = sqrt(s->norm);
+    s->anorm2 = s->anorm2 + s->alpha * s->alpha + s->beta * s->beta;
+    s->rho = sqrt(s->rhobar * s->rhobar + s->beta * s->beta);
+    const double c = s->rhobar / s->rho, sn = s->beta / s->rho;
+    s->theta = sn * s->alpha;
+    s->rhobar = -c * s->alpha;
+    s->phi = c * s->phibar;
+    s->phibar = sn * s->phibar;
+    s->c = c;
+  }
+}
+
+// after the w/dm update: convergence test of iteration `iter`
+__global__ void test_kernel(LsqrScalars* s, int iter, int max_iter, double atol) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || s->done) return;
+  s->rnorm = fabs(s->phibar);
+  const double arnorm = s->phibar * s->alpha * fabs(s->c);
+  s->test2 = arnorm / (sqrt(s->anorm2) * s->rnorm + 1.0e-30);
+  s->iters = iter;
+  if (s->test2 < atol || 1.0 + s->test2 <= 1.0 || iter >= max_iter) s->done = 1;
+}
+
+// ---------------------------------------------------------------- driver
+
+static inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
+                    LsqrScalars* host_out) {
+  const long long nd = S.ndata, np = S.nparm, na = nd + 2 * np;
+  LsqrScalars* sc = W.sc;
+  (void)hipMemsetAsync(sc, 0, sizeof(LsqrScalars), st);
+  // u = [r; -w L dm_sum; -wd dm_sum]
+  if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, S.dm_sum, W.sm, sc);
+  hipLaunchKernelGGL(rhs_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.sm, W.u);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.u, na, &sc->norm, sc);
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
+  hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 0, sc);
+  // v = A^T u, masked; alpha; v /= alpha; w = v
+  hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u, W.gtu,
+                     sc);
+  if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
+  hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 1, sc);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.v, np, &sc->norm, sc);
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 1, 0, max_iter, atol);
+  hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 0, sc);
+  hipLaunchKernelGGL(w_init_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int it = 1; it <= max_iter; it++) {
+    hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
+    hipLaunchKernelGGL(spmv_rows_kernel, dim3(nb(nd, 128)), dim3(128), 0, st, S.rp, S.col, S.val, S.ndata, W.v, W.gv,
+                       sc);
+    if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
+    hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.v, W.u, sc);
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.u, na, &sc->norm, sc);
+    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
+    hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
+    hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
+                       W.gtu, sc);
+    if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
+    hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.v, np, &sc->norm, sc);
+    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);
+    hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 1, sc);
+    hipLaunchKernelGGL(w_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm, sc);
+    hipLaunchKernelGGL(test_kernel, dim3(1), dim3(64), 0, st, sc, it, max_iter, atol);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (it % poll == 0 || it == max_iter) {
+      e = hipMemcpyAsync(host_out, sc, sizeof(LsqrScalars), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return e;
+      if (host_out->done) break;
+    }
+  }
+  e = hipMemcpyAsync(host_out, sc, sizeof(LsqrScalars), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
+
+// ---- system assembly (read_frechet_data2d + scaling, :280-486)
+
+// rows kept: copy, scaled by mvscale(j) * data_wt(i) (gval*mvscale first)
+__global__ void scale_rows_kernel(const int* __restrict__ src_rp, const int* __restrict__ src_col,
+                                  const double* __restrict__ src_val, const int* __restrict__ keep_row,
+                                  const double* __restrict__ wt, const int* __restrict__ rp, int ndata, int base,
+                                  const double* __restrict__ mvs, int nparm, int* col, double* val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndata) return;
+  const int r = keep_row[i];
+  int o = rp[i];
+  for (int k = src_rp[r]; k < src_rp[r + 1]; k++, o++) {
+    const int j = src_col[k] - base;
+    double g = src_val[k];
+    if (j >= 0 && j < nparm) g = g * mvs[j] * wt[i];
+    col[o] = j;
+    val[o] = g;
+  }
+}
+
+// gtg_diag(j) = sum over rows (ascending) of G(i,j)^2, from the column copy
+__global__ void gtg_kernel(const int* __restrict__ cp, const double* __restrict__ cval, int nparm, double* gtg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nparm) return;
+  double a = 0.0;
+  for (int k = cp[j]; k < cp[j + 1]; k++) a = a + cval[k] * cval[k];
+  gtg[j] = a;
+}
+
+hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double* src_val, const int* keep_row,
+                             const double* wt, const int* rp, int ndata, int base, const double* mvs, int nparm,
+                             int* col, double* val, hipStream_t st) {
+  if (ndata == 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(nb(ndata, 128)), dim3(128), 0, st, src_rp, src_col, src_val, keep_row,
+                     wt, rp, ndata, base, mvs, nparm, col, val);
+  return hipGetLastError();
+}
+
+hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st) {
+  hipLaunchKernelGGL(gtg_kernel, dim3(nb(nparm, 128)), dim3(128), 0, st, cp, cval, nparm, gtg);
+  return hipGetLastError();
+}
+
+}  // namespace fmt2d

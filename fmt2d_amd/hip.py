@@ -33,6 +33,21 @@ class ForwardStats(C.Structure):
                 ("capped", C.c_longlong), ("max_sweeps", C.c_int)]
 
 
+class SystemParams(C.Structure):
+    _fields_ = [("crit_chi", C.c_double), ("phase_wt_psed", C.c_double), ("phase_wt_pmp", C.c_double),
+                ("phase_wt_pgss", C.c_double), ("refl_weight", C.c_double), ("alt_mode", C.c_int)]
+
+
+class LsqrParams(C.Structure):
+    _fields_ = [("wdv", C.c_double), ("wdv_iface", C.c_double), ("w_smooth", C.c_double),
+                ("w_smooth_iface", C.c_double), ("smooth_type", C.c_int), ("max_iter", C.c_int)]
+
+
+class LsqrStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rnorm", C.c_double), ("bnorm", C.c_double), ("test2", C.c_double),
+                ("ms", C.c_double)]
+
+
 class Fmt2dError(RuntimeError):
     pass
 
@@ -65,6 +80,9 @@ def load() -> C.CDLL:
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
     lib.fmt2d_ray_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip]
     lib.fmt2d_frechet_csr.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int, _ip]
+    lib.fmt2d_build_system.argtypes = [C.c_void_p, _dp, C.POINTER(SystemParams), _ip, _ip, _dp]
+    lib.fmt2d_system_get.argtypes = [C.c_void_p, _dp, _dp]
+    lib.fmt2d_lsqr.argtypes = [C.c_void_p, C.POINTER(LsqrParams), _dp, _dp, C.POINTER(LsqrStats)]
     _LIB = lib
     return lib
 
@@ -76,7 +94,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
-           "fmt2d_frechet_csr")
+           "fmt2d_frechet_csr", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr")
 
 
 def _check(rc: int):
@@ -190,6 +208,32 @@ class Context:
         _check(self._lib.fmt2d_frechet_csr(self._h, index_base, rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
                                            _p(val), len(col), C.byref(nnz)))
         return rp, col[:nnz.value], val[:nnz.value]
+
+    def build_system(self, t_obs, crit_chi=1.0, phase_wt_psed=1.0, phase_wt_pmp=1.0, phase_wt_pgss=1.0,
+                     refl_weight=10.0, alt_mode=0):
+        t = np.ascontiguousarray(t_obs, dtype=np.float64)
+        p = SystemParams(crit_chi, phase_wt_psed, phase_wt_pmp, phase_wt_pgss, refl_weight, alt_mode)
+        nd, nz, eps = C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.fmt2d_build_system(self._h, _p(t), C.byref(p), C.byref(nd), C.byref(nz), C.byref(eps)))
+        self._ndata = nd.value
+        return nd.value, nz.value, eps.value
+
+    def system(self):
+        nv, npar = self.inversion_info()
+        g = np.zeros(npar)
+        r = np.zeros(max(self._ndata, 1))
+        _check(self._lib.fmt2d_system_get(self._h, _p(g), _p(r)))
+        return g, r[:self._ndata]
+
+    def lsqr(self, dm_sum=None, wdv=0.1, wdv_iface=0.1, w_smooth=0.3, w_smooth_iface=0.3, smooth_type=1,
+             max_iter=400):
+        nv, npar = self.inversion_info()
+        dms = np.zeros(npar) if dm_sum is None else np.ascontiguousarray(dm_sum, dtype=np.float64)
+        out = np.zeros(npar)
+        p = LsqrParams(wdv, wdv_iface, w_smooth, w_smooth_iface, smooth_type, max_iter)
+        st = LsqrStats()
+        _check(self._lib.fmt2d_lsqr(self._h, C.byref(p), _p(dms), _p(out), C.byref(st)))
+        return out, st
 
     def _nray_last(self):
         return self._nray

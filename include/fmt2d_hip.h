@@ -32,6 +32,12 @@
  *   fmt2d_frechet_csr compute_frechet_for_ray2d (inversion2d.f90:4548-4787) for
  *                     every valid ray, packed as CSR rows (read_frechet_data2d,
  *                     :3965-4052)
+ *   fmt2d_build_system / fmt2d_system_get
+ *                     data selection and weighting of lsqr_inversion2d
+ *                     (inversion2d.f90:280-486): outlier cut, phase weights,
+ *                     mvscale, G*mvscale*w, gtg_diag, eps_min
+ *   fmt2d_lsqr        run_lsqr (inversion2d.f90:6578-7167) with the Laplacian
+ *                     smoothing block (apply_laplacian_smooth_2d, :1161-1249)
  */
 #ifndef FMT2D_HIP_H
 #define FMT2D_HIP_H
@@ -127,6 +133,42 @@ int fmt2d_ray_points(fmt2d_ctx* ctx, int iray, int section, double* x, double* z
  * row_ptr, then with buffers of cap >= nnz.  index_base 1 gives Fortran
  * indexing for row_ptr and col. */
 int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, double* val, int cap, int* nnz);
+
+/* Data selection and scaling (lsqr_inversion2d's system assembly). */
+typedef struct {
+  double crit_chi;        /* inv_crit_chi: rows with |t_obs - t_calc| > crit_chi are outliers */
+  double phase_wt_psed;   /* data weights of PsedP, PmP, Pg_ss/Pg_ss2 rows */
+  double phase_wt_pmp;
+  double phase_wt_pgss;
+  double refl_weight;     /* mvscale of interface parameters */
+  int alt_mode;           /* 0 joint, 1 velocity only, 2 interface only */
+} fmt2d_system_params;
+
+typedef struct {
+  double wdv, wdv_iface;           /* damping weights (velocity, interface) */
+  double w_smooth, w_smooth_iface; /* smoothing weights */
+  int smooth_type;                 /* 1: Laplacian (the supported form) */
+  int max_iter;
+} fmt2d_lsqr_params;
+
+typedef struct {
+  int iterations;
+  double rnorm, bnorm, test2;
+  double ms;              /* wall time of the solve */
+} fmt2d_lsqr_stats;
+
+/* Builds the weighted, scaled system from the Frechet rows of the last trace
+ * and the observed times t_obs[nray] (< 0: no pick).  Returns the number of
+ * data rows, non-zeros and eps_min = max(1e-6, 1e-4 max gtg_diag). */
+int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_params* p, int* ndata, int* nnz,
+                       double* eps_min);
+/* gtg_diag (n_par) and the weighted residuals (ndata) of the built system;
+ * either pointer may be NULL. */
+int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual);
+/* One LSQR solve of [G; w L; wd I] dm = [r; -w L dm_sum; -wd dm_sum] with the
+ * reference's column mask and stopping rule (test2 < 1e-4). */
+int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
+               fmt2d_lsqr_stats* stats);
 
 #ifdef __cplusplus
 }
