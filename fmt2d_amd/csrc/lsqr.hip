@@ -67,12 +67,39 @@ __global__ void smooth_kernel(const LsqrSys S, const double* __restrict__ vin, d
   out[j] = S.mask[j] ? 0.0 : S.ws[j] * acc;
 }
 
-// front-to-back sum of a[i]^2 into *dst (one thread)
-__global__ void seqnorm_kernel(const double* __restrict__ a, long long n, double* dst, const LsqrScalars* s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || stopped(s)) return;
+// front-to-back sum of a[i]^2 into *dst.  One block: waves 1..3 square
+// the next chunk into LDS while lane 0 of wave 0 adds the current chunk in
+// order (the dependent add chain is the cost; the loads hide behind it).
+constexpr int SEQ_CHUNK = 2048;
+__global__ void __launch_bounds__(256) seqnorm_kernel(const double* __restrict__ a, long long n, double* dst,
+                                                     const LsqrScalars* s) {
+  if (stopped(s)) return;
+  __shared__ double buf[2][SEQ_CHUNK];
+  const int t = threadIdx.x;
+  const long long nch = (n + SEQ_CHUNK - 1) / SEQ_CHUNK;
   double acc = 0.0;
-  for (long long i = 0; i < n; i++) acc = acc + a[i] * a[i];
-  *dst = acc;
+  // first chunk
+  if (t >= 64 && nch > 0) {
+    const int m = (int)min((long long)SEQ_CHUNK, n);
+    for (int k = t - 64; k < m; k += 192) { const double x = a[k]; buf[0][k] = x * x; }
+  }
+  __syncthreads();
+  for (long long c = 0; c < nch; c++) {
+    const int cur = (int)(c & 1);
+    if (t >= 64) {
+      if (c + 1 < nch) {
+        const long long base = (c + 1) * SEQ_CHUNK;
+        const int m = (int)min((long long)SEQ_CHUNK, n - base);
+        for (int k = t - 64; k < m; k += 192) { const double x = a[base + k]; buf[cur ^ 1][k] = x * x; }
+      }
+    } else if (t == 0) {
+      const int m = (int)min((long long)SEQ_CHUNK, n - c * SEQ_CHUNK);
+      const double* b = buf[cur];
+      for (int k = 0; k < m; k++) acc = acc + b[k];
+    }
+    __syncthreads();
+  }
+  if (t == 0) *dst = acc;
 }
 
 // ---- element-wise steps
@@ -206,7 +233,7 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   // u = [r; -w L dm_sum; -wd dm_sum]
   if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, S.dm_sum, W.sm, sc);
   hipLaunchKernelGGL(rhs_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.sm, W.u);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.u, na, &sc->norm, sc);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 0, sc);
   // v = A^T u, masked; alpha; v /= alpha; w = v
@@ -214,7 +241,7 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
                      sc);
   if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
   hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 1, sc);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.v, np, &sc->norm, sc);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 1, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 0, sc);
   hipLaunchKernelGGL(w_init_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm);
@@ -226,14 +253,14 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
                        sc);
     if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
     hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.v, W.u, sc);
-    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.u, na, &sc->norm, sc);
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
     hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
     hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
     hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
                        W.gtu, sc);
     if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
     hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
-    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(64), 0, st, W.v, np, &sc->norm, sc);
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
     hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);
     hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 1, sc);
     hipLaunchKernelGGL(w_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm, sc);

@@ -233,3 +233,32 @@ def multiphase_case(dx: float = 0.5):
     obs = np.arange(5.0, 200.0, 10.0)
     shots = np.round(np.arange(0.5, 200.0 - 0.25, 0.5), 6)
     return m, phase_windows_geometry(m, obs, shots)
+
+
+def inversion_case(dx: float = 0.5, obs_x: Sequence[float] = (45.0, 95.0, 145.0), shot_dx: float = 1.0):
+    """A small inversion case: the plain model family without the
+    checkerboard as the starting model, the checkerboard model as the truth,
+    ``obs_x`` OBS and shots every ``shot_dx`` km at 10 m depth, Pg only.
+    Returns (true model, starting model, geometry without picks)."""
+    m_true = make_model(dx=dx)
+    m0 = make_model(dx=dx, checker_amp=0.0)
+    shots = np.round(np.arange(shot_dx / 2.0, 200.0 - shot_dx / 4.0, shot_dx), 6)
+    g = make_geometry(obs_x, shots, m_true, ray_types=(1,))
+    return m_true, m0, g
+
+
+def with_picks(g: Geometry, t_calc, valid, sigma: float = 0.05, seed: int = 1) -> Geometry:
+    """Geometry whose receiver lines carry picks t_calc + N(0, sigma) (seeded,
+    drawn line by line) and -1 where the ray is invalid."""
+    rng = np.random.default_rng(seed)
+    k = 0
+    recs = []
+    for rl in g.receivers:
+        lst = []
+        for (x, z, rt, t, u) in rl:
+            noise = rng.normal(0.0, sigma)
+            tobs = float(t_calc[k]) + noise if valid[k] else -1.0
+            lst.append((x, z, rt, tobs, sigma))
+            k += 1
+        recs.append(lst)
+    return Geometry(obs_x=g.obs_x, obs_z=g.obs_z, receivers=recs)

@@ -1,0 +1,64 @@
+"""HIP tomography system and LSQR vs the reference inversion.
+
+The whole iteration runs on the GPU on the small inversion case
+(fmt2d_amd.synth.inversion_case): forward + rays on the true model give the
+receiver times the picks are made from, then forward + rays + Frechet rows on
+the starting model, data selection/weighting and one LSQR solve.  The
+fixture (tests/golden/ref_lsqr_small.npz) holds the reference's picks,
+system and model update.  Bar: model update within 1e-4 relative; the path
+reproduces gtg_diag, the residuals, the iteration count and the model update
+bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = os.path.join(ROOT, "tests", "golden", "ref_lsqr_small.npz")
+
+
+def context_for(model, geom, d):
+    from fmt2d_amd import formats, hip, synth
+    synth.write_fmt2d_case(d, model, geom)
+    sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+    sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+    mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+    ctx = hip.Context(0)
+    ctx.set_model(sm, sx, sz, mx, mz)
+    ctx.set_inversion(True, [])
+    return ctx, formats.read_geom(os.path.join(d, "geom.dat"))
+
+
+def test_inversion_step_matches_reference(tmp_path):
+    from fmt2d_amd import synth
+    Z = np.load(FIX)
+    m_true, m0, g = synth.inversion_case()
+    print("[gpu] true-model forward + rays (small inversion case)", flush=True)
+    ctx, geom = context_for(m_true, g, str(tmp_path / "true"))
+    ctx.forward(list(geom.obs_x))
+    t, v = ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+    assert np.array_equal(v, Z["valid_true"])
+    assert np.array_equal(t[v], Z["t_true"][v])
+    ctx.close()
+    gi = synth.with_picks(g, t, v)
+    print("[gpu] starting-model forward + rays + system + LSQR", flush=True)
+    ctx, geom = context_for(m0, gi, str(tmp_path / "start"))
+    assert np.array_equal(geom.t_obs, Z["t_obs"])
+    ctx.forward(list(geom.obs_x))
+    ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+    nd, nnz, eps = ctx.build_system(geom.t_obs, crit_chi=1.0, refl_weight=1.0)
+    ndata, nparm, nnz_r, n_vel, stype, maxit, iters = (int(x) for x in Z["scal"])
+    assert (nd, nnz) == (ndata, nnz_r)
+    assert eps == Z["par"][4]
+    gtg, res = ctx.system()
+    assert np.array_equal(gtg, Z["gtg"]) and np.array_equal(res, Z["residual"])
+    dm, st = ctx.lsqr(wdv=0.1, wdv_iface=0.1, w_smooth=0.3, w_smooth_iface=0.3, max_iter=maxit)
+    rel = np.max(np.abs(dm - Z["dm"])) / np.max(np.abs(Z["dm"]))
+    print(f"LSQR iterations {st.iterations} (reference {iters}), model update max rel diff {rel:.1e}, "
+          f"bit-identical {np.mean(dm == Z['dm']):.4f}, {st.ms:.1f} ms")
+    assert rel < 1e-4
+    assert st.iterations == iters
+    assert np.array_equal(dm, Z["dm"])
