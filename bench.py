@@ -1,14 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X forward hot path: eikonal source-stages per second.
+"""Benchmark of the MI355X hot path.
 
-One step = one forward pass of the multistage eikonal solver over all OBS of
-the workload: for every OBS the twelve time fields of the five phases (Pg,
-PsedP, PmP, Pg_ss, Pg_ss2), solved on the device with the inputs (model,
-geometry) already resident.  A source-stage is one (OBS, phase) chain, so a
-step completes 5 x nobs source-stages.
+Headline (``value``): eikonal source-stages per second.  One step = one
+forward pass of the multistage eikonal solver over all OBS of the workload:
+for every OBS the twelve time fields of the five phases (Pg, PsedP, PmP,
+Pg_ss, Pg_ss2), solved on the device with the inputs (model, geometry)
+already resident.  A source-stage is one (OBS, phase) chain, so a step
+completes 5 x nobs source-stages.  Default workload: the full multistage
+forward on the 0.1 km grid (2001x301 nodes), 60 OBS every 3.3 km.
 
-Prints ONE JSON line (rank 0).  Default workload: the full multistage forward
-on the 0.1 km grid (2001x301 nodes), 60 OBS every 3.3 km.
+Second figure (``iteration``): seconds per inversion iteration = forward +
+rays + Frechet rows + system assembly + one LSQR solve (fixed damping) on the
+plain case (401x61, 20 OBS, 399 shots, Pg; picks from the checkerboard model
+with 50 ms seeded noise, starting model without it, Laplacian smoothing),
+with the reference's own inversion timed beside it.
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -80,6 +87,100 @@ def cpu_baseline(case_dir: str, obs_x, nsample: int):
             "sample": f"{len(sample)} OBS x 12 fields of the same model ({desc}), {sec:.2f} s"}
 
 
+def inversion_case_dir(d):
+    """Plain inversion case in directory d (true and starting models, geometry
+    without picks); returns (true model, starting model, geometry)."""
+    from fmt2d_amd import synth
+    m_true, g = synth.plain_pg_case()
+    m0 = synth.make_model(dx=0.5, checker_amp=0.0)
+    return m_true, m0, g
+
+
+def cpu_iteration_baseline(d, m0, gi):
+    """The reference inversion (oracle/_ref/invert2d, built from the
+    reference sources) on the same case, one core: seconds of its first
+    iteration's forward (FMM + rays + Frechet) and LSQR solve, from its log."""
+    from fmt2d_amd import synth
+    exe = os.path.join(ROOT, "oracle", "_ref", "invert2d")
+    if not os.path.exists(exe):
+        return None
+    synth.write_fmt2d_case(d, m0, gi)
+    with open(os.path.join(d, "frechet.in"), "w") as f:
+        f.write("1\n1\n1\n1\n0\n")
+    with open(os.path.join(d, "invert2d.in"), "w") as f:
+        f.write("model.smesh\nmoho.dat\nsed.dat\ngeom.dat\n")
+        for v in [1, 0.1, 0.3, 1.0, 400, 0.05, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0]:
+            f.write(f"{v}\n")
+        f.write("1 0.0 0.5 0.3\n-1\n")
+    cmd = [exe]
+    if os.path.exists("/usr/bin/taskset"):
+        cmd = ["/usr/bin/taskset", "-c", "0", exe]
+    r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=600)
+    fw = [ln for ln in r.stdout.splitlines() if "forward breakdown" in ln]
+    ls = [ln for ln in r.stdout.splitlines() if "LSQR converged" in ln]
+    if len(fw) < 2 or not ls:
+        return None
+    nums = [float(x) for x in fw[1].replace(",", " ").split() if x.replace(".", "", 1).isdigit()]
+    fwd = sum(nums[:3])
+    lsqr = float(ls[0].split("time=")[1].split()[0])
+    return {"value": round(fwd + lsqr, 4), "unit": "s/iteration", "cores": 1, "kind": "reference",
+            "sample": f"one iteration of the reference inversion on the same case: forward {fwd:.2f} s "
+                      f"(FMM+rays+Frechet), LSQR {lsqr:.2f} s"}
+
+
+def inversion_iteration(steps: int, warmup: int, cpu: bool):
+    """Seconds per inversion iteration on the GPU (see module docstring)."""
+    from fmt2d_amd import formats, hip, synth
+    m_true, m0, g = inversion_case_dir(None)
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_fmt2d_case(d, m_true, g)
+        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+        sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+        mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+        geom = formats.read_geom(os.path.join(d, "geom.dat"))
+        ctx = hip.Context(0)
+        ctx.set_model(sm, sx, sz, mx, mz)
+        ctx.forward(list(geom.obs_x))
+        t, v = ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+        ctx.close()
+        gi = synth.with_picks(g, t, v)
+        synth.write_fmt2d_case(d, m0, gi)
+        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+        geom = formats.read_geom(os.path.join(d, "geom.dat"))
+        ctx = hip.Context(0)
+        ctx.set_model(sm, sx, sz, mx, mz)
+        ctx.set_inversion(True, [])
+        parts = {"forward": [], "rays": [], "frechet_system": [], "lsqr": []}
+        iters = 0
+        for k in range(warmup + steps):
+            t0 = time.perf_counter()
+            ctx.forward(list(geom.obs_x))
+            t1 = time.perf_counter()
+            ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+            t2 = time.perf_counter()
+            nd, nnz, eps = ctx.build_system(geom.t_obs, crit_chi=1.0, refl_weight=1.0)
+            t3 = time.perf_counter()
+            dm, st = ctx.lsqr(wdv=0.1, wdv_iface=0.1, w_smooth=0.3, w_smooth_iface=0.3, max_iter=400)
+            t4 = time.perf_counter()
+            if k >= warmup:
+                for key, dt in zip(parts, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+                    parts[key].append(dt)
+            iters = st.iterations
+        ctx.close()
+        cpu_b = None
+        if cpu:
+            with tempfile.TemporaryDirectory() as d2:
+                cpu_b = cpu_iteration_baseline(d2, m0, gi)
+    per = {k: float(np.mean(v_)) for k, v_ in parts.items()}
+    total = sum(per.values())
+    return {"metric": "forward+Frechet+LSQR s/iteration", "value": round(total, 4), "unit": "s/iteration",
+            "higher_is_better": False,
+            "workload": "plain inversion case: 401x61 grid (0.5 km), 20 OBS, 399 shots, Pg; one iteration "
+                        "(forward, rays, Frechet CSR, system, one LSQR solve, Laplacian smoothing, damping 0.1)",
+            "breakdown_ms": {k: round(1e3 * v_, 2) for k, v_ in per.items()},
+            "ndata": nd, "nnz": nnz, "lsqr_iterations": iters, "cpu_baseline": cpu_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +188,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="multistage", choices=["multistage", "plain"])
     ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
+    ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -117,6 +219,7 @@ def main():
         cpu = cpu_baseline(d, obs, args.cpu_sample) if args.cpu_sample > 0 else None
         ctx.close()
 
+    it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
     wall = float(np.sum(walls))
     stages = 5 * len(obs)
     value = stages * args.steps / wall
@@ -146,6 +249,7 @@ def main():
                      "algorithmic_bytes": f"{BYTES_PER_NODE} B per region node per field (FP64 velocity "
                                           f"read, T and grad T written) = {alg_bytes} B per step"},
         "cpu_baseline": cpu,
+        "iteration": it,
     }
     print(json.dumps(out))
 
