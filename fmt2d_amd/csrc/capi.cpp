@@ -260,6 +260,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   }
   ctx->stats = fmt2d_forward_stats{};
   ctx->trace.clear();
+  ctx->trace_sweeps.clear();
   ctx->stats.nsrc = nsrc;
   ctx->stats.nfield = nfield;
   ctx->stats.region_nodes = region_nodes;
@@ -280,24 +281,56 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
   if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
-  if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3));
-  HIPCHK(hipMemsetAsync(ctx->acc.p, 0, 3 * sizeof(unsigned long long), st));
+  if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3 + 64));
+  HIPCHK(hipMemsetAsync(ctx->acc.p, 0, (3 + 64) * sizeof(unsigned long long), st));
   const int max_launch = 64 * (ntx + ntz) + 4000;
   const bool headwave = !(flags & 1);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
   const double window = ctx->window_tiles * 8.0 * std::max(m.dx, m.dz) / 4.0;
+  size_t nev = 0;  // event pairs recorded around fim launches
   for (int L = 0; L < 4; L++) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
     const FieldDesc* dl = ctx->descs.p + f0;
     if (L > 0) HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz, ctx->isecs.p, st));
-    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 2, st));
+    // counters: [0] due tiles of a launch, [1] / [2] sizes of active lists 0 / 1
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 3, st));
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
                              ctx->tcount.p + 1, st));
-    HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    int cur = 0, n = ctx->h_count[0], launches = 0;
-    while (n > 0) {
+    if (ctx->ttrace.n < (size_t)max_launch + 1) HIPCHK(ctx->ttrace.alloc((size_t)max_launch + 1));
+    if (ctx->tmax.n < (size_t)max_launch + 1) HIPCHK(ctx->tmax.alloc((size_t)max_launch + 1));
+    HIPCHK(hipMemsetAsync(ctx->tmax.p, 0, sizeof(int) * ((size_t)max_launch + 1), st));
+    // launches are queued in batches; the host reads the active count once per batch
+    constexpr int BATCH = 8;
+    const int fim_grid = 8192, sched_grid = 512;
+    int cur = 0, launches = 0;
+    bool done = false;
+    while (!done) {
+      for (int b = 0; b < BATCH; b++, launches++) {
+        if (launches >= max_launch) break;
+        const int nxt = cur ^ 1;
+        HIPCHK(launch_schedule(ctx->tlist[cur].p, ctx->tcount.p + 1 + cur, nf, ntx, ntiles, ctx->tkey.p,
+                               ctx->tfront.p, window, launches & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p,
+                               ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, launches > 0 ? ctx->ttrace.p + launches - 1
+                                                                                          : nullptr,
+                               sched_grid, st));
+        const size_t ei = 2 * nev++;
+        while (ctx->ev.size() < ei + 2) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          ctx->ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(ctx->ev[ei], st));
+        HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, ctx->tin.p,
+                          ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->colors,
+                          ctx->acc.p, ctx->tmax.p + launches, st));
+        HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
+        cur = nxt;
+      }
+      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (ctx->h_count[1 + cur] == 0) { done = true; break; }
       if (launches >= max_launch) {
+        const int n = ctx->h_count[1 + cur];
         std::vector<int2> tail(std::min(n, 8));
         (void)hipMemcpy(tail.data(), ctx->tlist[cur].p, tail.size() * sizeof(int2), hipMemcpyDeviceToHost);
         std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
@@ -309,30 +342,27 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         }
         return fail(FMT2D_ENOCONV, msg + ")");
       }
-      const int nxt = cur ^ 1;
-      HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 2, st));
-      HIPCHK(launch_schedule(ctx->tlist[cur].p, n, nf, ntx, ntiles, ctx->tkey.p, ctx->tfront.p, window,
-                             launches & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p, ctx->tlist[nxt].p,
-                             ctx->tcount.p + 1, st));
-      const size_t ei = 2 * (size_t)ctx->stats.launches + 2 * (size_t)launches;
-      while (ctx->ev.size() < ei + 2) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        ctx->ev.push_back(e);
-      }
-      HIPCHK(hipEventRecord(ctx->ev[ei], st));
-      HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, n, ctx->tcount.p, ntx, ntz, ctx->tin.p, ctx->tkey.p,
-                        ctx->tlist[nxt].p, ctx->tcount.p + 1, ctx->max_sweeps, ctx->colors, ctx->acc.p, st));
-      HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
-      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      ctx->stats.tiles += ctx->h_count[0];
-      ctx->trace.push_back(ctx->h_count[0]);
-      n = ctx->h_count[1];
-      cur = nxt;
-      launches++;
     }
-    ctx->stats.launches += launches;
+    {
+      // due tiles per launch (the last one is still in the due counter)
+      std::vector<int> tr(launches), mx(launches);
+      if (launches > 1)
+        HIPCHK(hipMemcpy(tr.data(), ctx->ttrace.p, sizeof(int) * (launches - 1), hipMemcpyDeviceToHost));
+      if (launches > 0) {
+        tr[launches - 1] = ctx->h_count[0];
+        HIPCHK(hipMemcpy(mx.data(), ctx->tmax.p, sizeof(int) * launches, hipMemcpyDeviceToHost));
+      }
+      // launches queued after the set emptied (rest of the last batch) had nothing due
+      int used = 0;
+      for (int k = 0; k < launches; k++)
+        if (tr[k] > 0) {
+          ctx->stats.tiles += tr[k];
+          ctx->trace.push_back(tr[k]);
+          ctx->trace_sweeps.push_back(mx[k]);
+          used++;
+        }
+      ctx->stats.launches += used;
+    }
     // heap-exhaustion stop of the reference (src/propagate2d.f90:74-80)
     {
       HIPCHK(launch_member_time(dl, ctx->regs.p, nf, nx, nz, st));
@@ -385,14 +415,15 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   ctx->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   double fim_ms = 0.0;
-  for (int k = 0; k < ctx->stats.launches; k++) {
+  for (size_t k = 0; k < nev; k++) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
     fim_ms += ms;
   }
   ctx->stats.ms_fim = fim_ms;
-  unsigned long long acc[3];
+  unsigned long long acc[3 + 64];
   HIPCHK(hipMemcpy(acc, ctx->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 64; k++) ctx->sweep_hist[k] = (long long)acc[3 + k];
   ctx->stats.sweeps = (long long)acc[0];
   ctx->stats.max_sweeps = (int)acc[1];
   ctx->stats.capped = (long long)acc[2];
@@ -400,7 +431,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
 }
 
 int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors) {
-  if (!ctx || !(window_tiles > 0.0) || max_sweeps < 1 || (colors != 1 && colors != 4))
+  if (!ctx || !(window_tiles > 0.0) || max_sweeps < 1 || (colors != 1 && colors != 2 && colors != 4))
     return fail(FMT2D_EINVAL, "bad tuning parameters");
   ctx->colors = colors;
   ctx->window_tiles = window_tiles;
@@ -408,11 +439,19 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
   return FMT2D_OK;
 }
 
-int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int max, int* n) {
+int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, int* n) {
   if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
   *n = (int)ctx->trace.size();
-  if (counts)
-    for (int i = 0; i < std::min(max, *n); i++) counts[i] = ctx->trace[i];
+  for (int i = 0; i < std::min(max, *n); i++) {
+    if (counts) counts[i] = ctx->trace[i];
+    if (max_sweeps) max_sweeps[i] = ctx->trace_sweeps[i];
+  }
+  return FMT2D_OK;
+}
+
+int fmt2d_forward_sweep_hist(fmt2d_ctx* ctx, long long* hist) {
+  if (!ctx || !hist) return fail(FMT2D_EINVAL, "NULL argument");
+  for (int k = 0; k < 64; k++) hist[k] = ctx->sweep_hist[k];
   return FMT2D_OK;
 }
 

@@ -84,10 +84,12 @@ struct fmt2d_ctx {
   DBuf<unsigned long long> tkey;      // tile arrival key
   DBuf<unsigned long long> tfront;    // per-field front key
   DBuf<int2> tlist[3];                // active set, its next state, due tiles
-  DBuf<int> tcount;                   // [due, next]
-  double window_tiles = 2.0;          // window in tile crossing times
+  DBuf<int> tcount;                   // [due, size of list 0, size of list 1]
+  DBuf<int> ttrace;                   // due tiles per launch of a level
+  DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
+  double window_tiles = 3.0;          // window in tile crossing times
   int max_sweeps = 64;
-  int colors = 4;                     // 4: four-colour Gauss-Seidel in a tile, 1: Jacobi
+  int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;
@@ -141,6 +143,8 @@ struct fmt2d_ctx {
   std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
   fmt2d_forward_stats stats{};
   std::vector<int> trace;              // active tiles per launch (all levels)
+  std::vector<int> trace_sweeps;       // most sweeps of a tile visit per launch
+  long long sweep_hist[64] = {};       // tile visits by sweep count (last bin: 64 or more)
   ~fmt2d_ctx() {
     for (auto e : ev) (void)hipEventDestroy(e);
     if (h_count) (void)hipHostFree(h_count);

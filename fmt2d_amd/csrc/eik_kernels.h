@@ -61,12 +61,16 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
                              unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
                              hipStream_t st);
-hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int ntiles, unsigned long long* tkey,
-                           unsigned long long* front, double window, int color, unsigned int* in_set, int2* due,
-                           int* count_due, int2* carry, int* count_carry, hipStream_t st);
-hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
+// per launch: clear the due / next counters and field fronts, then split
+// the active list (its count on the device) into due tiles and carry-over
+hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
+                           unsigned long long* tkey, unsigned long long* front, double window, int color,
+                           unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
+                           int* trace_prev, int grid, hipStream_t st);
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, hipStream_t st);
+                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
+                      hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

@@ -249,6 +249,19 @@ __device__ inline unsigned long long key_of(double t) {
   return t <= 0.0 ? 0ull : (unsigned long long)__double_as_longlong(t);
 }
 
+// Position for this lane's item in an append list: one atomic per wave.
+// Every lane of the wave must call it (pred false for lanes with no item).
+__device__ inline int wave_append(int* counter, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, 64);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
 __device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key, unsigned int* in_set,
                                 unsigned long long* tkey, int2* list_next, int* count_next) {
   const long long k = (long long)f * ntiles + t;
@@ -293,52 +306,67 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
       if ((j % TILE) == 0 || j == j0) add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key, in_set, tkey, list_next, count_next);
 }
 
-__global__ void reset_front_kernel(unsigned long long* front, int nfield) {
+// Clears the counters of a launch: due tiles and the next active list.
+// The previous launch's due count goes to its trace slot (when given).
+__global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned long long* front, int nfield,
+                                    int* trace_prev) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f == 0) {
+    if (trace_prev) *trace_prev = *count_due;
+    *count_due = 0;
+    *count_next = 0;
+  }
   if (f < nfield) front[f] = KEY_INF;
 }
 
-__global__ void front_kernel(const int2* __restrict__ list, int n, int ntiles, const unsigned long long* __restrict__ tkey,
-                             unsigned long long* front) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int2 e = list[k];
-  atomicMin(front + e.x, tkey[(long long)e.x * ntiles + e.y]);
+// earliest pending key per field over the active list (count on the device)
+__global__ void front_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntiles,
+                             const unsigned long long* __restrict__ tkey, unsigned long long* front) {
+  const int n = *count;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int2 e = list[k];
+    atomicMin(front + e.x, tkey[(long long)e.x * ntiles + e.y]);
+  }
 }
 
 // Splits the active set into tiles due this launch and tiles carried over.
 // Tiles of one checkerboard colour run per launch, so two edge-adjacent
 // tiles never iterate against each other's stale halos (which can lock a
 // pair into a two-cycle of the non-monotone second-order switch).
-__global__ void split_kernel(const int2* __restrict__ list, int n, int ntx, int ntiles, unsigned long long* tkey,
-                             const unsigned long long* __restrict__ front, double window, int color,
-                             unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int2 e = list[k];
-  const long long q = (long long)e.x * ntiles + e.y;
-  const double key = __longlong_as_double((long long)tkey[q]);
-  const double fr = __longlong_as_double((long long)front[e.x]);
-  const int tc = ((e.y % ntx) + (e.y / ntx)) & 1;
-  if (key <= fr + window && tc == color) {
-    tkey[q] = KEY_INF;
-    in_set[q] = 0u;
-    due[atomicAdd(count_due, 1)] = e;
-  } else {
-    carry[atomicAdd(count_carry, 1)] = e;
+__global__ void split_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntx, int ntiles,
+                             unsigned long long* tkey, const unsigned long long* __restrict__ front, double window,
+                             int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
+                             int* count_carry) {
+  const int n = *count;
+  const int stride = gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of rounds (wave_append)
+  const int rounds = (n + stride - 1) / stride;
+  for (int r = 0; r < rounds; r++) {
+    const int k = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    bool is_due = false, is_carry = false;
+    int2 e = make_int2(0, 0);
+    if (k < n) {
+      e = list[k];
+      const long long q = (long long)e.x * ntiles + e.y;
+      const double key = __longlong_as_double((long long)tkey[q]);
+      const double fr = __longlong_as_double((long long)front[e.x]);
+      const int tc = ((e.y % ntx) + (e.y / ntx)) & 1;
+      is_due = key <= fr + window && tc == color;
+      is_carry = !is_due;
+      if (is_due) { tkey[q] = KEY_INF; in_set[q] = 0u; }
+    }
+    const int pd = wave_append(count_due, is_due);
+    const int pc = wave_append(count_carry, is_carry);
+    if (is_due) due[pd] = e;
+    if (is_carry) carry[pc] = e;
   }
 }
 
-__global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
-                                                       const RegionDev* __restrict__ regs, const int2* __restrict__ list,
-                                                       const int* __restrict__ count_due, int ntx, int ntz,
-                                                       unsigned int* in_set, unsigned long long* tkey,
-                                                       int2* list_next, int* count_next, int max_sweeps,
-                                                       int colors, unsigned long long* acc) {
-  __shared__ TileLds L;
-  // the grid is sized for the whole active set; only the due tiles run
-  if ((int)blockIdx.x >= *count_due) return;
-  const int2 ft = list[blockIdx.x];
+// One visit of tile ft: stage the window, sweep to the tile's local fixed
+// point, write back, activate the neighbours whose halo changed.
+__device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
+                         int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
+                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, TileLds& L) {
   const int f = ft.x, t = ft.y;
   const FieldDesc& d = descs[f];
   const RegionDev& R = regs[d.region];
@@ -396,9 +424,12 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   bool more = true;
   // Sweeps.  colors == 4: Gauss-Seidel in five phases, the four parity
   // colours of the grid nodes (a node's x/z neighbours and cell diagonals
-  // have other colours), then the interface nodes.  colors == 1: Jacobi, all
-  // grid nodes from the same state, then the interface nodes.
-  const int nphase = colors == 4 ? 5 : 2;
+  // have other colours), then the interface nodes.  colors == 2: the two
+  // checkerboard colours (x/z neighbours differ), then the interface nodes.
+  // colors == 1: Jacobi, all grid nodes from the same state.  Within a
+  // phase the single wave computes every active node before any writes, so
+  // same-colour reads see the phase's starting state.
+  const int nphase = colors + 1;
   int sweep = 0;
   for (; sweep < max_sweeps; sweep++) {
     bool ch = false;
@@ -407,7 +438,7 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
       if (grid_phase) {
         bool upd = false;
         Best<double> b;
-        if (g_act && (colors == 1 || g_color == phase)) {
+        if (g_act && (colors == 1 || (colors == 2 ? ((gi + gj) & 1) : g_color) == phase)) {
           b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
           update_grid_node<double>(v, gi, gj, b);
           upd = __double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
@@ -435,9 +466,14 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
     if (!more) break;
   }
   if (lane == 0 && acc) {
+    // acc: [0] sweeps, [1] most sweeps of a visit, [2] visits that hit the
+    // cap, [3 + k] visits of k + 1 sweeps (last bin: 64 or more)
+    const int ns = min(sweep + 1, 64);
     atomicAdd(acc, (unsigned long long)(sweep + 1));
     atomicMax(acc + 1, (unsigned long long)(sweep + 1));
     if (more) atomicAdd(acc + 2, 1ull);
+    atomicAdd(acc + 3 + ns - 1, 1ull);
+    if (launch_max) atomicMax(launch_max, sweep + 1);
   }
   // ---- write back and activate neighbours whose halo changed
   if (g_dirty) {
@@ -472,11 +508,29 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
   }
   if (lane < 9 && (mask & (1u << lane))) {
     const int tix = t % ntx, tiz = t / ntx;
-    const int dxs[9] = {-1, 1, 0, 0, -1, 1, -1, 1, 0};
-    const int dzs[9] = {0, 0, -1, 1, -1, -1, 1, 1, 0};
-    const int nxi = tix + dxs[lane], nzi = tiz + dzs[lane];
+    // mask bit k -> tile offset: left, right, up, down, the four corners, self
+    // (2-bit fields of offset + 1)
+    const int ddx = (int)((0x18858u >> (2 * lane)) & 3u) - 1, ddz = (int)((0x1a085u >> (2 * lane)) & 3u) - 1;
+    const int nxi = tix + ddx, nzi = tiz + ddz;
     if (nxi >= 0 && nxi < ntx && nzi >= 0 && nzi < ntz)
       add_tile(f, nzi * ntx + nxi, ntiles, key_of(kmin), in_set, tkey, list_next, count_next);
+  }
+}
+
+// Persistent-style grid: blocks take the due tiles of this launch in turn
+// (the due count is known only on the device).
+__global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
+                                                       const RegionDev* __restrict__ regs, const int2* __restrict__ list,
+                                                       const int* __restrict__ count_due, int ntx, int ntz,
+                                                       unsigned int* in_set, unsigned long long* tkey,
+                                                       int2* list_next, int* count_next, int max_sweeps,
+                                                       int colors, unsigned long long* acc, int* launch_max) {
+  __shared__ TileLds L;
+  const int n = *count_due;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    fim_tile(descs, regs, list[k], ntx, ntz, in_set, tkey, list_next, count_next, max_sweeps, colors, acc, launch_max,
+             L);
+    __syncthreads();
   }
 }
 
@@ -912,21 +966,24 @@ hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int 
   return hipGetLastError();
 }
 
-hipError_t launch_schedule(const int2* list, int n, int nfield, int ntx, int ntiles, unsigned long long* tkey,
-                           unsigned long long* front, double window, int color, unsigned int* in_set, int2* due,
-                           int* count_due, int2* carry, int* count_carry, hipStream_t st) {
-  hipLaunchKernelGGL(reset_front_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, front, nfield);
-  hipLaunchKernelGGL(front_kernel, dim3((n + 255) / 256), dim3(256), 0, st, list, n, ntiles, tkey, front);
-  hipLaunchKernelGGL(split_kernel, dim3((n + 255) / 256), dim3(256), 0, st, list, n, ntx, ntiles, tkey, front, window,
-                     color, in_set, due, count_due, carry, count_carry);
+hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
+                           unsigned long long* tkey, unsigned long long* front, double window, int color,
+                           unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
+                           int* trace_prev, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(launch_reset_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, count_due, count_carry, front,
+                     nfield, trace_prev);
+  hipLaunchKernelGGL(front_kernel, dim3(grid), dim3(256), 0, st, list, count, ntiles, tkey, front);
+  hipLaunchKernelGGL(split_kernel, dim3(grid), dim3(256), 0, st, list, count, ntx, ntiles, tkey, front, window, color,
+                     in_set, due, count_due, carry, count_carry);
   return hipGetLastError();
 }
 
-hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int nmax, const int* count_due,
+hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, hipStream_t st) {
-  hipLaunchKernelGGL(fim_tiles_kernel, dim3(nmax), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
-                     tkey, list_next, count_next, max_sweeps, colors, acc);
+                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(fim_tiles_kernel, dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
+                     tkey, list_next, count_next, max_sweeps, colors, acc, launch_max);
   return hipGetLastError();
 }
 

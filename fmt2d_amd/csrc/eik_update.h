@@ -242,17 +242,27 @@ FMT_HD void member_data(const V& v, const Member& m, Real& x, Real& z, Real& t, 
 // region's top then bottom interface nodes in the cell, then the corners in
 // the region (src/propagate2d.f90:392-441).  `self` and `excl` (the node's
 // coincident partner, see update_grid_node) are left out, and members not yet
-// reached are skipped (they cannot lower the node).
+// reached are skipped (they cannot lower the node).  The members are returned
+// as a mask over six slots in that order: 0/1 top/bottom interface node,
+// 2..5 corners (cx+i, cz+j) with i outer, j inner (slot_member), so callers
+// walk them with bit scans and keep everything in registers.
+FMT_HD Member slot_member(int cx, int cz, int k) {
+  Member m;
+  if (k < 2) { m.kind = 1 + k; m.i = cx; m.j = 0; }
+  else { m.kind = 0; m.i = cx + ((k - 2) >> 1); m.j = cz + ((k - 2) & 1); }
+  return m;
+}
+
+FMT_HD int lowest_slot(unsigned m) { return __builtin_ctz(m); }
+
 template <class V>
-FMT_HD int irregular_members(const V& v, int cx, int cz, const Member& self, const Member& excl, Member* out) {
+FMT_HD unsigned irregular_members(const V& v, int cx, int cz, const Member& self, const Member& excl) {
   const RegionDev& R = v.reg();
-  int n = 0;
+  unsigned mask = 0;
   for (int s = 0; s < 2; s++) {
     if (v.ifcc(s, cx) == cz && v.ifok(s, cx)) {
       const bool me = (self.kind == 1 + s && self.i == cx) || (excl.kind == 1 + s && excl.i == cx);
-      if (!me && v.iT(s, cx) < FMT_HUGE_HALF) {
-        out[n].kind = 1 + s; out[n].i = cx; out[n].j = 0; n++;
-      }
+      if (!me && v.iT(s, cx) < FMT_HUGE_HALF) mask |= 1u << s;
     }
   }
   for (int i = 0; i <= 1; i++)
@@ -263,24 +273,25 @@ FMT_HD int irregular_members(const V& v, int cx, int cz, const Member& self, con
       if (excl.kind == 0 && excl.i == ii && excl.j == jj) continue;
       if (!v.in(ii, jj)) continue;
       if (v.T(ii, jj) >= FMT_HUGE_HALF) continue;
-      out[n].kind = 0; out[n].i = ii; out[n].j = jj; n++;
+      mask |= 1u << (2 + 2 * i + j);
     }
-  return n;
+  return mask;
 }
 
 template <typename Real, class V, class Vis>
 FMT_HD void irregular_cell(const V& v, int cx, int cz, const Member& self, const Member& excl, Real xu,
                            Real zu, Real vu, Vis& vis) {
-  Member m[8];
-  const int n = irregular_members(v, cx, cz, self, excl, m);
   const Real tol = (Real)v.reg().tol;
-  for (int a = 0; a < n; a++) {
+  // every member alone, then every later member paired with it, in slot order
+  for (unsigned ma = irregular_members(v, cx, cz, self, excl); ma; ma &= ma - 1) {
+    const Member A = slot_member(cx, cz, lowest_slot(ma));
     Real xa, za, ta, gax, gaz, va;
-    member_data<Real>(v, m[a], xa, za, ta, gax, gaz, va);
-    if (!v.fixed(m[a]) && vis.want(ta)) cand1<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, tol, vis);
-    for (int c = a + 1; c < n; c++) {
+    member_data<Real>(v, A, xa, za, ta, gax, gaz, va);
+    if (!v.fixed(A) && vis.want(ta)) cand1<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, tol, vis);
+    for (unsigned mc = ma & (ma - 1); mc; mc &= mc - 1) {
+      const Member Cm = slot_member(cx, cz, lowest_slot(mc));
       Real xc, zc, tc, gcx, gcz, vc;
-      member_data<Real>(v, m[c], xc, zc, tc, gcx, gcz, vc);
+      member_data<Real>(v, Cm, xc, zc, tc, gcx, gcz, vc);
       if (vis.want(ta > tc ? ta : tc))
         cand2<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, xc, zc, tc, gcx, gcz, vc, tol, vis);
     }
@@ -484,11 +495,10 @@ FMT_HD Real max_member_time_grid(const V& v, int iu, int ju) {
     if (!(fl & (RF_CELL0 << k))) continue;
     const int cx = iu + cell_dx(k), cz = ju + cell_dz(k);
     if (fl & (RF_IRR0 << k)) {
-      Member m[8];
-      const int n = irregular_members(v, cx, cz, self, none, m);
-      for (int a = 0; a < n; a++) {
-        if (v.fixed(m[a])) continue;
-        const Real t = m[a].kind == 0 ? (Real)v.T(m[a].i, m[a].j) : (Real)v.iT(m[a].kind - 1, m[a].i);
+      for (unsigned ma = irregular_members(v, cx, cz, self, none); ma; ma &= ma - 1) {
+        const Member A = slot_member(cx, cz, lowest_slot(ma));
+        if (v.fixed(A)) continue;
+        const Real t = A.kind == 0 ? (Real)v.T(A.i, A.j) : (Real)v.iT(A.kind - 1, A.i);
         mx = t > mx ? t : mx;
       }
     } else {
@@ -508,12 +518,12 @@ FMT_HD Real max_member_time_iface(const V& v, int s, int c) {
   if (c >= R.nx - 1 || v.ifcc(s, c) < 0) return -1;
   Member self; self.kind = 1 + s; self.i = c; self.j = 0;
   Member none; none.kind = -1; none.i = -1; none.j = -1;
-  Member m[8];
-  const int n = irregular_members(v, c, v.ifcc(s, c), self, none, m);
+  const int cz = v.ifcc(s, c);
   Real mx = -1;
-  for (int a = 0; a < n; a++) {
-    if (v.fixed(m[a])) continue;
-    const Real t = m[a].kind == 0 ? (Real)v.T(m[a].i, m[a].j) : (Real)v.iT(m[a].kind - 1, m[a].i);
+  for (unsigned ma = irregular_members(v, c, cz, self, none); ma; ma &= ma - 1) {
+    const Member A = slot_member(c, cz, lowest_slot(ma));
+    if (v.fixed(A)) continue;
+    const Real t = A.kind == 0 ? (Real)v.T(A.i, A.j) : (Real)v.iT(A.kind - 1, A.i);
     mx = t > mx ? t : mx;
   }
   return mx;

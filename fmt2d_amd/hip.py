@@ -70,7 +70,8 @@ def load() -> C.CDLL:
     lib.fmt2d_grid_info_get.argtypes = [C.c_void_p, C.POINTER(GridInfo)]
     lib.fmt2d_forward.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_forward_stats_get.argtypes = [C.c_void_p, C.POINTER(ForwardStats)]
-    lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
+    lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, _ip, C.c_int, _ip]
+    lib.fmt2d_forward_sweep_hist.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     lib.fmt2d_tf_region.argtypes = [C.c_int]
     lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
@@ -90,6 +91,7 @@ def load() -> C.CDLL:
 # exported symbols declared in include/fmt2d_hip.h
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
+           "fmt2d_forward_sweep_hist",
            "fmt2d_set_tuning",
            "fmt2d_tf_region",
            "fmt2d_get_field",
@@ -141,7 +143,7 @@ class Context:
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
 
-    def set_tuning(self, window_tiles: float = 2.0, max_sweeps: int = 64, colors: int = 4):
+    def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 64, colors: int = 2):
         _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
 
     def stats(self) -> ForwardStats:
@@ -149,14 +151,22 @@ class Context:
         _check(self._lib.fmt2d_forward_stats_get(self._h, C.byref(st)))
         return st
 
-    def trace(self):
+    def forward_trace(self):
+        """(due tiles, most sweeps of a tile visit) per launch of the last forward."""
         n = C.c_int()
-        _check(self._lib.fmt2d_forward_trace(self._h, None, 0, C.byref(n)))
-        out = np.zeros(max(n.value, 1), dtype=np.int32)
-        _check(self._lib.fmt2d_forward_trace(self._h, out.ctypes.data_as(_ip), n.value, C.byref(n)))
-        return out[:n.value]
+        _check(self._lib.fmt2d_forward_trace(self._h, None, None, 0, C.byref(n)))
+        due = np.zeros(max(n.value, 1), dtype=np.int32)
+        mx = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(self._lib.fmt2d_forward_trace(self._h, due.ctypes.data_as(_ip), mx.ctypes.data_as(_ip), n.value,
+                                             C.byref(n)))
+        return due[:n.value], mx[:n.value]
 
-    # ---- rays and Frechet rows
+    def sweep_hist(self):
+        """Tile visits of the last forward by sweep count (index k: k + 1 sweeps)."""
+        h = np.zeros(64, dtype=np.int64)
+        _check(self._lib.fmt2d_forward_sweep_hist(self._h, h.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return h
+
     def set_inversion(self, invert_velocity: bool = True, ifaces: Sequence[int] = ()):
         ids = np.ascontiguousarray(ifaces, dtype=np.int32)
         _check(self._lib.fmt2d_set_inversion(self._h, int(bool(invert_velocity)), len(ids),

@@ -97,10 +97,15 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags);
 int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
 /* Solver tuning: arrival-time window of the tile schedule in tile crossing
  * times (at 4 km/s), the sweep cap of one tile visit, and the in-tile sweep
- * order (4: four-colour Gauss-Seidel, 1: Jacobi).  Defaults 2.0, 64, 4. */
+ * order (4: four-colour Gauss-Seidel, 2: two-colour Gauss-Seidel, 1: Jacobi).
+ * Defaults 3.0, 64, 2. */
 int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors);
-/* Due-tile count of every solver launch of the last forward (diagnostics). */
-int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int max, int* n);
+/* Due-tile count and the most sweeps of one tile visit, per solver launch of
+ * the last forward (diagnostics); either array may be NULL. */
+int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, int* n);
+/* Tile visits of the last forward by sweep count: hist[k] visits of k + 1
+ * sweeps, hist[63] 64 or more (diagnostics). */
+int fmt2d_forward_sweep_hist(fmt2d_ctx* ctx, long long* hist);
 
 /* Region id (1..6) of time field itf (1..12). */
 int fmt2d_tf_region(int itf);
