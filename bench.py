@@ -87,6 +87,19 @@ def cpu_baseline(case_dir: str, obs_x, nsample: int):
             "sample": f"{len(sample)} OBS x 12 fields of the same model ({desc}), {sec:.2f} s"}
 
 
+def measured_traffic(workload_name: str):
+    """HBM bytes per tile-kernel dispatch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"fim_tiles_kernel_traffic_{workload_name}.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("hbm_bytes_per_dispatch:"):
+            return float(ln.split(":")[1])
+    return None
+
+
 def inversion_case_dir(d):
     """Plain inversion case in directory d (true and starting models, geometry
     without picks); returns (true model, starting model, geometry)."""
@@ -189,7 +202,20 @@ def main():
     ap.add_argument("--workload", default="multistage", choices=["multistage", "plain"])
     ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 times): the reference point that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "dispatches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -206,7 +232,7 @@ def main():
         obs = list(g.obs_x)
         for _ in range(args.warmup):
             ctx.forward(obs)
-        walls, fims, nodes, launches, tiles = [], [], 0, [], []
+        walls, fims, nodes, launches, tiles, disp = [], [], 0, [], [], []
         for _ in range(args.steps):
             t0 = time.perf_counter()
             ctx.forward(obs)
@@ -216,6 +242,7 @@ def main():
             nodes = st.region_nodes
             launches.append(st.launches)
             tiles.append(st.tiles)
+            disp.append(st.dispatches)
         cpu = cpu_baseline(d, obs, args.cpu_sample) if args.cpu_sample > 0 else None
         ctx.close()
 
@@ -225,7 +252,11 @@ def main():
     value = stages * args.steps / wall
     fim_s = float(np.mean(fims))
     alg_bytes = nodes * BYTES_PER_NODE
-    achieved = alg_bytes / fim_s / 1e9
+    # per dispatch: the step's algorithmic bytes spread over its dispatches,
+    # over the mean dispatch duration (the same ratio as bytes / summed time)
+    ndisp = float(np.mean(disp))
+    achieved = (alg_bytes / ndisp) / (fim_s / ndisp) / 1e9
+    traffic = measured_traffic(args.workload)
     out = {
         "metric": "eikonal source-stages/s",
         "value": round(value, 3),
@@ -242,12 +273,15 @@ def main():
         "config": {"workload": desc, "nobs": len(obs), "source_stages_per_step": stages,
                    "fields_per_step": 12 * len(obs), "region_nodes_per_step": int(nodes),
                    "solver_launches_per_step": int(np.mean(launches)),
-                   "tiles_per_step": int(np.mean(tiles)), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3)},
+                   "tiles_per_step": int(np.mean(tiles)), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3),
+                   "fim_kernel_dispatches_per_step": int(ndisp),
+                   "fim_kernel_us_per_dispatch": round(1e6 * fim_s / ndisp, 2)},
         "roofline": {"bound": "hbm", "kernel": "fim_tiles_kernel", "achieved": round(achieved, 3),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 6),
-                     "traffic": None,
+                     "traffic": traffic,
                      "algorithmic_bytes": f"{BYTES_PER_NODE} B per region node per field (FP64 velocity "
-                                          f"read, T and grad T written) = {alg_bytes} B per step"},
+                                          f"read, T and grad T written) = {alg_bytes} B per step, "
+                                          f"{alg_bytes / ndisp:.0f} B per dispatch"},
         "cpu_baseline": cpu,
         "iteration": it,
     }

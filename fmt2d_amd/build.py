@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libfmt2d_hip.so")
 
 ARCH = os.environ.get("FMT2D_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["eik_kernels.hip", "rays.hip", "csr.hip", "lsqr.hip"]
-HOST_SOURCES = ["capi.cpp", "capi_rays.cpp", "capi_lsqr.cpp", "model.cpp"]
+HOST_SOURCES = ["capi.cpp", "capi_rays.cpp", "capi_lsqr.cpp", "capi_step.cpp", "model.cpp"]
 # no contraction into fused multiply-adds: the eikonal operator reproduces the
 # reference's FP64 rounding
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-variable",

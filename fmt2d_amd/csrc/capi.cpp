@@ -63,9 +63,21 @@ int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, cons
   ctx->nsrc = 0;
   ctx->nray = 0;
   ctx->m = HostModel();
+  // a new model starts a new inversion: its initial velocities and seafloor
+  // are taken again at the next set-up
+  ctx->v_init.clear();
+  ctx->topo_init.clear();
   std::string e = build_model(ctx->m, nx_s, nz_s, xpos, topo, zrel, vel, vwater, nsed, sed_x, sed_z, nmoho, moho_x,
                               moho_z);
   if (!e.empty()) return fail(FMT2D_EINVAL, e);
+  return model_upload(ctx);
+}
+
+}  // extern "C"
+
+// Device copies of the host model (grid, regions, intersections, PmP
+// head-wave map); after fmt2d_set_model and after a model step.
+int model_upload(fmt2d_ctx* ctx) {
   HostModel& m = ctx->m;
   hipStream_t st = ctx->stream;
   const int nx = m.nx, nz = m.nz;
@@ -147,8 +159,11 @@ int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, cons
   }
   HIPCHK(hipStreamSynchronize(st));
   ctx->have_model = true;
+  ctx->rgeom_ok = false;
   return FMT2D_OK;
 }
+
+extern "C" {
 
 int fmt2d_grid_info_get(fmt2d_ctx* ctx, fmt2d_grid_info* info) {
   if (!ctx || !info) return fail(FMT2D_EINVAL, "NULL argument");
@@ -421,6 +436,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     fim_ms += ms;
   }
   ctx->stats.ms_fim = fim_ms;
+  ctx->stats.dispatches = (int)nev;
   unsigned long long acc[3 + 64];
   HIPCHK(hipMemcpy(acc, ctx->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
   for (int k = 0; k < 64; k++) ctx->sweep_hist[k] = (long long)acc[3 + k];

@@ -1,16 +1,133 @@
 // C ABI of the tomography system and its LSQR solve: data selection and
 // weighting (read_frechet_data2d and the scaling of lsqr_inversion2d,
-// src/inversion2d.f90:280-486, 3965-4052) and run_lsqr (:6578-7167).
+// src/inversion2d.f90:280-486, 3965-4052), the Gaussian smoothing operators
+// (:1252-1636) and run_lsqr (:6578-7167).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <utility>
 #include <string>
 #include <vector>
 
 #include "ctx.h"
+
+namespace {
+
+// CSR rows (1-based columns as in the reference, converted at upload)
+struct RowsCsr {
+  std::vector<int> rp, col;
+  std::vector<double> val;
+};
+
+RowsCsr pack_rows(std::vector<std::vector<std::pair<int, double>>>& rows) {
+  RowsCsr r;
+  r.rp.assign(rows.size() + 1, 0);
+  for (size_t i = 0; i < rows.size(); i++) r.rp[i + 1] = r.rp[i] + (int)rows[i].size();
+  r.col.reserve(r.rp.back());
+  r.val.reserve(r.rp.back());
+  for (auto& row : rows)
+    for (auto& e : row) { r.col.push_back(e.first - 1); r.val.push_back(e.second); }
+  return r;
+}
+
+// build_gauss_smooth_matrix (inversion2d.f90:1252-1402): for every inverted
+// velocity node, R_h holds its row neighbours within int(3 corr_x / dx) + 1
+// columns and R_v its column neighbours within 3 corr_z, ascending, each
+// exp(-d^2 / corr^2) * mvscale(neighbour), then the diagonal
+// -sum(beta) * mvscale(node).
+void gauss_rows(const std::vector<double>& vx, const std::vector<double>& vz, const std::vector<int>& inv, int vnx,
+                int vnz, double corr_x, double corr_z, const std::vector<double>& mvs, RowsCsr& rh, RowsCsr& rv) {
+  const int nparm = (int)mvs.size();
+  std::vector<std::vector<std::pair<int, double>>> h(nparm), v(nparm);
+  const double dx = vx[1] - vx[0];
+  const int kmin = -(int)(3.0 * corr_x / dx) - 1, kmax = (int)(3.0 * corr_x / dx) + 1;
+  auto at = [&](int ix, int iz) { return inv[(size_t)(iz - 1) * vnx + (ix - 1)]; };  // 1-based
+  for (int iz = 1; iz <= vnz; iz++)
+    for (int ix = 1; ix <= vnx; ix++) {
+      const int idx = at(ix, iz);
+      if (idx <= 0) continue;
+      double bsum = 0.0;
+      auto& rowh = h[idx - 1];
+      for (int jx = std::max(1, ix + kmin); jx <= std::min(vnx, ix + kmax); jx++) {
+        if (jx == ix) continue;
+        const int jdx = at(jx, iz);
+        if (jdx <= 0) continue;
+        const double d = vx[jx - 1] - vx[ix - 1];
+        const double beta = std::exp(-(d * d) / (corr_x * corr_x));
+        bsum = bsum + beta;
+        rowh.push_back({jdx, beta * mvs[jdx - 1]});
+      }
+      rowh.push_back({idx, -bsum * mvs[idx - 1]});
+      bsum = 0.0;
+      auto& rowv = v[idx - 1];
+      for (int jz = 1; jz <= vnz; jz++) {
+        if (jz == iz) continue;
+        const double d = vz[jz - 1] - vz[iz - 1];
+        if (std::fabs(d) > 3.0 * corr_z) continue;
+        const int jdx = at(ix, jz);
+        if (jdx <= 0) continue;
+        const double beta = std::exp(-(d * d) / (corr_z * corr_z));
+        bsum = bsum + beta;
+        rowv.push_back({jdx, beta * mvs[jdx - 1]});
+      }
+      rowv.push_back({idx, -bsum * mvs[idx - 1]});
+    }
+  rh = pack_rows(h);
+  rv = pack_rows(v);
+}
+
+// build_iface_gauss_smooth_matrix (inversion2d.f90:1524-1616): the same
+// 1-D rows along each inverted interface's input nodes.
+RowsCsr iface_gauss_rows(const fmt2d_ctx* ctx, double corr, const std::vector<double>& mvs) {
+  const int nparm = (int)mvs.size();
+  std::vector<std::vector<std::pair<int, double>>> rows(nparm);
+  for (int k : ctx->inv_ifaces) {
+    const int start = ctx->hgeom.if_start[k], nxi = ctx->hgeom.if_nnode[k];
+    const std::vector<double>& x = ctx->m.if_x[k];
+    const double dx = x[1] - x[0];
+    const int kmin = -(int)(3.0 * corr / dx) - 1, kmax = (int)(3.0 * corr / dx) + 1;
+    for (int ix = 1; ix <= nxi; ix++) {
+      const int idx = start + ix;
+      if (idx < 1 || idx > nparm) continue;
+      double bsum = 0.0;
+      auto& row = rows[idx - 1];
+      for (int jx = std::max(1, ix + kmin); jx <= std::min(nxi, ix + kmax); jx++) {
+        if (jx == ix) continue;
+        const int jdx = start + jx;
+        if (jdx < 1 || jdx > nparm) continue;
+        const double d = x[jx - 1] - x[ix - 1];
+        const double beta = std::exp(-(d * d) / (corr * corr));
+        bsum = bsum + beta;
+        row.push_back({jdx, beta * mvs[jdx - 1]});
+      }
+      row.push_back({idx, -bsum * mvs[idx - 1]});
+    }
+  }
+  return pack_rows(rows);
+}
+
+// column copy of a square CSR, rows ascending within a column (the order in
+// which the reference's transposed products scatter)
+void csc_of(const RowsCsr& a, int n, std::vector<int>& cp, std::vector<int>& row, std::vector<double>& val) {
+  cp.assign(n + 1, 0);
+  for (int c : a.col) cp[c + 1]++;
+  for (int j = 0; j < n; j++) cp[j + 1] += cp[j];
+  row.resize(a.col.size());
+  val.resize(a.col.size());
+  std::vector<int> fill(cp.begin(), cp.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++) {
+      const int o = fill[a.col[k]]++;
+      row[o] = i;
+      val[o] = a.val[k];
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -45,16 +162,34 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
     rp.push_back(rp.back() + (rp_all[r + 1] - rp_all[r]));
   }
   const int nd = (int)keep.size(), nz_ = rp.back();
+  // RMS of the selected, unweighted residuals (compute_rms2d, :332, :3819-3839)
+  {
+    double ss = 0.0;
+    for (int i = 0; i < nd; i++) {
+      const double rc = t_obs[keep[i]] - ctx->h_tcalc[keep[i]];
+      ss = ss + rc * rc;
+    }
+    ctx->sys_rms = nd > 0 ? sqrt(ss / (double)nd) : 0.0;
+    ctx->sys_crit_chi = p->crit_chi;
+    ctx->sys_alt_mode = p->alt_mode;
+  }
   // parameter scaling: 1/max(v_init,1) on the velocity grid, refl_weight on
   // interface nodes, zero for the parameters the alternating mode fixes
   std::vector<double> mvs(nparm, 1.0);
   {
     std::vector<int> inv((size_t)ctx->hgeom.vnx * ctx->hgeom.vnz);
     HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
+    if (ctx->v_init.size() != m.vgrid.size()) ctx->v_init = m.vgrid;
+    const std::vector<double>& vi = ctx->v_init;
     for (size_t q = 0; q < inv.size(); q++)
-      if (inv[q] > 0) mvs[inv[q] - 1] = 1.0 / std::max(m.vgrid[q], 1.0);
+      if (inv[q] > 0) mvs[inv[q] - 1] = 1.0 / std::max(vi[q], 1.0);
     for (int j = n_vel; j < nparm; j++) mvs[j] = p->refl_weight;
     if (p->alt_mode == 1) for (int j = n_vel; j < nparm; j++) mvs[j] = 0.0;
+    // the smoothing operators carry the velocity scaling even when the
+    // velocity columns are fixed (lsqr_inversion2d recomputes it, :451-463)
+    ctx->h_mvs_smooth = mvs;
+    ctx->h_mvs_model = mvs;
+    for (int j = n_vel; j < nparm; j++) ctx->h_mvs_model[j] = p->refl_weight;
     if (p->alt_mode == 2) for (int j = 0; j < n_vel; j++) mvs[j] = 0.0;
   }
   DBuf<int> dkeep;
@@ -101,11 +236,22 @@ int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual) {
 
 int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats) {
+  return lsqr_solve(ctx, p, dm_sum, dm_out, stats);
+}
+
+}  // extern "C"
+
+// One run_lsqr solve of the built system (fmt2d_lsqr; the damping search
+// calls it repeatedly).
+int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
+               fmt2d_lsqr_stats* stats) {
   if (!ctx || !p || !dm_sum || !dm_out) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
-  if (p->smooth_type != 1)
-    return fail(FMT2D_EINVAL, "smooth_type 1 (Laplacian) is supported; Gaussian smoothing (2, 3) is not");
+  if (p->smooth_type < 1 || p->smooth_type > 3)
+    return fail(FMT2D_EINVAL, "smooth_type must be 1 (Laplacian), 2 (Gaussian) or 3 (hybrid)");
   if (p->max_iter < 1) return fail(FMT2D_EINVAL, "max_iter must be >= 1");
+  if (p->smooth_type == 2 && p->w_smooth > 0.0 && !(p->corr_x > 0.0 && p->corr_z > 0.0))
+    return fail(FMT2D_EINVAL, "Gaussian smoothing needs corr_x > 0 and corr_z > 0");
   HIPCHK(hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = ctx->stream;
@@ -151,6 +297,39 @@ int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
       }
     }
   }
+  // Gaussian operators: built on first use for this parameterisation and
+  // these correlation lengths, with the scaling of that system, then kept
+  // (the reference builds them once, inversion2d.f90:516-522)
+  const bool want_g = p->smooth_type == 2 && p->w_smooth > 0.0;
+  const bool want_i = (p->smooth_type == 2 || p->smooth_type == 3) && p->w_smooth_iface > 0.0 && p->corr_x_iface > 0.0;
+  if (want_g && !(ctx->gs_nparm == nparm && ctx->gs_key[0] == p->corr_x && ctx->gs_key[1] == p->corr_z)) {
+    std::vector<int> inv((size_t)ctx->hgeom.vnx * ctx->hgeom.vnz);
+    HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
+    std::vector<double> vz(ctx->hgeom.vnz);
+    for (int j = 0; j < ctx->hgeom.vnz; j++) vz[j] = (double)j * ctx->m.dz;
+    RowsCsr rh, rv;
+    gauss_rows(ctx->m.vx, vz, inv, ctx->hgeom.vnx, ctx->hgeom.vnz, p->corr_x, p->corr_z, ctx->h_mvs_smooth, rh, rv);
+    std::vector<int> cp, row;
+    std::vector<double> cv;
+    HIPCHK(upload(ctx->gh_rp, rh.rp, st)); HIPCHK(upload(ctx->gh_col, rh.col, st)); HIPCHK(upload(ctx->gh_val, rh.val, st));
+    csc_of(rh, nparm, cp, row, cv);
+    HIPCHK(upload(ctx->gh_cp, cp, st)); HIPCHK(upload(ctx->gh_row, row, st)); HIPCHK(upload(ctx->gh_cval, cv, st));
+    HIPCHK(upload(ctx->gv_rp, rv.rp, st)); HIPCHK(upload(ctx->gv_col, rv.col, st)); HIPCHK(upload(ctx->gv_val, rv.val, st));
+    csc_of(rv, nparm, cp, row, cv);
+    HIPCHK(upload(ctx->gv_cp, cp, st)); HIPCHK(upload(ctx->gv_row, row, st)); HIPCHK(upload(ctx->gv_cval, cv, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->gs_nparm = nparm;
+    ctx->gs_key[0] = p->corr_x;
+    ctx->gs_key[1] = p->corr_z;
+  }
+  if (want_i && !(ctx->gsi_nparm == nparm && ctx->gs_key[2] == p->corr_x_iface)) {
+    RowsCsr ri = iface_gauss_rows(ctx, p->corr_x_iface, ctx->h_mvs_smooth);
+    HIPCHK(upload(ctx->gi_rp, ri.rp, st)); HIPCHK(upload(ctx->gi_col, ri.col, st)); HIPCHK(upload(ctx->gi_val, ri.val, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->gsi_nparm = nparm;
+    ctx->gsi_nnz = (int)ri.col.size();
+    ctx->gs_key[2] = p->corr_x_iface;
+  }
   std::vector<double> dms(dm_sum, dm_sum + nparm);
   HIPCHK(upload(ctx->l_mask, mask, st));
   HIPCHK(upload(ctx->l_ws, ws, st));
@@ -158,11 +337,11 @@ int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   HIPCHK(upload(ctx->l_lapnb, lap, st));
   HIPCHK(upload(ctx->l_ifnb, ifn, st));
   HIPCHK(upload(ctx->l_dmsum, dms, st));
-  const size_t na = (size_t)nd + 2 * (size_t)nparm;
+  const size_t na = (size_t)nd + (p->smooth_type == 2 ? 3 : 2) * (size_t)nparm;
   if (ctx->l_u.n < na) HIPCHK(ctx->l_u.alloc(na));
   if (ctx->l_v.n < (size_t)nparm) {
     HIPCHK(ctx->l_v.alloc(nparm)); HIPCHK(ctx->l_w.alloc(nparm)); HIPCHK(ctx->l_dm.alloc(nparm));
-    HIPCHK(ctx->l_gtu.alloc(nparm)); HIPCHK(ctx->l_sm.alloc(nparm));
+    HIPCHK(ctx->l_gtu.alloc(nparm)); HIPCHK(ctx->l_sm.alloc(nparm)); HIPCHK(ctx->l_sm2.alloc(nparm));
   }
   if (ctx->l_gv.n < (size_t)std::max(nd, 1)) HIPCHK(ctx->l_gv.alloc((size_t)std::max(nd, 1)));
   if (!ctx->l_sc.p) HIPCHK(ctx->l_sc.alloc(1));
@@ -174,7 +353,22 @@ int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   S.lap_nb = ctx->l_lapnb.p; S.if_nb = ctx->l_ifnb.p;
   S.smooth_on = (p->w_smooth > 0.0 || p->w_smooth_iface > 0.0) ? 1 : 0;
   S.damp_on = (p->wdv > 0.0 || p->wdv_iface > 0.0) ? 1 : 0;
-  LsqrWork W{ctx->l_u.p, ctx->l_v.p, ctx->l_w.p, ctx->l_dm.p, ctx->l_gv.p, ctx->l_gtu.p, ctx->l_sm.p, ctx->l_sc.p};
+  S.stype = p->smooth_type;
+  S.w_smooth = p->w_smooth;
+  S.w_smooth_iface = p->w_smooth_iface;
+  S.rh_rp = S.rh_col = S.rh_cp = S.rh_crow = nullptr; S.rh_val = S.rh_cval = nullptr;
+  S.rv_rp = S.rv_col = S.rv_cp = S.rv_crow = nullptr; S.rv_val = S.rv_cval = nullptr;
+  S.rif_rp = S.rif_col = nullptr; S.rif_val = nullptr;
+  if (want_g) {
+    S.rh_rp = ctx->gh_rp.p; S.rh_col = ctx->gh_col.p; S.rh_val = ctx->gh_val.p;
+    S.rh_cp = ctx->gh_cp.p; S.rh_crow = ctx->gh_row.p; S.rh_cval = ctx->gh_cval.p;
+    S.rv_rp = ctx->gv_rp.p; S.rv_col = ctx->gv_col.p; S.rv_val = ctx->gv_val.p;
+    S.rv_cp = ctx->gv_cp.p; S.rv_crow = ctx->gv_row.p; S.rv_cval = ctx->gv_cval.p;
+  }
+  if (want_i) { S.rif_rp = ctx->gi_rp.p; S.rif_col = ctx->gi_col.p; S.rif_val = ctx->gi_val.p; }
+  S.gsi = (p->smooth_type == 2 && want_i) ? 1 : 0;
+  LsqrWork W{ctx->l_u.p, ctx->l_v.p,  ctx->l_w.p,   ctx->l_dm.p, ctx->l_gv.p,
+             ctx->l_gtu.p, ctx->l_sm.p, ctx->l_sm2.p, ctx->l_sc.p};
   LsqrScalars out{};
   HIPCHK(lsqr_run(S, W, p->max_iter, 1.0e-4, 8, st, &out));
   HIPCHK(hipMemcpy(dm_out, ctx->l_dm.p, sizeof(double) * nparm, hipMemcpyDeviceToHost));
@@ -188,4 +382,4 @@ int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   return FMT2D_OK;
 }
 
-}  // extern "C"
+

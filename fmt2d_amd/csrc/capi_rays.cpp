@@ -57,7 +57,10 @@ int build_ray_geom(fmt2d_ctx* ctx) {
   G.vx = ctx->rvx.p; G.vz = ctx->rvz.p;
   std::vector<int> inv((size_t)vnx * vnz, 0);
   int np = 0;
-  const std::vector<double>& topo = m.if_z[2];
+  // the numbering is fixed when the inversion is set up (a later seafloor
+  // update does not renumber, frechet2d.f90:44-60)
+  if (ctx->topo_init.size() != m.if_z[2].size()) ctx->topo_init = m.if_z[2];
+  const std::vector<double>& topo = ctx->topo_init;
   if (ctx->inv_vel) {
     for (int iz = 1; iz <= vnz; iz++)
       for (int ix = 1; ix <= vnx; ix++) {
@@ -149,6 +152,12 @@ int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const 
   ctx->inv_vel = invert_velocity ? 1 : 0;
   ctx->inv_ifaces = ids;
   ctx->rgeom_ok = false;
+  // velocity_init of the inversion: parameter scaling uses the model of the
+  // set-up, not the updated one (inversion2d.f90:186-191, 451-463)
+  ctx->v_init = ctx->m.vgrid;
+  ctx->topo_init = ctx->m.if_z[2];
+  ctx->gs_nparm = -1;
+  ctx->gsi_nparm = -1;
   return FMT2D_OK;
 }
 

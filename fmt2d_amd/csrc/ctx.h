@@ -126,11 +126,24 @@ struct fmt2d_ctx {
   // tomography system (selected, weighted, scaled rows) and its column copy
   int sys_ndata = -1, sys_nnz = 0;
   double sys_eps_min = 0.0;
+  double sys_rms = 0.0;                // RMS of the selected residuals (before weighting)
+  double sys_crit_chi = 0.0;
+  int sys_alt_mode = 0;
+  std::vector<double> h_mvs_model;     // mvscale of the model update (alternating zeroing undone)
   DBuf<int> s_rp, s_col, s_cp, s_crow;
   DBuf<double> s_val, s_cval, s_res, s_gtg, s_mvs;
   std::vector<double> h_gtg;
   // LSQR work arrays
-  DBuf<double> l_u, l_v, l_w, l_dm, l_gv, l_gtu, l_sm, l_dmsum, l_ws, l_wd;
+  DBuf<double> l_u, l_v, l_w, l_dm, l_gv, l_gtu, l_sm, l_sm2, l_dmsum, l_ws, l_wd;
+  // Gaussian smoothing operators, built once per parameterisation and
+  // correlation lengths (row and column copies, 0-based)
+  std::vector<double> h_mvs_smooth;    // mvscale the operators carry (velocity_init based)
+  std::vector<double> v_init;          // velocity grid when the inversion was set up
+  std::vector<double> topo_init;       // seafloor then (fixes the velocity parameter numbering)
+  double gs_key[3] = {-1, -1, -1};     // corr_x, corr_z of R_h/R_v; [2] corr_x_iface of R_if
+  int gs_nparm = -1, gsi_nparm = -1, gsi_nnz = 0;
+  DBuf<int> gh_rp, gh_col, gh_cp, gh_row, gv_rp, gv_col, gv_cp, gv_row, gi_rp, gi_col;
+  DBuf<double> gh_val, gh_cval, gv_val, gv_cval, gi_val;
   DBuf<uint8_t> l_mask;
   DBuf<int4> l_lapnb;
   DBuf<int2> l_ifnb;
@@ -161,3 +174,6 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 
 // Frechet rows of the last trace on the device (capi_rays.cpp)
 int frechet_build(fmt2d_ctx* ctx);
+int model_upload(fmt2d_ctx* ctx);
+int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
+               fmt2d_lsqr_stats* stats);

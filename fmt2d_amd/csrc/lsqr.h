@@ -27,10 +27,21 @@ struct LsqrSys {
   const int4* lap_nb;     // velocity parameters: neighbours x-1, x+1, z-1, z+1 (-1 none)
   const int2* if_nb;      // interface parameters: (1 one-sided | 2 central | 0 none, neighbour)
   int smooth_on, damp_on;
+  // smoothing operator: 1 Laplacian, 2 Gaussian R_h / R_v blocks (R_if in
+  // the R_h block's interface rows), 3 Laplacian with R_if on interface rows
+  int stype;
+  double w_smooth, w_smooth_iface;
+  const int *rh_rp, *rh_col, *rh_cp, *rh_crow;   // R_h rows and columns (0-based)
+  const double *rh_val, *rh_cval;
+  const int *rv_rp, *rv_col, *rv_cp, *rv_crow;   // R_v
+  const double *rv_val, *rv_cval;
+  const int *rif_rp, *rif_col;                   // R_if rows (nullptr: not built)
+  const double* rif_val;
+  int gsi;                                       // smooth_type 2: R_if in use
 };
 
 struct LsqrWork {
-  double *u, *v, *w, *dm, *gv, *gtu, *sm;
+  double *u, *v, *w, *dm, *gv, *gtu, *sm, *sm2;
   LsqrScalars* sc;
 };
 

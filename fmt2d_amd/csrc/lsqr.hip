@@ -1,5 +1,5 @@
 // LSQR on the augmented, smoothed and damped tomography system, on the
-// device (run_lsqr, src/inversion2d.f90:6578-7167, smooth_type 1).
+// device (run_lsqr, src/inversion2d.f90:6578-7167, smooth_type 1, 2, 3).
 //
 // Krylov iterations amplify rounding differences (reordering a single dot
 // product moves the reference's model update by ~1%), so every reduction
@@ -41,11 +41,27 @@ __global__ void spmv_cols_kernel(const int* __restrict__ cp, const int* __restri
   x[j] = acc;
 }
 
+// row j of a Gaussian smoothing matrix times x, entries in stored order
+// (apply_gauss_smooth_h / _v / apply_iface_gauss_smooth_h, :1405-1521)
+__device__ inline double row_dot(const int* __restrict__ rp, const int* __restrict__ col,
+                                 const double* __restrict__ val, int j, const double* __restrict__ x) {
+  double a = 0.0;
+  for (int k = rp[j]; k < rp[j + 1]; k++) a = a + val[k] * x[col[k]];
+  return a;
+}
+
 // apply_laplacian_smooth_2d (:1161-1249), then the mask and weight of the
-// smoothing block: out = masked ? 0 : w_j * (L vin)_j
+// smoothing block: out = masked ? 0 : w_j * (L vin)_j.  smooth_type 3: the
+// interface rows are w_if * (R_if vin)_j instead (0 without w_if).
 __global__ void smooth_kernel(const LsqrSys S, const double* __restrict__ vin, double* out, const LsqrScalars* s) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= S.nparm || stopped(s)) return;
+  if (S.stype == 3 && j >= S.n_vel) {
+    double r = 0.0;
+    if (S.rif_rp) r = row_dot(S.rif_rp, S.rif_col, S.rif_val, j, vin);
+    out[j] = S.w_smooth_iface > 0.0 ? S.w_smooth_iface * r : 0.0;
+    return;
+  }
   double acc = 0.0;
   if (j < S.n_vel) {
     const int4 nb = S.lap_nb[j];
@@ -65,6 +81,36 @@ __global__ void smooth_kernel(const LsqrSys S, const double* __restrict__ vin, d
     }
   }
   out[j] = S.mask[j] ? 0.0 : S.ws[j] * acc;
+}
+
+// smooth_type 2 blocks of A x: H = w R_h x, V = w R_v x (masked columns 0),
+// R_h block's interface rows w_if R_if x when R_if is in use, else 0.  With
+// rhs, the jumping right-hand side -H, -V written into u (the reference
+// leaves -w*0 in the interface rows when R_if is not in use).
+__global__ void gauss_fwd_kernel(const LsqrSys S, const double* __restrict__ x, double* H, double* V, int rhs,
+                                 const LsqrScalars* s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= S.nparm || stopped(s)) return;
+  const bool masked = S.mask[j] != 0;
+  double h = 0.0, v = 0.0;
+  if (S.w_smooth > 0.0) {
+    double a = row_dot(S.rh_rp, S.rh_col, S.rh_val, j, x);
+    double b = row_dot(S.rv_rp, S.rv_col, S.rv_val, j, x);
+    if (masked) { a = 0.0; b = 0.0; }
+    h = S.w_smooth * a;
+    v = S.w_smooth * b;
+    if (rhs) { h = -h; v = -v; }
+  }
+  if (j >= S.n_vel) {
+    if (S.gsi) {
+      const double r = S.w_smooth_iface * row_dot(S.rif_rp, S.rif_col, S.rif_val, j, x);
+      h = rhs ? -r : r;
+    } else if (!rhs) {
+      h = 0.0;
+    }
+  }
+  H[j] = h;
+  V[j] = v;
 }
 
 // front-to-back sum of a[i]^2 into *dst.  One block: waves 1..3 square
@@ -104,15 +150,20 @@ __global__ void __launch_bounds__(256) seqnorm_kernel(const double* __restrict__
 
 // ---- element-wise steps
 
+// blocks of the augmented system: data, one (Laplacian / hybrid) or two
+// (Gaussian) smoothing blocks, damping
+__device__ inline int nblk(const LsqrSys& S) { return S.stype == 2 ? 2 : 1; }
+
 // initial right-hand side: u = [r; -smooth(dm_sum); masked ? 0 : -wd dm_sum]
+// (smooth_type 2: the smoothing blocks were written by gauss_fwd_kernel)
 __global__ void rhs_kernel(const LsqrSys S, const double* __restrict__ sm, double* u) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nd = S.ndata, np = S.nparm;
-  if (i >= nd + 2 * np) return;
+  const long long nd = S.ndata, np = S.nparm, nb = nblk(S);
+  if (i >= nd + (nb + 1) * np) return;
   if (i < nd) u[i] = S.res[i];
-  else if (i < nd + np) u[i] = S.smooth_on ? -sm[i - nd] : 0.0;
+  else if (i < nd + nb * np) { if (nb == 1) u[i] = S.smooth_on ? -sm[i - nd] : 0.0; }
   else {
-    const int j = (int)(i - nd - np);
+    const int j = (int)(i - nd - nb * np);
     u[i] = (S.damp_on && !S.mask[j]) ? -S.wd[j] * S.dm_sum[j] : 0.0;
   }
 }
@@ -120,15 +171,17 @@ __global__ void rhs_kernel(const LsqrSys S, const double* __restrict__ sm, doubl
 // u = (what == 0) u / beta ; (what == 1) u := Av - alpha u, Av assembled from
 // G v (gv), the smoothing block (sm) and the damping block
 __global__ void u_update_kernel(const LsqrSys S, const double* __restrict__ gv, const double* __restrict__ sm,
-                                const double* __restrict__ v, double* u, const LsqrScalars* s) {
+                                const double* __restrict__ sm2, const double* __restrict__ v, double* u,
+                                const LsqrScalars* s) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nd = S.ndata, np = S.nparm;
-  if (i >= nd + 2 * np || stopped(s)) return;
+  const long long nd = S.ndata, np = S.nparm, nb = nblk(S);
+  if (i >= nd + (nb + 1) * np || stopped(s)) return;
   double av;
   if (i < nd) av = gv[i];
-  else if (i < nd + np) av = S.smooth_on ? sm[i - nd] : 0.0;
+  else if (i < nd + np) av = (nb == 2 || S.smooth_on) ? sm[i - nd] : 0.0;
+  else if (i < nd + nb * np) av = sm2[i - nd - np];
   else {
-    const int j = (int)(i - nd - np);
+    const int j = (int)(i - nd - nb * np);
     av = (S.damp_on && !S.mask[j]) ? S.wd[j] * v[j] : 0.0;
   }
   u[i] = av - s->alpha * u[i];
@@ -143,14 +196,38 @@ __global__ void scale_kernel(double* a, long long n, const double* __restrict__ 
   a[i] = a[i] / b;
 }
 
-// v = (G^T u + smooth(u_L)) + damping(u_D) - beta_factor * v, masked to 0
+// column j of a Gaussian matrix (its transpose's row), rows ascending
+// (apply_gauss_smooth_h_T / _v_T scatter in row order)
+__device__ inline double col_dot(const int* __restrict__ cp, const int* __restrict__ row,
+                                 const double* __restrict__ val, int j, const double* __restrict__ x) {
+  double a = 0.0;
+  for (int k = cp[j]; k < cp[j + 1]; k++) a = a + val[k] * x[row[k]];
+  return a;
+}
+
+// v = (G^T u + smoothing blocks^T) + damping(u_D) - beta_factor * v, masked
+// to 0.  smooth_type 2: + w R_h^T u_H + w R_v^T u_V, then w_if R_if u_H on
+// the interface rows (R_if is applied untransposed, as the reference does).
 __global__ void v_update_kernel(const LsqrSys S, const double* __restrict__ gtu, const double* __restrict__ sm,
                                 const double* __restrict__ u, double* v, int first, const LsqrScalars* s) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= S.nparm || stopped(s)) return;
+  const long long nd = S.ndata, np = S.nparm;
   double a = gtu[j];
-  if (S.smooth_on) a = a + sm[j];
-  if (S.damp_on && !S.mask[j]) a = a + S.wd[j] * u[(long long)S.ndata + S.nparm + j];
+  if (S.stype == 2) {
+    const double* uh = u + nd;
+    if (S.w_smooth > 0.0) {
+      double h = col_dot(S.rh_cp, S.rh_crow, S.rh_cval, j, uh);
+      double w = col_dot(S.rv_cp, S.rv_crow, S.rv_cval, j, u + nd + np);
+      if (S.mask[j]) { h = 0.0; w = 0.0; }
+      a = a + S.w_smooth * h;
+      a = a + S.w_smooth * w;
+    }
+    if (S.gsi && j >= S.n_vel) a = a + S.w_smooth_iface * row_dot(S.rif_rp, S.rif_col, S.rif_val, j, uh);
+  } else if (S.smooth_on) {
+    a = a + sm[j];
+  }
+  if (S.damp_on && !S.mask[j]) a = a + S.wd[j] * u[nd + (long long)nblk(S) * np + j];
   if (!first) a = a - s->beta * v[j];
   v[j] = S.mask[j] ? 0.0 : a;
 }
@@ -227,11 +304,16 @@ static inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) 
 
 hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
                     LsqrScalars* host_out) {
-  const long long nd = S.ndata, np = S.nparm, na = nd + 2 * np;
+  const bool gauss = S.stype == 2;
+  const long long nd = S.ndata, np = S.nparm, na = nd + (gauss ? 3 : 2) * np;
   LsqrScalars* sc = W.sc;
   (void)hipMemsetAsync(sc, 0, sizeof(LsqrScalars), st);
-  // u = [r; -w L dm_sum; -wd dm_sum]
-  if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, S.dm_sum, W.sm, sc);
+  // u = [r; -w L dm_sum; -wd dm_sum] (Gaussian: [r; -w R_h dm_sum; -w R_v dm_sum; -wd dm_sum])
+  if (gauss)
+    hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, S.dm_sum, W.u + nd, W.u + nd + np, 1,
+                       sc);
+  else if (S.smooth_on)
+    hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, S.dm_sum, W.sm, sc);
   hipLaunchKernelGGL(rhs_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.sm, W.u);
   hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
@@ -239,7 +321,8 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   // v = A^T u, masked; alpha; v /= alpha; w = v
   hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u, W.gtu,
                      sc);
-  if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
+  if (!gauss && S.smooth_on)
+    hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
   hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 1, sc);
   hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 1, 0, max_iter, atol);
@@ -251,14 +334,18 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
     hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
     hipLaunchKernelGGL(spmv_rows_kernel, dim3(nb(nd, 128)), dim3(128), 0, st, S.rp, S.col, S.val, S.ndata, W.v, W.gv,
                        sc);
-    if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
-    hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.v, W.u, sc);
+    if (gauss)
+      hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, W.v, W.sm, W.sm2, 0, sc);
+    else if (S.smooth_on)
+      hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
+    hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.sm2, W.v, W.u, sc);
     hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
     hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
     hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
     hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
                        W.gtu, sc);
-    if (S.smooth_on) hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
+    if (!gauss && S.smooth_on)
+      hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
     hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
     hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
     hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);

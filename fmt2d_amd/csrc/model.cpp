@@ -144,6 +144,11 @@ std::string build_model(HostModel& m, int nx_s, int nz_s, const double* xpos, co
       m.vgrid[(size_t)j * nx + i] = v;
     }
   }
+  return rebuild_geometry(m);
+}
+
+std::string rebuild_geometry(HostModel& m) {
+  const int nx = m.nx, nz = m.nz;
   // ---- intersections: normals, cut cells (:1335-1500)
   m.assoc.assign((size_t)nx * nz, 0);
   for (int k = 1; k <= NIF; k++) {

@@ -51,4 +51,10 @@ std::string build_model(HostModel& m, int nx_s, int nz_s, const double* xpos, co
                         const double* sed_x, const double* sed_z, int nmoho, const double* moho_x,
                         const double* moho_z);
 
+// Intersections, regions, cut cells and region velocities from the model's
+// velocity grid and interface nodes (rebuild_geometry2d,
+// src/inversion2d.f90:3661-3816); build_model ends with it, a model step
+// (new velocities or interface depths) calls it again.
+std::string rebuild_geometry(HostModel& m);
+
 }  // namespace fmt2d

@@ -30,7 +30,7 @@ class ForwardStats(C.Structure):
     _fields_ = [("nsrc", C.c_int), ("nfield", C.c_int), ("launches", C.c_int),
                 ("tiles", C.c_longlong), ("region_nodes", C.c_longlong), ("ms_total", C.c_double),
                 ("ms_fim", C.c_double), ("exit_nodes", C.c_longlong), ("sweeps", C.c_longlong),
-                ("capped", C.c_longlong), ("max_sweeps", C.c_int)]
+                ("capped", C.c_longlong), ("max_sweeps", C.c_int), ("dispatches", C.c_int)]
 
 
 class SystemParams(C.Structure):
@@ -40,12 +40,32 @@ class SystemParams(C.Structure):
 
 class LsqrParams(C.Structure):
     _fields_ = [("wdv", C.c_double), ("wdv_iface", C.c_double), ("w_smooth", C.c_double),
-                ("w_smooth_iface", C.c_double), ("smooth_type", C.c_int), ("max_iter", C.c_int)]
+                ("w_smooth_iface", C.c_double), ("smooth_type", C.c_int), ("max_iter", C.c_int),
+                ("corr_x", C.c_double), ("corr_z", C.c_double), ("corr_x_iface", C.c_double)]
 
 
 class LsqrStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("rnorm", C.c_double), ("bnorm", C.c_double), ("test2", C.c_double),
                 ("ms", C.c_double)]
+
+
+class DampingParams(C.Structure):
+    _fields_ = [("lam_damp", C.c_double), ("lam_damp_iface", C.c_double), ("target_dv", C.c_double),
+                ("target_dd", C.c_double), ("max_rel_pert", C.c_double)]
+
+
+class DampingResult(C.Structure):
+    _fields_ = [("wd", C.c_double), ("wd_iface", C.c_double), ("avg_dmv", C.c_double), ("avg_dmd", C.c_double),
+                ("solves", C.c_int), ("iterations", C.c_int)]
+
+
+class LineSearchParams(C.Structure):
+    _fields_ = [("crit_chi", C.c_double), ("iface_step_ratio", C.c_double), ("first_step", C.c_int)]
+
+
+class LineSearchResult(C.Structure):
+    _fields_ = [("step", C.c_double), ("rms_orig", C.c_double), ("rms_new", C.c_double),
+                ("next_first_step", C.c_int), ("forwards", C.c_int)]
 
 
 class Fmt2dError(RuntimeError):
@@ -84,6 +104,12 @@ def load() -> C.CDLL:
     lib.fmt2d_build_system.argtypes = [C.c_void_p, _dp, C.POINTER(SystemParams), _ip, _ip, _dp]
     lib.fmt2d_system_get.argtypes = [C.c_void_p, _dp, _dp]
     lib.fmt2d_lsqr.argtypes = [C.c_void_p, C.POINTER(LsqrParams), _dp, _dp, C.POINTER(LsqrStats)]
+    lib.fmt2d_damped_update.argtypes = [C.c_void_p, C.POINTER(LsqrParams), C.POINTER(DampingParams), _dp, _dp,
+                                        C.POINTER(DampingResult)]
+    lib.fmt2d_apply_model_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    lib.fmt2d_model_get.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
+    lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp,
+                                      C.POINTER(LineSearchResult)]
     _LIB = lib
     return lib
 
@@ -96,7 +122,8 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
-           "fmt2d_frechet_csr", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr")
+           "fmt2d_frechet_csr", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
+           "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_apply_model_step", "fmt2d_line_search")
 
 
 def _check(rc: int):
@@ -236,14 +263,52 @@ class Context:
         return g, r[:self._ndata]
 
     def lsqr(self, dm_sum=None, wdv=0.1, wdv_iface=0.1, w_smooth=0.3, w_smooth_iface=0.3, smooth_type=1,
-             max_iter=400):
+             max_iter=400, corr_x=0.0, corr_z=0.0, corr_x_iface=0.0):
+        """One LSQR solve of the built system; returns (model update, LsqrStats)."""
         nv, npar = self.inversion_info()
         dms = np.zeros(npar) if dm_sum is None else np.ascontiguousarray(dm_sum, dtype=np.float64)
         out = np.zeros(npar)
-        p = LsqrParams(wdv, wdv_iface, w_smooth, w_smooth_iface, smooth_type, max_iter)
+        p = LsqrParams(wdv, wdv_iface, w_smooth, w_smooth_iface, smooth_type, max_iter, corr_x, corr_z, corr_x_iface)
         st = LsqrStats()
         _check(self._lib.fmt2d_lsqr(self._h, C.byref(p), _p(dms), _p(out), C.byref(st)))
         return out, st
+
+    def damped_update(self, dm_sum=None, lam_damp=0.1, lam_damp_iface=0.1, target_dv=0.03, target_dd=0.5,
+                      max_rel_pert=0.3, w_smooth=0.3, w_smooth_iface=0.3, smooth_type=1, max_iter=400, corr_x=0.0,
+                      corr_z=0.0, corr_x_iface=0.0):
+        """The iteration's model update with fixed or automatic damping;
+        returns (capped update, DampingResult)."""
+        nv, npar = self.inversion_info()
+        dms = np.zeros(npar) if dm_sum is None else np.ascontiguousarray(dm_sum, dtype=np.float64)
+        out = np.zeros(npar)
+        p = LsqrParams(0.0, 0.0, w_smooth, w_smooth_iface, smooth_type, max_iter, corr_x, corr_z, corr_x_iface)
+        d = DampingParams(lam_damp, lam_damp_iface, target_dv, target_dd, max_rel_pert)
+        r = DampingResult()
+        _check(self._lib.fmt2d_damped_update(self._h, C.byref(p), C.byref(d), _p(dms), _p(out), C.byref(r)))
+        return out, r
+
+    def model(self, what=0):
+        """Velocity grid (nz, nx) for what=0, else interface `what`'s node depths."""
+        n = C.c_int()
+        _check(self._lib.fmt2d_model_get(self._h, int(what), None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        _check(self._lib.fmt2d_model_get(self._h, int(what), _p(out), n.value, C.byref(n)))
+        if what == 0:
+            return out.reshape(self.info.nz, self.info.nx)
+        return out
+
+    def apply_model_step(self, step, step_iface, dm):
+        dm = np.ascontiguousarray(dm, dtype=np.float64)
+        _check(self._lib.fmt2d_apply_model_step(self._h, float(step), float(step_iface), _p(dm)))
+
+    def line_search(self, t_obs, dm, crit_chi=1.0, iface_step_ratio=0.5, first_step=1):
+        """Line search on the RMS; returns (step-scaled update, LineSearchResult)."""
+        t_obs = np.ascontiguousarray(t_obs, dtype=np.float64)
+        dm = np.array(dm, dtype=np.float64)
+        p = LineSearchParams(crit_chi, iface_step_ratio, first_step)
+        r = LineSearchResult()
+        _check(self._lib.fmt2d_line_search(self._h, C.byref(p), _p(t_obs), _p(dm), C.byref(r)))
+        return dm, r
 
     def _nray_last(self):
         return self._nray

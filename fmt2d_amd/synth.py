@@ -235,15 +235,20 @@ def multiphase_case(dx: float = 0.5):
     return m, phase_windows_geometry(m, obs, shots)
 
 
-def inversion_case(dx: float = 0.5, obs_x: Sequence[float] = (45.0, 95.0, 145.0), shot_dx: float = 1.0):
+def inversion_case(dx: float = 0.5, obs_x: Sequence[float] = (45.0, 95.0, 145.0), shot_dx: float = 1.0,
+                   phases: bool = False):
     """A small inversion case: the plain model family without the
     checkerboard as the starting model, the checkerboard model as the truth,
-    ``obs_x`` OBS and shots every ``shot_dx`` km at 10 m depth, Pg only.
+    ``obs_x`` OBS and shots every ``shot_dx`` km at 10 m depth, Pg only (or,
+    with ``phases``, all five phases in the reference's windows).
     Returns (true model, starting model, geometry without picks)."""
     m_true = make_model(dx=dx)
     m0 = make_model(dx=dx, checker_amp=0.0)
     shots = np.round(np.arange(shot_dx / 2.0, 200.0 - shot_dx / 4.0, shot_dx), 6)
-    g = make_geometry(obs_x, shots, m_true, ray_types=(1,))
+    if phases:
+        g = phase_windows_geometry(m_true, obs_x, shots)
+    else:
+        g = make_geometry(obs_x, shots, m_true, ray_types=(1,))
     return m_true, m0, g
 
 

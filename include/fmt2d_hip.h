@@ -76,6 +76,7 @@ typedef struct {
   long long sweeps;         /* in-tile sweeps over all tile visits */
   long long capped;         /* tile visits that reached the sweep cap */
   int max_sweeps;           /* most sweeps of one tile visit */
+  int dispatches;           /* tile-kernel dispatches (launches plus empty ones that close a batch) */
 } fmt2d_forward_stats;
 
 /* Context on HIP device `device` (owns all device memory). */
@@ -152,8 +153,10 @@ typedef struct {
 typedef struct {
   double wdv, wdv_iface;           /* damping weights (velocity, interface) */
   double w_smooth, w_smooth_iface; /* smoothing weights */
-  int smooth_type;                 /* 1: Laplacian (the supported form) */
+  int smooth_type;                 /* 1 Laplacian, 2 Gaussian (R_h, R_v, interface R_if), 3 Laplacian + R_if */
   int max_iter;
+  double corr_x, corr_z;           /* Gaussian correlation lengths (km), smooth_type 2 */
+  double corr_x_iface;             /* interface correlation length (km), smooth_type 2 and 3; <= 0: none */
 } fmt2d_lsqr_params;
 
 typedef struct {
@@ -171,9 +174,68 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
  * either pointer may be NULL. */
 int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual);
 /* One LSQR solve of [G; w L; wd I] dm = [r; -w L dm_sum; -wd dm_sum] with the
- * reference's column mask and stopping rule (test2 < 1e-4). */
+ * reference's column mask and stopping rule (test2 < 1e-4); smooth_type 2
+ * solves [G; w R_h; w R_v; wd I] (run_lsqr, inversion2d.f90:6578-7167; the
+ * Gaussian operators of :1252-1636 are built on first use for the current
+ * parameterisation and correlation lengths, then reused). */
 int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats);
+
+/* Damping of the model update (lsqr_inversion2d). */
+typedef struct {
+  double lam_damp;        /* > 0: fixed velocity damping; <= 0: automatic search */
+  double lam_damp_iface;  /* fixed interface damping (> 0, interfaces inverted); else lam_damp */
+  double target_dv;       /* search target of the mean velocity update (<= 0: 0.03) */
+  double target_dd;       /* search target of the mean interface depth update, km (<= 0: 0.5) */
+  double max_rel_pert;    /* cap of a velocity parameter's update */
+} fmt2d_damping_params;
+
+typedef struct {
+  double wd, wd_iface;    /* damping weights of the final solve */
+  double avg_dmv, avg_dmd; /* mean velocity / interface-depth update of the final solve (with dm_sum) */
+  int solves;             /* LSQR solves run */
+  int iterations;         /* iterations of the final solve */
+} fmt2d_damping_result;
+
+/* The iteration's model update: fixed damping, or the automatic search (no
+ * damping first, then secant searches of the interface and the velocity
+ * damping weights for the target mean updates, one LSQR solve per trial),
+ * then inactive and fixed parameters zeroed and the velocity update capped
+ * (lsqr_inversion2d, inversion2d.f90:533-773).  p carries the smoothing
+ * settings; its damping weights are ignored. */
+int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const fmt2d_damping_params* d,
+                        const double* dm_sum, double* dm_out, fmt2d_damping_result* res);
+/* The model as the context holds it: what 0 the velocity grid (nz x nx,
+ * x fastest; the propagation grid nodes), 1..5 an interface's node depths
+ * (2 seafloor, 3 sediment base, 4 Moho).  With out == NULL only *n. */
+int fmt2d_model_get(fmt2d_ctx* ctx, int what, double* out, int cap, int* n);
+/* apply_model_step (inversion2d.f90:7173-7240): slowness update of the
+ * velocity grid with the 2-10 km/s and +-30 % of the initial model clamps,
+ * interface depths moved by at most 0.5 km, geometry rebuilt. */
+int fmt2d_apply_model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm);
+
+typedef struct {
+  double crit_chi;          /* data selection of the RMS (inv_crit_chi) */
+  double iface_step_ratio;  /* interface step = ratio x velocity step */
+  int first_step;           /* ladder index to start at, 1..6 (1.0, 0.75, 0.5, 0.25, 0.125, 0.0625) */
+} fmt2d_line_search_params;
+
+typedef struct {
+  double step;              /* accepted step (0: every step failed, model restored) */
+  double rms_orig, rms_new;
+  int next_first_step;      /* ladder index for the next iteration */
+  int forwards;             /* trial forwards run */
+} fmt2d_line_search_result;
+
+/* Line search of lsqr_inversion2d (inversion2d.f90:775-889): for each step
+ * of the ladder from first_step, the model moved by step x dm (interfaces by
+ * step x iface_step_ratio), a forward and the rays of the last trace (no
+ * Frechet rows), the RMS of the selected residuals (compute_rms_after_forward,
+ * :3298-3339); stops at the first step below the RMS before the update,
+ * applies the best one and scales dm in place by the accepted steps.  Needs
+ * the system of this iteration (fmt2d_build_system). */
+int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const double* t_obs, double* dm,
+                      fmt2d_line_search_result* res);
 
 #ifdef __cplusplus
 }

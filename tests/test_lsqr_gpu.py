@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FIX = os.path.join(ROOT, "tests", "golden", "ref_lsqr_small.npz")
 
 
-def context_for(model, geom, d):
+def context_for(model, geom, d, ifaces=()):
     from fmt2d_amd import formats, hip, synth
     synth.write_fmt2d_case(d, model, geom)
     sm = formats.read_smesh(os.path.join(d, "model.smesh"))
@@ -28,7 +28,7 @@ def context_for(model, geom, d):
     mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
     ctx = hip.Context(0)
     ctx.set_model(sm, sx, sz, mx, mz)
-    ctx.set_inversion(True, [])
+    ctx.set_inversion(True, list(ifaces))
     return ctx, formats.read_geom(os.path.join(d, "geom.dat"))
 
 
@@ -62,3 +62,40 @@ def test_inversion_step_matches_reference(tmp_path):
     assert rel < 1e-4
     assert st.iterations == iters
     assert np.array_equal(dm, Z["dm"])
+
+
+GAUSS = os.path.join(ROOT, "tests", "golden", "ref_lsqr_gauss.npz")
+
+
+@pytest.mark.parametrize("stype", [2, 3])
+def test_gaussian_smoothing_step_matches_reference(tmp_path, stype):
+    """Gaussian (2) and hybrid (3) smoothing on the all-phase small case with
+    the sediment base and the Moho inverted (tests/golden/ref_lsqr_gauss.npz):
+    system and model update against the reference's, same bar."""
+    from fmt2d_amd import synth
+    Z = np.load(GAUSS)
+    p = f"t{stype}_"
+    ndata, nparm, nnz_r, n_vel, st_, maxit, iters = (int(x) for x in Z[p + "scal"])
+    wdv, wdv_i, ws, ws_i, eps_r, rnorm, bnorm, test2, corr_x, corr_z, corr_xi = Z[p + "par"]
+    m_true, m0, g = synth.inversion_case(phases=True)
+    print(f"[gpu] all-phase case, smooth_type {stype}: forward + rays + system + LSQR", flush=True)
+    ctx, geom = context_for(m0, g, str(tmp_path / "start"), ifaces=(3, 4))
+    ctx.forward(list(geom.obs_x))
+    ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+    # the reference's first iteration with the alternating schedule (-1) is
+    # velocity-only: interface columns scaled by 0
+    nd, nnz, eps = ctx.build_system(Z["t_obs"], crit_chi=1.0, refl_weight=1.0, alt_mode=1)
+    assert ctx.inversion_info() == (n_vel, nparm)
+    assert (nd, nnz) == (ndata, nnz_r)
+    assert eps == eps_r
+    gtg, res = ctx.system()
+    assert np.array_equal(gtg, Z["gtg"]) and np.array_equal(res, Z["residual"])
+    dm, st = ctx.lsqr(wdv=wdv, wdv_iface=wdv_i, w_smooth=ws, w_smooth_iface=ws_i, smooth_type=stype,
+                      max_iter=maxit, corr_x=corr_x, corr_z=corr_z, corr_x_iface=corr_xi)
+    ref = Z[p + "dm"]
+    rel = np.max(np.abs(dm - ref)) / np.max(np.abs(ref))
+    print(f"smooth_type {stype}: LSQR iterations {st.iterations} (reference {iters}), model update max rel diff "
+          f"{rel:.1e}, bit-identical {np.mean(dm == ref):.4f}, {st.ms:.1f} ms")
+    assert rel < 1e-4
+    assert st.iterations == iters
+    assert np.array_equal(dm, ref)
