@@ -10,10 +10,12 @@ completes 5 x nobs source-stages.  Default workload: the full multistage
 forward on the 0.1 km grid (2001x301 nodes), 60 OBS every 3.3 km.
 
 Second figure (``iteration``): seconds per inversion iteration = forward +
-rays + Frechet rows + system assembly + one LSQR solve (fixed damping) on the
-plain case (401x61, 20 OBS, 399 shots, Pg; picks from the checkerboard model
-with 50 ms seeded noise, starting model without it, Laplacian smoothing),
-with the reference's own inversion timed beside it.
+rays + Frechet rows + system assembly + one LSQR solve (fixed damping, update
+capped) on the plain case (401x61, 20 OBS, 399 shots, Pg; picks from the
+checkerboard model with 50 ms seeded noise, starting model without it,
+Laplacian smoothing), with the reference's own inversion timed beside it.
+The line search that completes the iteration (forwards without Frechet rows,
+model steps) is reported apart, for both.
 
 Prints ONE JSON line.
 """
@@ -131,14 +133,18 @@ def cpu_iteration_baseline(d, m0, gi):
     r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=600)
     fw = [ln for ln in r.stdout.splitlines() if "forward breakdown" in ln]
     ls = [ln for ln in r.stdout.splitlines() if "LSQR converged" in ln]
+    lsf = [ln for ln in r.stdout.splitlines() if "no frechet" in ln]
     if len(fw) < 2 or not ls:
         return None
     nums = [float(x) for x in fw[1].replace(",", " ").split() if x.replace(".", "", 1).isdigit()]
     fwd = sum(nums[:3])
     lsqr = float(ls[0].split("time=")[1].split()[0])
+    line = sum(float(ln.split(":")[1].split()[0]) for ln in lsf)
     return {"value": round(fwd + lsqr, 4), "unit": "s/iteration", "cores": 1, "kind": "reference",
+            "line_search_s": round(line, 4),
             "sample": f"one iteration of the reference inversion on the same case: forward {fwd:.2f} s "
-                      f"(FMM+rays+Frechet), LSQR {lsqr:.2f} s"}
+                      f"(FMM+rays+Frechet), LSQR {lsqr:.2f} s; its line search ({len(lsf)} forward(s) without "
+                      f"Frechet rows) {line:.2f} s, reported apart"}
 
 
 def inversion_iteration(steps: int, warmup: int, cpu: bool):
@@ -164,8 +170,12 @@ def inversion_iteration(steps: int, warmup: int, cpu: bool):
         ctx.set_model(sm, sx, sz, mx, mz)
         ctx.set_inversion(True, [])
         parts = {"forward": [], "rays": [], "frechet_system": [], "lsqr": []}
+        line = []
         iters = 0
+        ls_forwards = 0
         for k in range(warmup + steps):
+            ctx.set_model(sm, sx, sz, mx, mz)  # every step starts from the same model
+            ctx.set_inversion(True, [])
             t0 = time.perf_counter()
             ctx.forward(list(geom.obs_x))
             t1 = time.perf_counter()
@@ -173,12 +183,17 @@ def inversion_iteration(steps: int, warmup: int, cpu: bool):
             t2 = time.perf_counter()
             nd, nnz, eps = ctx.build_system(geom.t_obs, crit_chi=1.0, refl_weight=1.0)
             t3 = time.perf_counter()
-            dm, st = ctx.lsqr(wdv=0.1, wdv_iface=0.1, w_smooth=0.3, w_smooth_iface=0.3, max_iter=400)
+            dm, dr = ctx.damped_update(lam_damp=0.1, lam_damp_iface=0.1, target_dv=0.05, max_rel_pert=0.3,
+                                       w_smooth=0.3, w_smooth_iface=0.3, max_iter=400)
             t4 = time.perf_counter()
+            dm2, ls = ctx.line_search(geom.t_obs, dm, crit_chi=1.0, iface_step_ratio=0.5, first_step=1)
+            t5 = time.perf_counter()
             if k >= warmup:
                 for key, dt in zip(parts, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
                     parts[key].append(dt)
-            iters = st.iterations
+                line.append(t5 - t4)
+            iters = dr.iterations
+            ls_forwards = ls.forwards
         ctx.close()
         cpu_b = None
         if cpu:
@@ -189,8 +204,10 @@ def inversion_iteration(steps: int, warmup: int, cpu: bool):
     return {"metric": "forward+Frechet+LSQR s/iteration", "value": round(total, 4), "unit": "s/iteration",
             "higher_is_better": False,
             "workload": "plain inversion case: 401x61 grid (0.5 km), 20 OBS, 399 shots, Pg; one iteration "
-                        "(forward, rays, Frechet CSR, system, one LSQR solve, Laplacian smoothing, damping 0.1)",
+                        "(forward, rays, Frechet CSR, system, one LSQR solve, Laplacian smoothing, damping 0.1, "
+                        "cap of the update)",
             "breakdown_ms": {k: round(1e3 * v_, 2) for k, v_ in per.items()},
+            "line_search_ms": round(1e3 * float(np.mean(line)), 2), "line_search_forwards": ls_forwards,
             "ndata": nd, "nnz": nnz, "lsqr_iterations": iters, "cpu_baseline": cpu_b}
 
 
