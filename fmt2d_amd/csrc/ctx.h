@@ -89,7 +89,7 @@ struct fmt2d_ctx {
   DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
   double window_tiles = 3.0;          // window in tile crossing times
   int max_sweeps = 64;
-  int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
+  int colors = 1;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;

@@ -362,6 +362,24 @@ __global__ void split_kernel(const int2* __restrict__ list, const int* __restric
   }
 }
 
+// Tile nodes (bit b*TILE + a) of the box [a0,a1] x [b0,b1], clipped.
+__device__ inline unsigned long long tile_box(int a0, int a1, int b0, int b1) {
+  a0 = max(a0, 0); a1 = min(a1, TILE - 1); b0 = max(b0, 0); b1 = min(b1, TILE - 1);
+  if (a0 > a1 || b0 > b1) return 0ull;
+  const unsigned long long row = ((1ull << (a1 - a0 + 1)) - 1ull) << a0;
+  unsigned long long m = 0ull;
+  for (int b = b0; b <= b1; b++) m |= row << (TILE * b);
+  return m;
+}
+
+// Interface nodes (bit s*TILE + column) of columns [a0,a1], both sides.
+__device__ inline unsigned tile_iface_cols(int a0, int a1) {
+  a0 = max(a0, 0); a1 = min(a1, TILE - 1);
+  if (a0 > a1) return 0u;
+  const unsigned m = ((1u << (a1 - a0 + 1)) - 1u) << a0;
+  return m | (m << TILE);
+}
+
 // One visit of tile ft: stage the window, sweep to the tile's local fixed
 // point, write back, activate the neighbours whose halo changed.
 __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
@@ -422,6 +440,22 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
   const int wi = is * WIN + (ic - wx0);
   bool g_dirty = false, i_dirty = false;
   bool more = true;
+  // Nodes whose inputs did not change since their last evaluation give the
+  // same result again and are skipped.  A grid node reads tile nodes at most
+  // two rows/columns away and the interface nodes of its own and adjacent
+  // columns; an interface node reads the grid nodes within two columns and
+  // two rows of its cell (rows cz-2 .. cz+3 with its coincident partner) and
+  // the interface nodes of its column.  Changed-node masks of every phase
+  // (one bit per tile node, lane = node) accumulate until a node is
+  // evaluated again; the first sweep evaluates everything.
+  const int la = lane & (TILE - 1), lb = lane / TILE;
+  const unsigned long long g_dep = tile_box(la - 2, la + 2, lb - 2, lb + 2);
+  const unsigned g_idep = tile_iface_cols(la - 2, la + 2);
+  unsigned long long i_dep = 0ull;
+  if (i_act) i_dep = tile_box(la - 2, la + 2, i_cc - tz0 - 2, i_cc - tz0 + 3);
+  const unsigned i_idep = tile_iface_cols(la - 2, la + 2);
+  unsigned long long g_pend = ~0ull, i_pend = ~0ull;
+  unsigned g_ipend = 0xFFFFu, i_ipend = 0xFFFFu;
   // Sweeps.  colors == 4: Gauss-Seidel in five phases, the four parity
   // colours of the grid nodes (a node's x/z neighbours and cell diagonals
   // have other colours), then the interface nodes.  colors == 2: the two
@@ -435,10 +469,12 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     bool ch = false;
     for (int phase = 0; phase < nphase; phase++) {
       const bool grid_phase = phase < nphase - 1;
+      bool upd = false;
       if (grid_phase) {
-        bool upd = false;
         Best<double> b;
-        if (g_act && (colors == 1 || (colors == 2 ? ((gi + gj) & 1) : g_color) == phase)) {
+        if (g_act && (colors == 1 || (colors == 2 ? ((gi + gj) & 1) : g_color) == phase) &&
+            ((g_pend & g_dep) | (unsigned long long)(g_ipend & g_idep))) {
+          g_pend = 0ull; g_ipend = 0u;
           b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
           update_grid_node<double>(v, gi, gj, b);
           upd = __double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
@@ -450,15 +486,22 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
           L.T[wg] = b.T; L.GX[wg] = b.gx; L.GZ[wg] = b.gz; L.tie[wg] = (uint8_t)b.tie;
           ch = true; g_dirty = true;
         }
-      } else if (i_act) {
-        Best<double> b;
-        b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
-        update_iface_node<double>(v, is, ic, b);
-        if (__double_as_longlong(b.T) != __double_as_longlong(L.iT[wi]) ||
-            __double_as_longlong(b.gx) != __double_as_longlong(L.iGX[wi]) ||
-            __double_as_longlong(b.gz) != __double_as_longlong(L.iGZ[wi])) {
-          L.iT[wi] = b.T; L.iGX[wi] = b.gx; L.iGZ[wi] = b.gz; ch = true; i_dirty = true;
+        const unsigned long long chg = __ballot(upd);
+        g_pend |= chg; i_pend |= chg;
+      } else {
+        if (i_act && ((i_pend & i_dep) | (unsigned long long)(i_ipend & i_idep))) {
+          i_pend = 0ull; i_ipend = 0u;
+          Best<double> b;
+          b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
+          update_iface_node<double>(v, is, ic, b);
+          if (__double_as_longlong(b.T) != __double_as_longlong(L.iT[wi]) ||
+              __double_as_longlong(b.gx) != __double_as_longlong(L.iGX[wi]) ||
+              __double_as_longlong(b.gz) != __double_as_longlong(L.iGZ[wi])) {
+            L.iT[wi] = b.T; L.iGX[wi] = b.gx; L.iGZ[wi] = b.gz; ch = true; i_dirty = true; upd = true;
+          }
         }
+        const unsigned ichg = (unsigned)(__ballot(upd) & 0xFFFFull);
+        g_ipend |= ichg; i_ipend |= ichg;
       }
       __syncthreads();
     }
