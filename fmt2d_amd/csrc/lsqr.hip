@@ -20,26 +20,42 @@ namespace fmt2d {
 
 __device__ inline bool stopped(const LsqrScalars* s) { return s->done != 0; }
 
-// y[i] = sum_k G(i,k) x(col k), ascending k (sparse_Ax, :2997-3018)
-__global__ void spmv_rows_kernel(const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ val,
-                                 int nrow, const double* __restrict__ x, double* y, const LsqrScalars* s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per row: y[i] = sum_k M(i,k) x(idx k), ascending k.  The lanes
+// form the products of 64 entries at a time (coalesced loads, gathers of x)
+// into LDS; lane 0 adds them front to back, so the sum rounds as the
+// reference's sequential loop does.  The next chunk's loads are issued
+// before lane 0's additions and complete behind them.
+__global__ void __launch_bounds__(64) ordered_spmv_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                         const double* __restrict__ val, int nrow,
+                                                         const double* __restrict__ x, double* y,
+                                                         const LsqrScalars* s) {
+  __shared__ double buf[64];
+  const int i = blockIdx.x;
   if (i >= nrow || stopped(s)) return;
+  const int lane = threadIdx.x;
+  const int a = ptr[i], b = ptr[i + 1];
   double acc = 0.0;
-  for (int k = rp[i]; k < rp[i + 1]; k++) acc = acc + val[k] * x[col[k]];
-  y[i] = acc;
+  double p = 0.0;
+  if (a + lane < b) p = val[a + lane] * x[idx[a + lane]];
+  for (int c = a; c < b; c += 64) {
+    buf[lane] = p;
+    __syncthreads();
+    const int k = c + 64 + lane;
+    double q = 0.0;
+    if (k < b) q = val[k] * x[idx[k]];
+    if (lane == 0) {
+      const int m = min(64, b - c);
+      for (int t = 0; t < m; t++) acc = acc + buf[t];
+    }
+    __syncthreads();
+    p = q;
+  }
+  if (lane == 0) y[i] = acc;
 }
 
-// x[j] = sum_i G(i,j) u(i), ascending i (sparse_ATx, :3021-3042), from the
-// column-major copy of G
-__global__ void spmv_cols_kernel(const int* __restrict__ cp, const int* __restrict__ row, const double* __restrict__ val,
-                                 int ncol, const double* __restrict__ u, double* x, const LsqrScalars* s) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncol || stopped(s)) return;
-  double acc = 0.0;
-  for (int k = cp[j]; k < cp[j + 1]; k++) acc = acc + val[k] * u[row[k]];
-  x[j] = acc;
-}
+// G x by rows (sparse_Ax, :2997-3018) and G^T u by columns of the stable
+// column copy (sparse_ATx, :3021-3042: each column's rows ascending, the
+// order in which the reference's scatter accumulates) use ordered_spmv_kernel.
 
 // row j of a Gaussian smoothing matrix times x, entries in stored order
 // (apply_gauss_smooth_h / _v / apply_iface_gauss_smooth_h, :1405-1521)
@@ -319,8 +335,8 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 0, sc);
   // v = A^T u, masked; alpha; v /= alpha; w = v
-  hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u, W.gtu,
-                     sc);
+  hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
+                     W.gtu, sc);
   if (!gauss && S.smooth_on)
     hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
   hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 1, sc);
@@ -332,8 +348,9 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   if (e != hipSuccess) return e;
   for (int it = 1; it <= max_iter; it++) {
     hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
-    hipLaunchKernelGGL(spmv_rows_kernel, dim3(nb(nd, 128)), dim3(128), 0, st, S.rp, S.col, S.val, S.ndata, W.v, W.gv,
-                       sc);
+    if (nd > 0)
+      hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nd), dim3(64), 0, st, S.rp, S.col, S.val, S.ndata, W.v,
+                         W.gv, sc);
     if (gauss)
       hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, W.v, W.sm, W.sm2, 0, sc);
     else if (S.smooth_on)
@@ -342,7 +359,7 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
     hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
     hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
     hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
-    hipLaunchKernelGGL(spmv_cols_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
+    hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
                        W.gtu, sc);
     if (!gauss && S.smooth_on)
       hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
