@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--workload", default="multistage", choices=["multistage", "plain"])
     ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
+    ap.add_argument("--iteration-only", action="store_true",
+                    help="only the inversion-iteration figure (for profiling its kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference point that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -236,6 +238,11 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.iteration_only:
+        print(json.dumps({"iteration": inversion_iteration(max(1, min(args.steps, 3)), args.warmup,
+                                                           args.cpu_sample > 0)}))
+        return
 
     from fmt2d_amd import hip, formats
     m, g, desc = workload(args.workload)
