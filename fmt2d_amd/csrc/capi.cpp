@@ -104,6 +104,31 @@ int model_upload(fmt2d_ctx* ctx) {
   }
   rd[0] = rd[1];
   HIPCHK(upload(ctx->regs, rd, st));
+  // tiles (8x8) with cut cells, coincident nodes or interface nodes of the
+  // region: their visits are the slow ones and start first in a launch
+  {
+    const int ntx = (nx + 7) / 8, ntz = (nz + 7) / 8, ntiles = ntx * ntz;
+    std::vector<uint8_t> heavy((size_t)(NREG + 1) * ntiles, 0);
+    for (int r = 1; r <= NREG; r++) {
+      const HostRegion& R = m.reg[r];
+      uint8_t* h = heavy.data() + (size_t)r * ntiles;
+      for (int j = 0; j < nz; j++)
+        for (int i = 0; i < nx; i++) {
+          const uint16_t fl = R.flags[(size_t)j * nx + i];
+          if ((fl & RF_IN) && (fl & ((RF_IRR0 * 15) | (RF_COI0 * 3)))) h[(j / 8) * ntx + i / 8] = 1;
+        }
+      const int ids[2] = {R.itop, R.ibot};
+      for (int s2 = 0; s2 < 2; s2++) {
+        const HostInterface& is = m.isec[ids[s2]];
+        for (int c = 0; c < nx; c++) {
+          const int cz = is.cc[c];
+          if (cz < 0) continue;
+          for (int j = cz; j <= std::min(cz + 1, nz - 1); j++) h[(j / 8) * ntx + c / 8] = 1;
+        }
+      }
+    }
+    HIPCHK(upload(ctx->theavy, heavy, st));
+  }
   std::vector<IsecDev> idv(NIF + 1);
   for (int k = 1; k <= NIF; k++) {
     const HostInterface& is = m.isec[k];
@@ -307,8 +332,9 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
     const FieldDesc* dl = ctx->descs.p + f0;
     if (L > 0) HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz, ctx->isecs.p, st));
-    // counters: [0] due tiles of a launch, [1] / [2] sizes of active lists 0 / 1
-    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 3, st));
+    // counters: [0] / [3] due tiles of a launch cut by an interface / not,
+    // [1] / [2] sizes of active lists 0 / 1
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 4, st));
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
                              ctx->tcount.p + 1, st));
     if (ctx->ttrace.n < (size_t)max_launch + 1) HIPCHK(ctx->ttrace.alloc((size_t)max_launch + 1));
@@ -327,7 +353,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
                                ctx->tfront.p, window, launches & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p,
                                ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, launches > 0 ? ctx->ttrace.p + launches - 1
                                                                                           : nullptr,
-                               sched_grid, st));
+                               dl, ctx->theavy.p, (int)ntt, sched_grid, st));
         const size_t ei = 2 * nev++;
         while (ctx->ev.size() < ei + 2) {
           hipEvent_t e;
@@ -337,11 +363,11 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         HIPCHK(hipEventRecord(ctx->ev[ei], st));
         HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, ctx->tin.p,
                           ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->colors,
-                          ctx->acc.p, ctx->tmax.p + launches, st));
+                          ctx->acc.p, ctx->tmax.p + launches, (int)ntt, st));
         HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
         cur = nxt;
       }
-      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       if (ctx->h_count[1 + cur] == 0) { done = true; break; }
       if (launches >= max_launch) {
@@ -364,7 +390,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       if (launches > 1)
         HIPCHK(hipMemcpy(tr.data(), ctx->ttrace.p, sizeof(int) * (launches - 1), hipMemcpyDeviceToHost));
       if (launches > 0) {
-        tr[launches - 1] = ctx->h_count[0];
+        tr[launches - 1] = ctx->h_count[0] + ctx->h_count[3];
         HIPCHK(hipMemcpy(mx.data(), ctx->tmax.p, sizeof(int) * launches, hipMemcpyDeviceToHost));
       }
       // launches queued after the set emptied (rest of the last batch) had nothing due

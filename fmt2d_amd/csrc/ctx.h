@@ -87,6 +87,7 @@ struct fmt2d_ctx {
   DBuf<int> tcount;                   // [due, size of list 0, size of list 1]
   DBuf<int> ttrace;                   // due tiles per launch of a level
   DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
+  DBuf<uint8_t> theavy;               // per (region, tile): cut by an interface (slow visits)
   double window_tiles = 3.0;          // window in tile crossing times
   int max_sweeps = 64;
   int colors = 1;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi

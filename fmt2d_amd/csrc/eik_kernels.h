@@ -66,11 +66,12 @@ hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int 
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
                            unsigned long long* tkey, unsigned long long* front, double window, int color,
                            unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* trace_prev, int grid, hipStream_t st);
+                           int* trace_prev, const FieldDesc* descs, const uint8_t* heavy, int due_cap, int grid,
+                           hipStream_t st);
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
                       int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
-                      hipStream_t st);
+                      int due_cap, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

@@ -306,14 +306,20 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
       if ((j % TILE) == 0 || j == j0) add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key, in_set, tkey, list_next, count_next);
 }
 
+// Due tiles of a launch: count_due[0] tiles cut by an interface (placed
+// from the front of the due list), count_due[3] the others (placed from its
+// end), so the slow tiles start first.
+__device__ inline int due_count(const int* count_due) { return count_due[0] + count_due[3]; }
+
 // Clears the counters of a launch: due tiles and the next active list.
 // The previous launch's due count goes to its trace slot (when given).
 __global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned long long* front, int nfield,
                                     int* trace_prev) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f == 0) {
-    if (trace_prev) *trace_prev = *count_due;
-    *count_due = 0;
+    if (trace_prev) *trace_prev = due_count(count_due);
+    count_due[0] = 0;
+    count_due[3] = 0;
     *count_next = 0;
   }
   if (f < nfield) front[f] = KEY_INF;
@@ -336,7 +342,8 @@ __global__ void front_kernel(const int2* __restrict__ list, const int* __restric
 __global__ void split_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntx, int ntiles,
                              unsigned long long* tkey, const unsigned long long* __restrict__ front, double window,
                              int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
-                             int* count_carry) {
+                             int* count_carry, const FieldDesc* __restrict__ descs,
+                             const uint8_t* __restrict__ heavy, int due_cap) {
   const int n = *count;
   const int stride = gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of rounds (wave_append)
@@ -355,9 +362,12 @@ __global__ void split_kernel(const int2* __restrict__ list, const int* __restric
       is_carry = !is_due;
       if (is_due) { tkey[q] = KEY_INF; in_set[q] = 0u; }
     }
-    const int pd = wave_append(count_due, is_due);
+    const bool hv = is_due && heavy[(long long)descs[e.x].region * ntiles + e.y] != 0;
+    const int ph = wave_append(count_due, hv);
+    const int pl = wave_append(count_due + 3, is_due && !hv);
     const int pc = wave_append(count_carry, is_carry);
-    if (is_due) due[pd] = e;
+    if (hv) due[ph] = e;
+    else if (is_due) due[due_cap - 1 - pl] = e;
     if (is_carry) carry[pc] = e;
   }
 }
@@ -567,12 +577,13 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
                                                        const int* __restrict__ count_due, int ntx, int ntz,
                                                        unsigned int* in_set, unsigned long long* tkey,
                                                        int2* list_next, int* count_next, int max_sweeps,
-                                                       int colors, unsigned long long* acc, int* launch_max) {
+                                                       int colors, unsigned long long* acc, int* launch_max,
+                                                       int due_cap) {
   __shared__ TileLds L;
-  const int n = *count_due;
+  const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
-    fim_tile(descs, regs, list[k], ntx, ntz, in_set, tkey, list_next, count_next, max_sweeps, colors, acc, launch_max,
-             L);
+    const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
+    fim_tile(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, max_sweeps, colors, acc, launch_max, L);
     __syncthreads();
   }
 }
@@ -1012,21 +1023,22 @@ hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int 
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
                            unsigned long long* tkey, unsigned long long* front, double window, int color,
                            unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* trace_prev, int grid, hipStream_t st) {
+                           int* trace_prev, const FieldDesc* descs, const uint8_t* heavy, int due_cap, int grid,
+                           hipStream_t st) {
   hipLaunchKernelGGL(launch_reset_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, count_due, count_carry, front,
                      nfield, trace_prev);
   hipLaunchKernelGGL(front_kernel, dim3(grid), dim3(256), 0, st, list, count, ntiles, tkey, front);
   hipLaunchKernelGGL(split_kernel, dim3(grid), dim3(256), 0, st, list, count, ntx, ntiles, tkey, front, window, color,
-                     in_set, due, count_due, carry, count_carry);
+                     in_set, due, count_due, carry, count_carry, descs, heavy, due_cap);
   return hipGetLastError();
 }
 
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
                       int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
-                      hipStream_t st) {
+                      int due_cap, hipStream_t st) {
   hipLaunchKernelGGL(fim_tiles_kernel, dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
-                     tkey, list_next, count_next, max_sweeps, colors, acc, launch_max);
+                     tkey, list_next, count_next, max_sweeps, colors, acc, launch_max, due_cap);
   return hipGetLastError();
 }
 
