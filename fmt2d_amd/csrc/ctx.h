@@ -89,8 +89,8 @@ struct fmt2d_ctx {
   DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
   DBuf<uint8_t> theavy;               // per (region, tile): cut by an interface (slow visits)
   double window_tiles = 3.0;          // window in tile crossing times
-  int max_sweeps = 64;
-  int colors = 1;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
+  int max_sweeps = 8;                 // a visit over the cap stays active and continues later
+  int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;

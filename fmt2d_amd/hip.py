@@ -170,7 +170,7 @@ class Context:
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
 
-    def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 64, colors: int = 1):
+    def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 8, colors: int = 2):
         _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
 
     def stats(self) -> ForwardStats:

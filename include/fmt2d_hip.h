@@ -99,7 +99,7 @@ int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
 /* Solver tuning: arrival-time window of the tile schedule in tile crossing
  * times (at 4 km/s), the sweep cap of one tile visit, and the in-tile sweep
  * order (4: four-colour Gauss-Seidel, 2: two-colour Gauss-Seidel, 1: Jacobi).
- * Defaults 3.0, 64, 1. */
+ * Defaults 3.0, 8, 2. */
 int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors);
 /* Due-tile count and the most sweeps of one tile visit, per solver launch of
  * the last forward (diagnostics); either array may be NULL. */
