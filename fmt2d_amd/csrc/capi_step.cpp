@@ -246,6 +246,19 @@ int fmt2d_model_get(fmt2d_ctx* ctx, int what, double* out, int cap, int* n) {
   return FMT2D_OK;
 }
 
+int fmt2d_model_put(fmt2d_ctx* ctx, int what, const double* in, int n) {
+  if (!ctx || !in) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  std::vector<double>* v = nullptr;
+  if (what == 0) v = &ctx->m.vgrid;
+  else if (what >= 2 && what <= 4) v = &ctx->m.if_z[what];
+  else return fail(FMT2D_EINVAL, "what must be 0 (velocity grid) or an interface 2..4");
+  if (n != (int)v->size()) return fail(FMT2D_EINVAL, "size does not match the model");
+  HIPCHK(hipSetDevice(ctx->device));
+  memcpy(v->data(), in, sizeof(double) * v->size());
+  return rebuild(ctx);
+}
+
 int fmt2d_apply_model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm) {
   if (!ctx || !dm) return fail(FMT2D_EINVAL, "NULL argument");
   if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");

@@ -108,6 +108,7 @@ def load() -> C.CDLL:
                                         C.POINTER(DampingResult)]
     lib.fmt2d_apply_model_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     lib.fmt2d_model_get.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
+    lib.fmt2d_model_put.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp,
                                       C.POINTER(LineSearchResult)]
     _LIB = lib
@@ -123,7 +124,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
            "fmt2d_frechet_csr", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
-           "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_apply_model_step", "fmt2d_line_search")
+           "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
 
 
 def _check(rc: int):
@@ -296,6 +297,10 @@ class Context:
         if what == 0:
             return out.reshape(self.info.nz, self.info.nx)
         return out
+
+    def model_put(self, what, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        _check(self._lib.fmt2d_model_put(self._h, int(what), _p(v), len(v)))
 
     def apply_model_step(self, step, step_iface, dm):
         dm = np.ascontiguousarray(dm, dtype=np.float64)

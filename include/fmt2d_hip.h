@@ -209,6 +209,10 @@ int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const fmt2d_
  * x fastest; the propagation grid nodes), 1..5 an interface's node depths
  * (2 seafloor, 3 sediment base, 4 Moho).  With out == NULL only *n. */
 int fmt2d_model_get(fmt2d_ctx* ctx, int what, double* out, int cap, int* n);
+/* Replaces the velocity grid (what 0) or an interface's node depths (2..4)
+ * and rebuilds the geometry, as restore_model_state does
+ * (inversion2d.f90:7242-7276); n must match the model's size. */
+int fmt2d_model_put(fmt2d_ctx* ctx, int what, const double* in, int n);
 /* apply_model_step (inversion2d.f90:7173-7240): slowness update of the
  * velocity grid with the 2-10 km/s and +-30 % of the initial model clamps,
  * interface depths moved by at most 0.5 km, geometry rebuilt. */
