@@ -1,7 +1,9 @@
-"""Builds the in-tree HIP library fmt2d_amd/libfmt2d_hip.so for gfx950.
+"""Builds the in-tree HIP library fmt2d_amd/libfmt2d_hip.so for gfx950 and
+the Fortran host program fmt2d_amd/fortran/invert2d_hip on top of it.
 
 Run ``python -m fmt2d_amd.build``.  The library carries every kernel of the
-path and the C ABI declared in include/fmt2d_hip.h.
+path and the C ABI declared in include/fmt2d_hip.h; the Fortran program
+binds that ABI through iso_c_binding (fmt2d_amd/fortran/fmt2d_hip_mod.f90).
 """
 from __future__ import annotations
 
@@ -14,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfmt2d_hip.so")
+FORTRAN = os.path.join(HERE, "fortran")
+INVERT2D = os.path.join(FORTRAN, "invert2d_hip")
 
 ARCH = os.environ.get("FMT2D_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["eik_kernels.hip", "rays.hip", "csr.hip", "lsqr.hip"]
@@ -56,7 +60,39 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def fortran_compiler():
+    for c in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"), shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose: bool = False):
+    """The Fortran program, linked to the library next to it (rpath
+    $ORIGIN/..).  Skipped, with a note, when no Fortran compiler is found."""
+    fc = fortran_compiler()
+    if fc is None:
+        print("fmt2d_amd.build: no Fortran compiler (amdflang); invert2d_hip not built", file=sys.stderr)
+        return None
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    mod_o = os.path.join(objdir, "fmt2d_hip_mod.o")
+    tmp = INVERT2D + ".tmp"
+    cmds = [[fc, "-O3", "-module-dir", objdir, "-c", os.path.join(FORTRAN, "fmt2d_hip_mod.f90"), "-o", mod_o],
+            [fc, "-O3", "-module-dir", objdir, os.path.join(FORTRAN, "invert2d_hip.f90"), mod_o, "-L", HERE,
+             "-lfmt2d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd))
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed:\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, INVERT2D)
+    return INVERT2D
 
 
 if __name__ == "__main__":

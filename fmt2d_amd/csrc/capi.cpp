@@ -240,7 +240,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   ctx->nfield = nfield;
   ctx->src_x.assign(src_x, src_x + nsrc);
   ctx->src_z.resize(nsrc);
-  for (int s = 0; s < nsrc; s++) ctx->src_z[s] = m.iface_z_at(2, src_x[s]);
+  for (int s = 0; s < nsrc; s++) ctx->src_z[s] = m.setup_seafloor_at(src_x[s]);
   ctx->nray = 0;
   std::vector<FieldDesc> fd(nfield);
   std::vector<StageLink> lk(nfield);

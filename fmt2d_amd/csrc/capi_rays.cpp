@@ -42,7 +42,8 @@ int build_ray_geom(fmt2d_ctx* ctx) {
   G.reg[0] = G.reg[1];
   HIPCHK(upload(ctx->rsfx, m.if_x[2], st));
   HIPCHK(upload(ctx->rsfz, m.if_z[2], st));
-  G.sfx = ctx->rsfx.p; G.sfz = ctx->rsfz.p; G.nsf = (int)m.if_x[2].size();
+  HIPCHK(upload(ctx->rsfz0, m.sf_setup, st));
+  G.sfx = ctx->rsfx.p; G.sfz = ctx->rsfz.p; G.sfz_setup = ctx->rsfz0.p; G.nsf = (int)m.if_x[2].size();
   G.sfdx = m.if_x[2][1] - m.if_x[2][0];
   G.vwater = m.vwater;
   HIPCHK(upload(ctx->rpvel, m.vgrid, st));

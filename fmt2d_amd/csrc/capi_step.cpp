@@ -239,7 +239,7 @@ int fmt2d_model_get(fmt2d_ctx* ctx, int what, double* out, int cap, int* n) {
   else if (what >= 1 && what <= NIF) v = &ctx->m.if_z[what];
   else return fail(FMT2D_EINVAL, "what must be 0 (velocity grid) or an interface 1..5");
   *n = (int)v->size();
-  if (out) {
+  if (out && cap > 0) {
     if (cap < *n) return fail(FMT2D_EINVAL, "output buffer too small");
     memcpy(out, v->data(), sizeof(double) * v->size());
   }
@@ -267,7 +267,7 @@ int fmt2d_apply_model_step(fmt2d_ctx* ctx, double step, double step_iface, const
 }
 
 int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const double* t_obs, double* dm,
-                      fmt2d_line_search_result* res) {
+                      int* valid_io, fmt2d_line_search_result* res) {
   if (!ctx || !p || !t_obs || !dm || !res) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
   if (p->first_step < 1 || p->first_step > 6) return fail(FMT2D_EINVAL, "first_step must be 1..6");
@@ -278,6 +278,10 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   const std::vector<double> src_x = ctx->src_x;
   const std::vector<RayJob> jobs = ctx->hjobs;
   std::vector<int> js(nray), jp(nray), valid(nray);
+  // a ray lost by one trial forward stays lost for the rest of the iteration
+  // (ray%valid is only set again when the next iteration starts, :283-288)
+  std::vector<int> alive = valid_io ? std::vector<int>(valid_io, valid_io + nray) : ctx->h_valid;
+  if ((int)alive.size() != nray) return fail(FMT2D_ESTATE, "ray validity does not match the last trace");
   std::vector<double> jx(nray), jz(nray), tc(nray);
   for (int r = 0; r < nray; r++) { js[r] = jobs[r].src; jp[r] = jobs[r].path; jx[r] = jobs[r].rx; jz[r] = jobs[r].rz; }
   const ModelState saved = save_state(ctx);
@@ -295,8 +299,9 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
     // compute_rms_after_forward (:3298-3339)
     int n = 0;
     double ss = 0.0;
+    for (int r = 0; r < nray; r++) alive[r] = alive[r] && valid[r];
     for (int r = 0; r < nray; r++)
-      if (valid[r] && t_obs[r] >= 0.0) {
+      if (alive[r] && t_obs[r] >= 0.0) {
         const double rc2 = t_obs[r] - tc[r];
         if (fabs(rc2) <= p->crit_chi) { n++; ss = ss + rc2 * rc2; }
       }
@@ -312,6 +317,7 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   const double step_if = ctx->inv_ifaces.empty() ? 0.0 : step_len * p->iface_step_ratio;
   for (int j = 0; j < nparm; j++)
     dm[j] = (j < n_vel || ctx->inv_ifaces.empty()) ? step_len * dm[j] : step_if * dm[j];
+  if (valid_io) std::copy(alive.begin(), alive.end(), valid_io);
   res->step = step_len;
   res->rms_orig = rms_orig;
   res->rms_new = step_len > 0.0 ? rms_best : rms_orig;

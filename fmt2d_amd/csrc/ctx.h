@@ -107,7 +107,7 @@ struct fmt2d_ctx {
   DBuf<RayGeomDev> rgeom;
   DBuf<int> rorder[NREG + 1];
   DBuf<double> risx[NIF + 1], risz[NIF + 1], rifx_in[NIF + 1];
-  DBuf<double> rsfx, rsfz, rpvel, rvx, rvz;
+  DBuf<double> rsfx, rsfz, rsfz0, rpvel, rvx, rvz;
   DBuf<int> rinv;
   DBuf<RayField> rfields;
   int nray = 0;

@@ -13,10 +13,7 @@
 
 namespace fmt2d {
 
-double HostModel::iface_z_at(int k, double px) const {
-  // linear interpolation on the interface nodes (src/fmt2d_main.f90:1389-1415)
-  const std::vector<double>& x = if_x[k];
-  const std::vector<double>& z = if_z[k];
+static double iface_interp(const std::vector<double>& x, const std::vector<double>& z, double px) {
   const int n = (int)x.size();
   if (n < 2) return z[0];
   const double d = x[1] - x[0];
@@ -27,6 +24,11 @@ double HostModel::iface_z_at(int k, double px) const {
   w = std::min(1.0, std::max(0.0, w));
   return (1.0 - w) * z[i - 1] + w * z[i];
 }
+
+// linear interpolation on the interface nodes (src/fmt2d_main.f90:1389-1415)
+double HostModel::iface_z_at(int k, double px) const { return iface_interp(if_x[k], if_z[k], px); }
+
+double HostModel::setup_seafloor_at(double px) const { return iface_interp(if_x[2], sf_setup, px); }
 
 // interpolate_iface_z2d (src/propagate2d.f90:1615-1645): node spacing is
 // taken as the grid dx because interface nodes sit on grid columns.
@@ -92,6 +94,7 @@ std::string build_model(HostModel& m, int nx_s, int nz_s, const double* xpos, co
   std::vector<double> xs(xpos, xpos + nx_s);
   m.if_x[1] = xs; m.if_z[1].assign(nx_s, 0.0);
   m.if_x[2] = xs; m.if_z[2].assign(topo, topo + nx_s);
+  m.sf_setup = m.if_z[2];
   m.if_x[3].assign(sed_x, sed_x + nsed); m.if_z[3].assign(sed_z, sed_z + nsed);
   m.if_x[4].assign(moho_x, moho_x + nmoho); m.if_z[4].assign(moho_z, moho_z + nmoho);
   m.if_x[5] = xs; m.if_z[5].assign(nx_s, zrel[nz_s - 1] + maxtopo);

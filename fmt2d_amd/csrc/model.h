@@ -40,7 +40,12 @@ struct HostModel {
   HostRegion reg[NREG + 1];
   // interfaces as read (for placing sources/receivers on the seafloor)
   std::vector<double> if_x[NIF + 1], if_z[NIF + 1];
+  // seafloor depths as read at set-up: sources and the in-water test of
+  // receivers stay on it when the seafloor is later inverted
+  // (invert2d_main.f90:517-531, 544-546)
+  std::vector<double> sf_setup;
   double iface_z_at(int k, double px) const;
+  double setup_seafloor_at(double px) const;
 };
 
 // Builds the model of src/fmt2d_main.f90:91-418 from the .smesh content and

@@ -34,6 +34,7 @@ struct RayGeomDev {
   RayRegionDev reg[7];
   const double* sfx;       // seafloor interface as read (intrface(2)%x, %z_nodes)
   const double* sfz;
+  const double* sfz_setup; // seafloor depths at set-up (in-water test of receivers)
   int nsf;
   double sfdx;
   double vwater;

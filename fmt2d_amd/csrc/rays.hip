@@ -183,15 +183,17 @@ __device__ void iface_normal(const RayGeomDev& G, int k, double px, double& nx_,
   nz_ = 1.0 / nl;
 }
 
-// interpolate_iface_z_inv (inversion2d.f90:6315-6341) on the seafloor as read
-__device__ double seafloor_z(const RayGeomDev& G, double px) {
-  if (G.nsf < 2) return G.sfz[0];
+// interpolate_iface_z_inv (inversion2d.f90:6315-6341) on seafloor depths z
+// (the current ones, or those at set-up)
+__device__ double seafloor_z(const RayGeomDev& G, const double* z, double px) {
+  if (G.nsf < 2) return z[0];
   const int i = ifloor((px - G.sfx[0]) / G.sfdx) + 1;
-  if (i < 1) return G.sfz[0];
-  if (i >= G.nsf) return G.sfz[G.nsf - 1];
+  if (i < 1) return z[0];
+  if (i >= G.nsf) return z[G.nsf - 1];
   const double w = clamp01((px - G.sfx[i - 1]) / G.sfdx);
-  return (1.0 - w) * G.sfz[i - 1] + w * G.sfz[i];
+  return (1.0 - w) * z[i - 1] + w * z[i];
 }
+__device__ double seafloor_z(const RayGeomDev& G, double px) { return seafloor_z(G, G.sfz, px); }
 
 // get_velocity_at2d (rays2d.f90:1312-1348) on the propagation grid
 __device__ double velocity_at(const RayGeomDev& G, double px, double pz) {
@@ -564,9 +566,10 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
   o.headwave = 0; o.diffracted = 0;
   const int nsec = o.nsec, s = J.src;
   const double sx = src_x[s], sz = src_z[s];
-  // receiver placement (invert2d_main.f90:517-531, inversion2d.f90:4502-4519)
+  // receiver placement (invert2d_main.f90:517-531, inversion2d.f90:4502-4519):
+  // in water when above the seafloor of the set-up
   double rx = J.rx, rz = J.rz, wt = 0.0;
-  const double zsf = seafloor_z(G, J.rx);
+  const double zsf = seafloor_z(G, G.sfz_setup, J.rx);
   if (J.rz < zsf) {
     if (J.path == 4 || J.path == 5) {
       rx = J.rx; rz = J.rz; wt = 0.0;

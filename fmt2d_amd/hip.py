@@ -109,7 +109,7 @@ def load() -> C.CDLL:
     lib.fmt2d_apply_model_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     lib.fmt2d_model_get.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
     lib.fmt2d_model_put.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
-    lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp,
+    lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp, _ip,
                                       C.POINTER(LineSearchResult)]
     _LIB = lib
     return lib
@@ -306,13 +306,22 @@ class Context:
         dm = np.ascontiguousarray(dm, dtype=np.float64)
         _check(self._lib.fmt2d_apply_model_step(self._h, float(step), float(step_iface), _p(dm)))
 
-    def line_search(self, t_obs, dm, crit_chi=1.0, iface_step_ratio=0.5, first_step=1):
-        """Line search on the RMS; returns (step-scaled update, LineSearchResult)."""
+    def line_search(self, t_obs, dm, crit_chi=1.0, iface_step_ratio=0.5, first_step=1, valid=None):
+        """Line search on the RMS; returns (step-scaled update, LineSearchResult).
+
+        valid: the iteration's ray validity (default: that of the last trace);
+        when given, it is updated in place with the rays the trial forwards
+        lost."""
         t_obs = np.ascontiguousarray(t_obs, dtype=np.float64)
         dm = np.array(dm, dtype=np.float64)
         p = LineSearchParams(crit_chi, iface_step_ratio, first_step)
         r = LineSearchResult()
-        _check(self._lib.fmt2d_line_search(self._h, C.byref(p), _p(t_obs), _p(dm), C.byref(r)))
+        vp = None
+        if valid is not None:
+            if valid.dtype != np.int32 or not valid.flags.c_contiguous:
+                raise TypeError("valid must be a contiguous int32 array")
+            vp = valid.ctypes.data_as(_ip)
+        _check(self._lib.fmt2d_line_search(self._h, C.byref(p), _p(t_obs), _p(dm), vp, C.byref(r)))
         return dm, r
 
     def _nray_last(self):

@@ -207,7 +207,8 @@ int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const fmt2d_
                         const double* dm_sum, double* dm_out, fmt2d_damping_result* res);
 /* The model as the context holds it: what 0 the velocity grid (nz x nx,
  * x fastest; the propagation grid nodes), 1..5 an interface's node depths
- * (2 seafloor, 3 sediment base, 4 Moho).  With out == NULL only *n. */
+ * (2 seafloor, 3 sediment base, 4 Moho).  With out == NULL or cap == 0
+ * only *n. */
 int fmt2d_model_get(fmt2d_ctx* ctx, int what, double* out, int cap, int* n);
 /* Replaces the velocity grid (what 0) or an interface's node depths (2..4)
  * and rebuilds the geometry, as restore_model_state does
@@ -237,9 +238,13 @@ typedef struct {
  * Frechet rows), the RMS of the selected residuals (compute_rms_after_forward,
  * :3298-3339); stops at the first step below the RMS before the update,
  * applies the best one and scales dm in place by the accepted steps.  Needs
- * the system of this iteration (fmt2d_build_system). */
+ * the system of this iteration (fmt2d_build_system).  A ray that one trial
+ * forward loses stays out of the RMS of the later trials, as ray%valid is
+ * only set again at the start of the next iteration (:283-288).  valid
+ * (nray, may be NULL): in, the validity of the iteration's rays (NULL: that
+ * of the last trace); out, and-ed with every trial forward's. */
 int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const double* t_obs, double* dm,
-                      fmt2d_line_search_result* res);
+                      int* valid, fmt2d_line_search_result* res);
 
 #ifdef __cplusplus
 }
