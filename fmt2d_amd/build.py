@@ -1,9 +1,10 @@
 """Builds the in-tree HIP library fmt2d_amd/libfmt2d_hip.so for gfx950 and
-the Fortran host program fmt2d_amd/fortran/invert2d_hip on top of it.
+the Fortran host programs on top of it: fmt2d_amd/fortran/fmt2d_hip
+(forward modelling) and fmt2d_amd/fortran/invert2d_hip (inversion).
 
 Run ``python -m fmt2d_amd.build``.  The library carries every kernel of the
-path and the C ABI declared in include/fmt2d_hip.h; the Fortran program
-binds that ABI through iso_c_binding (fmt2d_amd/fortran/fmt2d_hip_mod.f90).
+path and the C ABI declared in include/fmt2d_hip.h; the Fortran programs
+bind that ABI through iso_c_binding (fmt2d_amd/fortran/fmt2d_hip_mod.f90).
 """
 from __future__ import annotations
 
@@ -18,6 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfmt2d_hip.so")
 FORTRAN = os.path.join(HERE, "fortran")
 INVERT2D = os.path.join(FORTRAN, "invert2d_hip")
+FMT2D = os.path.join(FORTRAN, "fmt2d_hip")
+F_MODULES = ["fmt2d_hip_mod.f90", "fmt2d_io.f90"]
+F_PROGRAMS = {"fmt2d_hip.f90": FMT2D, "invert2d_hip.f90": INVERT2D}
 
 ARCH = os.environ.get("FMT2D_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["eik_kernels.hip", "rays.hip", "csr.hip", "lsqr.hip"]
@@ -72,27 +76,33 @@ def fortran_compiler():
 
 
 def build_fortran(verbose: bool = False):
-    """The Fortran program, linked to the library next to it (rpath
+    """The Fortran programs, linked to the library next to them (rpath
     $ORIGIN/..).  Skipped, with a note, when no Fortran compiler is found."""
     fc = fortran_compiler()
     if fc is None:
-        print("fmt2d_amd.build: no Fortran compiler (amdflang); invert2d_hip not built", file=sys.stderr)
+        print("fmt2d_amd.build: no Fortran compiler (amdflang); Fortran programs not built", file=sys.stderr)
         return None
-    objdir = os.path.join(HERE, "_obj")
-    os.makedirs(objdir, exist_ok=True)
-    mod_o = os.path.join(objdir, "fmt2d_hip_mod.o")
-    tmp = INVERT2D + ".tmp"
-    cmds = [[fc, "-O3", "-module-dir", objdir, "-c", os.path.join(FORTRAN, "fmt2d_hip_mod.f90"), "-o", mod_o],
-            [fc, "-O3", "-module-dir", objdir, os.path.join(FORTRAN, "invert2d_hip.f90"), mod_o, "-L", HERE,
-             "-lfmt2d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]]
-    for cmd in cmds:
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd))
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"Fortran build failed:\n{' '.join(cmd)}\n{r.stderr}")
-    os.replace(tmp, INVERT2D)
-    return INVERT2D
+
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    mods = []
+    for src in F_MODULES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        run([fc, "-O3", "-module-dir", objdir, "-c", os.path.join(FORTRAN, src), "-o", obj])
+        mods.append(obj)
+    for src, exe in F_PROGRAMS.items():
+        tmp = exe + ".tmp"
+        run([fc, "-O3", "-module-dir", objdir, os.path.join(FORTRAN, src)] + mods +
+            ["-L", HERE, "-lfmt2d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+        os.replace(tmp, exe)
+    return list(F_PROGRAMS.values())
 
 
 if __name__ == "__main__":

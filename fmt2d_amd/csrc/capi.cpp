@@ -141,6 +141,13 @@ int model_upload(fmt2d_ctx* ctx) {
   }
   idv[0] = idv[1];
   HIPCHK(upload(ctx->isecs, idv, st));
+  for (int k = 1; k <= NIF; k++) {
+    HIPCHK(upload(ctx->isnp[k][0], m.isec[k].pnrm_x, st));
+    HIPCHK(upload(ctx->isnp[k][1], m.isec[k].pnrm_z, st));
+    idv[k].nx = ctx->isnp[k][0].p; idv[k].nz = ctx->isnp[k][1].p;
+  }
+  idv[0] = idv[1];
+  HIPCHK(upload(ctx->isecs_prog, idv, st));
   // PmP head-wave map.  The reference addresses the PmP region (6) with the
   // Moho's rright_node_id, i.e. region 4's top-node numbering
   // (inversion2d.f90:4298-4301, propagate2d.f90:1593-1597).
@@ -204,7 +211,12 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (!ctx || (nsrc > 0 && !src_x)) return fail(FMT2D_EINVAL, "NULL argument");
   if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
   if (nsrc < 1) return fail(FMT2D_EINVAL, "nsrc must be >= 1");
+  if (flags & ~3) return fail(FMT2D_EINVAL, "unknown forward flag");
   HIPCHK(hipSetDevice(ctx->device));
+  // the forward program's variant has no head-wave post-pass
+  const bool program = (flags & FMT2D_FWD_PROGRAM) != 0;
+  if (program) flags |= FMT2D_FWD_NO_HEADWAVE;
+  ctx->fwd_program = program;
   HostModel& m = ctx->m;
   hipStream_t st = ctx->stream;
   const int nx = m.nx, nz = m.nz;
@@ -324,14 +336,16 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3 + 64));
   HIPCHK(hipMemsetAsync(ctx->acc.p, 0, (3 + 64) * sizeof(unsigned long long), st));
   const int max_launch = 64 * (ntx + ntz) + 4000;
-  const bool headwave = !(flags & 1);
+  const bool headwave = !(flags & FMT2D_FWD_NO_HEADWAVE);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
   const double window = ctx->window_tiles * 8.0 * std::max(m.dx, m.dz) / 4.0;
   size_t nev = 0;  // event pairs recorded around fim launches
   for (int L = 0; L < 4; L++) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
     const FieldDesc* dl = ctx->descs.p + f0;
-    if (L > 0) HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz, ctx->isecs.p, st));
+    if (L > 0)
+      HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz,
+                                     program ? ctx->isecs_prog.p : ctx->isecs.p, st));
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
     // [1] / [2] sizes of active lists 0 / 1
     HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 4, st));

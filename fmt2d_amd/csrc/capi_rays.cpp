@@ -103,6 +103,7 @@ int build_ray_geom(fmt2d_ctx* ctx) {
 // Frechet rows of the last trace, assembled on the device once per trace.
 int frechet_build(fmt2d_ctx* ctx) {
   if (ctx->g_nnz >= 0) return FMT2D_OK;
+  if (ctx->fwd_program) return fail(FMT2D_ESTATE, "Frechet rows need a forward without FMT2D_FWD_PROGRAM");
   hipStream_t st = ctx->stream;
   const int nray = ctx->nray;
   // pass 1: contributions per ray
@@ -219,7 +220,7 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   if (ctx->rtsf.n < (size_t)ctx->nsrc * 3 * nsf) HIPCHK(ctx->rtsf.alloc((size_t)ctx->nsrc * 3 * nsf));
   HIPCHK(launch_seafloor_times(ctx->rgeom.p, ctx->rfields.p, ctx->nsrc, nsf, ctx->rtsf.p, st));
   HIPCHK(launch_trace(ctx->rgeom.p, ctx->rfields.p, ctx->d_srcx.p, ctx->d_srcz.p, ctx->rtsf.p, ctx->rjobs.p, nray,
-                      ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, st));
+                      ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, ctx->fwd_program ? 1 : 0, st));
   std::vector<RayOut> out(nray);
   HIPCHK(hipMemcpyAsync(out.data(), ctx->rout.p, sizeof(RayOut) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -243,7 +244,7 @@ int fmt2d_ray_info(fmt2d_ctx* ctx, int iray, double* entry_xz, double* water_tim
   entry_xz[0] = o.ex; entry_xz[1] = o.ez;
   *water_time = o.water_time;
   for (int k = 0; k < RAY_MAXSEC; k++) npts[k] = o.npts[k];
-  flags[0] = o.headwave; flags[1] = o.diffracted;
+  flags[0] = o.headwave; flags[1] = o.diffracted; flags[2] = o.direct;
   return FMT2D_OK;
 }
 

@@ -70,6 +70,9 @@ struct fmt2d_ctx {
   DBuf<RegionDev> regs;
   DBuf<double> isn[NIF + 1][4];  // nx, nz, vel_top, vel_bot
   DBuf<IsecDev> isecs;
+  DBuf<double> isnp[NIF + 1][2];  // normals of the forward program's reflection
+  DBuf<IsecDev> isecs_prog;       // isecs with those normals
+  bool fwd_program = false;       // last forward ran as the forward program (FMT2D_FWD_PROGRAM)
   DBuf<int> hw_slot;
   DBuf<double> hw_vhw;
   DBuf<HeadwaveMap> hw_map;

@@ -150,6 +150,25 @@ std::string build_model(HostModel& m, int nx_s, int nz_s, const double* xpos, co
   return rebuild_geometry(m);
 }
 
+// interface_normal2d (rays2d.f90:1264-1306) at node c's own x: the cell is
+// found by floor on the grid spacing, and the slope is a central difference
+// on that cell's nodes (one-sided at the first cell).
+static void program_normal(const HostInterface& is, double dx, int c, double& nx_, double& nz_) {
+  const int n = is.n;
+  if (n < 2) { nx_ = 0.0; nz_ = 1.0; return; }
+  int ix = (int)floor((is.x[c] - is.x[0]) / dx) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= n) ix = n - 1;
+  double dzdx;
+  if (ix > 1 && ix < n - 1) dzdx = (is.z[ix] - is.z[ix - 2]) / (2.0 * dx);
+  else if (ix == 1 && n >= 3) dzdx = (is.z[ix] - is.z[ix - 1]) / dx;
+  else if (ix == n - 1 && n >= 3) dzdx = (is.z[ix] - is.z[ix - 2]) / (2.0 * dx);
+  else dzdx = 0.0;
+  const double nl = sqrt(1.0 + dzdx * dzdx);
+  nx_ = -dzdx / nl;
+  nz_ = 1.0 / nl;
+}
+
 std::string rebuild_geometry(HostModel& m) {
   const int nx = m.nx, nz = m.nz;
   // ---- intersections: normals, cut cells (:1335-1500)
@@ -160,6 +179,7 @@ std::string rebuild_geometry(HostModel& m) {
     is.x = m.if_x[k];
     is.z = m.if_z[k];
     is.nrm_x.resize(nx); is.nrm_z.resize(nx);
+    is.pnrm_x.resize(nx); is.pnrm_z.resize(nx);
     is.cc.assign(nx, -1); is.ok.assign(nx, 0);
     for (int c = 0; c < nx; c++) {
       double dzdx;
@@ -169,6 +189,7 @@ std::string rebuild_geometry(HostModel& m) {
       const double nl = sqrt(1.0 + dzdx * dzdx);
       is.nrm_x[c] = -dzdx / nl;
       is.nrm_z[c] = 1.0 / nl;
+      program_normal(is, m.dx, c, is.pnrm_x[c], is.pnrm_z[c]);
       const double zv = is.z[c];
       int izc = -1;
       for (int j = 0; j < nz - 1; j++)

@@ -14,6 +14,9 @@ constexpr int NREG = 6;  // water, sediment, crust, mantle, seafloor->bottom, se
 struct HostInterface {      // one intersection of an interface with the grid
   int n = 0;
   std::vector<double> x, z, nrm_x, nrm_z, vel_top, vel_bot;
+  // normals as the forward program's reflection takes them: interface_normal2d
+  // evaluated at each node's x (rays2d.f90:1264-1306, fmt2d_main.f90:1563)
+  std::vector<double> pnrm_x, pnrm_z;
   std::vector<int> cc;      // 0-based cut-cell row of node c (cell column c), -1 none
   std::vector<uint8_t> ok;  // node inside its own cut cell
 };

@@ -76,13 +76,14 @@ struct RayOut {
   int nsec;
   int npts[RAY_MAXSEC];
   int headwave, diffracted;
+  int direct;          // Pg taken as the direct water wave (no sections)
 };
 
 hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, int nsrc, int nsf, double* tsf,
                                  hipStream_t st);
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
                         const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
-                        RayOut* out, hipStream_t st);
+                        RayOut* out, int program, hipStream_t st);
 // Frechet contributions of each ray in the reference's accumulation order:
 // pass 1 (keys == nullptr) counts them per ray, pass 2 writes
 // key = ray << 32 | (parameter - 1), value = weight * length (or the

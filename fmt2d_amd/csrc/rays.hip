@@ -555,7 +555,7 @@ __global__ void seafloor_times_kernel(const RayGeomDev* __restrict__ Gp, const R
 __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
                              const double* __restrict__ src_x, const double* __restrict__ src_z,
                              const double* __restrict__ tsf, const RayJob* __restrict__ jobs, int nray,
-                             const long long* __restrict__ pt_off, double2* pts, RayOut* out) {
+                             const long long* __restrict__ pt_off, double2* pts, RayOut* out, int program) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nray) return;
   const RayGeomDev& G = *Gp;
@@ -563,7 +563,7 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
   RayOut o;
   o.t = 0.0; o.water_time = 0.0; o.valid = 1; o.nsec = c_path_ntf[J.path];
   for (int k = 0; k < RAY_MAXSEC; k++) o.npts[k] = 0;
-  o.headwave = 0; o.diffracted = 0;
+  o.headwave = 0; o.diffracted = 0; o.direct = 0;
   const int nsec = o.nsec, s = J.src;
   const double sx = src_x[s], sz = src_z[s];
   // receiver placement (invert2d_main.f90:517-531, inversion2d.f90:4502-4519):
@@ -574,7 +574,10 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
     if (J.path == 4 || J.path == 5) {
       rx = J.rx; rz = J.rz; wt = 0.0;
     } else {
-      // find_optimal_entry_inv (inversion2d.f90:6344-6438)
+      // find_optimal_entry_inv (inversion2d.f90:6344-6438); the forward
+      // program's find_optimal_entry (fmt2d_main.f90:1803-1905) also passes
+      // over seafloor points within 3 dx of the source
+      const double min_src = program ? 3.0 * G.dx : -1.0;
       const int itf = c_path_tf[J.path][nsec - 1];
       const RayField& F = fields[s * 12 + itf - 1];
       const RayRegionDev& R = G.reg[F.region];
@@ -583,6 +586,10 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
       const double z0 = seafloor_z(G, J.rx);
       double xb = J.rx, zb = z0, wb = (z0 - J.rz) / G.vwater;
       for (int i = 0; i < G.nsf; i++) {
+        if (program) {
+          const double qx = G.sfx[i] - sx, qz = G.sfz[i] - sz;
+          if (sqrt(qx * qx + qz * qz) < min_src) continue;
+        }
         const double tf_ = ts[i];
         if (tf_ > RAY_HUGE * 0.5) continue;
         const double ddx = G.sfx[i] - J.rx, ddz = G.sfz[i] - J.rz;
@@ -598,6 +605,10 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
         for (int k = 1; k <= 9; k++) {
           const double xt = xl + (double)k * dxr;
           const double zt = seafloor_z(G, xt);
+          if (program) {
+            const double qx = xt - sx, qz = zt - sz;
+            if (sqrt(qx * qx + qz * qz) < min_src) continue;
+          }
           const double tf_ = interp_time(G, R, F, xt, zt);
           if (tf_ > RAY_HUGE * 0.5) continue;
           const double ddx = xt - J.rx, ddz = zt - J.rz;
@@ -607,9 +618,24 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
         }
       }
       rx = xb; rz = zb; wt = wb;
+      if (program && J.path == 1) {
+        // Pg: the direct water wave when it is no later than Pg through the
+        // entry point (fmt2d_main.f90:1249-1280)
+        const double ddx = J.rx - sx, ddz = J.rz - sz;
+        const double t_direct = sqrt(ddx * ddx + ddz * ddz) / G.vwater;
+        if (t_direct <= interp_time(G, R, F, rx, rz) + wt) {
+          o.ex = rx; o.ez = rz;
+          o.t = t_direct; o.water_time = 0.0; o.direct = 1;
+          out[r] = o;
+          return;
+        }
+      }
     }
   }
-  o.ex = rx; o.ez = rz; o.water_time = wt;
+  o.ex = rx; o.ez = rz;
+  // the forward program keeps the water leg of PsedP and PmP rays only
+  // (fmt2d_main.f90:1290-1294, 1334-1337); it is in the receiver time either way
+  o.water_time = (program && J.path == 1) ? 0.0 : wt;
   // trace_ray_from_receiver2d (rays2d.f90:7-153)
   RayState st{true, false, false};
   double x_in = rx, z_in = rz, x_out = 0.0, z_out = 0.0;
@@ -647,6 +673,7 @@ __global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* 
   }
   o.valid = st.valid ? 1 : 0;
   if (o.valid) o.t = o.t + wt;
+  else if (program) o.water_time = 0.0;  // set on valid rays only (:1336)
   o.headwave = st.headwave; o.diffracted = st.diffracted;
   out[r] = o;
 }
@@ -785,10 +812,10 @@ hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, in
 
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
                         const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
-                        RayOut* out, hipStream_t st) {
+                        RayOut* out, int program, hipStream_t st) {
   if (nray == 0) return hipSuccess;
   hipLaunchKernelGGL(trace_kernel, dim3((nray + 63) / 64), dim3(64), 0, st, G, fields, src_x, src_z, tsf, jobs, nray,
-                     pt_off, pts, out);
+                     pt_off, pts, out, program);
   return hipGetLastError();
 }
 

@@ -127,7 +127,7 @@ module fmt2d_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: iray
       real(c_double), intent(out) :: entry_xz(2), water_time
-      integer(c_int), intent(out) :: npts(4), flags(2)
+      integer(c_int), intent(out) :: npts(4), flags(3)
     end function
     integer(c_int) function fmt2d_ray_points(ctx, iray, section, x, z, cap, npts) bind(C, name="fmt2d_ray_points")
       import :: c_int, c_ptr, c_double

@@ -20,6 +20,10 @@ LIB_PATH = os.path.join(HERE, "libfmt2d_hip.so")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
+# fmt2d_forward flags (include/fmt2d_hip.h)
+FWD_NO_HEADWAVE = 1
+FWD_PROGRAM = 2
+
 
 class GridInfo(C.Structure):
     _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
@@ -167,9 +171,13 @@ class Context:
         _check(self._lib.fmt2d_grid_info_get(self._h, C.byref(info)))
         self.info = info
 
-    def forward(self, src_x: Sequence[float], headwave: bool = True):
+    def forward(self, src_x: Sequence[float], headwave: bool = True, program: bool = False):
+        """All fields of every source.  program: the forward program's variant
+        (FMT2D_FWD_PROGRAM: its reflection normals, no head-wave pass, its ray
+        rules in the next trace)."""
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
-        _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), 0 if headwave else 1))
+        flags = (0 if headwave else FWD_NO_HEADWAVE) | (FWD_PROGRAM if program else 0)
+        _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), flags))
 
     def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 8, colors: int = 2):
         _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
@@ -223,10 +231,11 @@ class Context:
         e = np.zeros(2)
         w = C.c_double()
         npts = np.zeros(4, dtype=np.int32)
-        fl = np.zeros(2, dtype=np.int32)
+        fl = np.zeros(3, dtype=np.int32)
         _check(self._lib.fmt2d_ray_info(self._h, iray, _p(e), C.byref(w), npts.ctypes.data_as(_ip),
                                         fl.ctypes.data_as(_ip)))
-        return {"entry": e, "water_time": w.value, "npts": npts, "headwave": bool(fl[0]), "diffracted": bool(fl[1])}
+        return {"entry": e, "water_time": w.value, "npts": npts, "headwave": bool(fl[0]), "diffracted": bool(fl[1]),
+                "direct": bool(fl[2])}
 
     def ray_points(self, iray: int, section: int):
         n = C.c_int()

@@ -57,6 +57,16 @@ extern "C" {
 /* number of time fields per source: tf1..tf12 of the reference */
 #define FMT2D_NTF 12
 
+/* fmt2d_forward flags */
+#define FMT2D_FWD_NO_HEADWAVE 1  /* skip the PmP head-wave post-pass (inversion2d.f90:4294-4409) */
+#define FMT2D_FWD_PROGRAM 2      /* as the forward program fmt2d (fmt2d_main.f90:757-1363): no
+                                  * head-wave post-pass, reflection normals from interface_normal2d
+                                  * at each node (:1563), and the rays of fmt2d_trace by its rules:
+                                  * entry search skipping seafloor points within 3 dx of the source
+                                  * (:1803-1905), Pg of a receiver in the water compared with the
+                                  * direct water wave (:1249-1280), the water leg reported for
+                                  * PsedP and PmP only (:1334-1337) */
+
 typedef struct fmt2d_ctx fmt2d_ctx;
 
 typedef struct {
@@ -93,7 +103,8 @@ int fmt2d_grid_info_get(fmt2d_ctx* ctx, fmt2d_grid_info* info);
 
 /* All twelve time fields of every source (sources placed on the seafloor at
  * src_x), solved as four dependency levels of batched stages.  Fields stay
- * resident on the device.  flags: bit0 = skip the PmP head-wave post-pass. */
+ * resident on the device.  flags: FMT2D_FWD_* (0: as the inversion's
+ * run_forward2d, inversion2d.f90:4145-4545). */
 int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags);
 int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
 /* Solver tuning: arrival-time window of the tile schedule in tile crossing
@@ -129,7 +140,9 @@ int fmt2d_inversion_info(fmt2d_ctx* ctx, int* n_vel_par, int* n_par);
 int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const double* rx, const double* rz,
                 double* t_rcv, int* valid);
 /* Ray r of the last trace: entry point (x, z) the ray starts from, water-leg
- * time, points per section (4), flags (headwave, diffracted). */
+ * time, points per section (4), flags[3] (headwave, diffracted, direct water
+ * wave: a Pg of FMT2D_FWD_PROGRAM taken as the straight water path, no
+ * sections). */
 int fmt2d_ray_info(fmt2d_ctx* ctx, int iray, double* entry_xz, double* water_time, int* npts, int* flags);
 /* Points of section `section` (1 = at the source) of ray r; with x == NULL
  * only *npts is returned. */
