@@ -17,6 +17,13 @@ Laplacian smoothing), with the reference's own inversion timed beside it.
 The line search that completes the iteration (forwards without Frechet rows,
 model steps) is reported apart, for both.
 
+Third figure (``programs``): wall seconds of the Fortran host programs
+(fmt2d_amd/fortran/fmt2d_hip and invert2d_hip, process start to exit) on the
+plain case, beside the reference's own fmt2d and invert2d on the same input
+files: the forward program over the 20 OBS (all 12 fields, 7980 Pg rays,
+arrival files), and a two-iteration inversion (fixed damping, with the
+initial and final forwards and every output file).
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -211,6 +218,59 @@ def inversion_iteration(steps: int, warmup: int, cpu: bool):
             "ndata": nd, "nnz": nnz, "lsqr_iterations": iters, "cpu_baseline": cpu_b}
 
 
+def run_program(exe, d, timeout=900):
+    t0 = time.perf_counter()
+    r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+    sec = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"{exe} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return sec
+
+
+def programs_figure(cpu: bool):
+    """Wall seconds of the Fortran host programs on the plain case, and of the
+    reference's programs on the same files (see module docstring)."""
+    from fmt2d_amd import synth
+    from fmt2d_amd.build import FMT2D, INVERT2D
+    if not (os.path.exists(FMT2D) and os.path.exists(INVERT2D)):
+        return None
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    out = {"workload": "plain case: 401x61 grid (0.5 km), 20 OBS, 399 shots per OBS at 10 m depth, Pg"}
+    m_true, m0, g = inversion_case_dir(None)
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_fmt2d_case(d, m_true, g)
+        ours = run_program(FMT2D, d)
+        arr = open(os.path.join(d, "arrivals.dat")).read()
+        ref = None
+        if cpu and os.path.exists(os.path.join(ref_dir, "fmt2d")):
+            ref = run_program(os.path.join(ref_dir, "fmt2d"), d)
+            ref_same = open(os.path.join(d, "arrivals.dat")).read() == arr
+        out["fmt2d"] = {"seconds": round(ours, 3), "reference_seconds": None if ref is None else round(ref, 3),
+                        "arrivals_identical": None if ref is None else ref_same}
+        # picks from the forward program's arrivals (valid rays), as the
+        # example scripts make them; then a two-iteration inversion
+        rows = [ln.split() for ln in arr.splitlines()]
+        t = np.array([float(r[3]) for r in rows])
+        gi = synth.with_picks(g, t, t > 0)
+        synth.write_fmt2d_case(d, m0, gi)
+        with open(os.path.join(d, "frechet.in"), "w") as f:
+            f.write("1\n1\n1\n1\n0\n")
+        with open(os.path.join(d, "invert2d.in"), "w") as f:
+            f.write("model.smesh\nmoho.dat\nsed.dat\ngeom.dat\n")
+            for v in [2, 0.1, 0.3, 1.0, 400, 0.05, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0]:
+                f.write(f"{v}\n")
+            f.write("1 0.0 0.5 0.3\n-1\n")
+        ours = run_program(INVERT2D, d)
+        log = open(os.path.join(d, "invert_log_latest.txt")).read()
+        ref = None
+        if cpu and os.path.exists(os.path.join(ref_dir, "invert2d")):
+            ref = run_program(os.path.join(ref_dir, "invert2d"), d)
+            ref_same = open(os.path.join(d, "invert_log_latest.txt")).read() == log
+        out["invert2d"] = {"seconds": round(ours, 3), "reference_seconds": None if ref is None else round(ref, 3),
+                           "iterations": 2, "log_identical": None if ref is None else ref_same}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +279,7 @@ def main():
     ap.add_argument("--workload", default="multistage", choices=["multistage", "plain"])
     ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
+    ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--iteration-only", action="store_true",
                     help="only the inversion-iteration figure (for profiling its kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -271,6 +332,7 @@ def main():
         ctx.close()
 
     it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
+    progs = None if args.no_programs else programs_figure(args.cpu_sample > 0)
     wall = float(np.sum(walls))
     stages = 5 * len(obs)
     value = stages * args.steps / wall
@@ -308,6 +370,7 @@ def main():
                                           f"{alg_bytes / ndisp:.0f} B per dispatch"},
         "cpu_baseline": cpu,
         "iteration": it,
+        "programs": progs,
     }
     print(json.dumps(out))
 
