@@ -74,6 +74,7 @@ int build_ray_geom(fmt2d_ctx* ctx) {
       }
   }
   HIPCHK(upload(ctx->rinv, inv, st));
+  ctx->h_inv = inv;
   G.inv_idx = ctx->rinv.p;
   G.n_vel_par = np;
   int group = ctx->inv_vel ? 1 : 0, last = np;
@@ -138,10 +139,55 @@ int frechet_build(fmt2d_ctx* ctx) {
   ctx->g_col.p = co.col; ctx->g_col.n = (size_t)std::max(co.nnz, 1);
   ctx->g_val.p = co.val; ctx->g_val.n = (size_t)std::max(co.nnz, 1);
   ctx->g_nnz = co.nnz;
+  ctx->gt_ok = false;
   return FMT2D_OK;
 }
 
 extern "C" {
+
+int fmt2d_param_index(fmt2d_ctx* ctx, int* node_param, int cap, int* n) {
+  if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
+  *n = (int)ctx->h_inv.size();
+  if (node_param && cap > 0) {
+    if (cap < *n) return fail(FMT2D_EINVAL, "output buffer too small");
+    memcpy(node_param, ctx->h_inv.data(), sizeof(int) * ctx->h_inv.size());
+  }
+  return FMT2D_OK;
+}
+
+int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* y) {
+  if (!ctx || !x || !y) return fail(FMT2D_EINVAL, "NULL argument");
+  if (transpose != 0 && transpose != 1) return fail(FMT2D_EINVAL, "transpose must be 0 or 1");
+  if (ctx->nray < 1) return fail(FMT2D_ESTATE, "no rays traced");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int rc0 = frechet_build(ctx);
+  if (rc0) return rc0;
+  hipStream_t st = ctx->stream;
+  const int nray = ctx->nray, np = ctx->n_par, nnz = ctx->g_nnz;
+  if (transpose && !ctx->gt_ok) {
+    HIPCHK(ctx->gt_cp.alloc((size_t)np + 1));
+    HIPCHK(ctx->gt_row.alloc((size_t)std::max(nnz, 1)));
+    HIPCHK(ctx->gt_val.alloc((size_t)std::max(nnz, 1)));
+    HIPCHK(build_csc(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, np, nnz, ctx->gt_cp.p, ctx->gt_row.p,
+                     ctx->gt_val.p, st));
+    ctx->gt_ok = true;
+  }
+  const int nin = transpose ? nray : np, nout = transpose ? np : nray;
+  DBuf<double> dx, dy;
+  HIPCHK(dx.alloc((size_t)std::max(nin, 1)));
+  HIPCHK(dy.alloc((size_t)std::max(nout, 1)));
+  HIPCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * nin, hipMemcpyHostToDevice, st));
+  if (transpose)
+    HIPCHK(launch_ordered_spmv(ctx->gt_cp.p, ctx->gt_row.p, ctx->gt_val.p, np, dx.p, dy.p, st));
+  else
+    HIPCHK(launch_ordered_spmv(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, dx.p, dy.p, st));
+  HIPCHK(hipMemcpyAsync(y, dy.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return FMT2D_OK;
+}
 
 int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids) {
   if (!ctx || (n_iface > 0 && !iface_ids) || n_iface < 0 || n_iface > 3)

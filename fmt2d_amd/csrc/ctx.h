@@ -112,6 +112,7 @@ struct fmt2d_ctx {
   DBuf<double> risx[NIF + 1], risz[NIF + 1], rifx_in[NIF + 1];
   DBuf<double> rsfx, rsfz, rsfz0, rpvel, rvx, rvz;
   DBuf<int> rinv;
+  std::vector<int> h_inv;              // parameter number of each velocity-grid node (0: none)
   DBuf<RayField> rfields;
   int nray = 0;
   std::vector<RayJob> hjobs;
@@ -126,6 +127,9 @@ struct fmt2d_ctx {
   // Frechet rows of the last trace on the device (0-based columns)
   DBuf<int> g_rp, g_col;
   DBuf<double> g_val;
+  DBuf<int> gt_cp, gt_row;             // column copy of G for G^T x (fmt2d_frechet_apply)
+  DBuf<double> gt_val;
+  bool gt_ok = false;
   int g_nnz = -1;                      // -1: not built for the current trace
   // tomography system (selected, weighted, scaled rows) and its column copy
   int sys_ndata = -1, sys_nnz = 0;

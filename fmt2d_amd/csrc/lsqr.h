@@ -50,6 +50,10 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
 hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double* src_val, const int* keep_row,
                              const double* wt, const int* rp, int ndata, int base, const double* mvs, int nparm,
                              int* col, double* val, hipStream_t st);
+// y[i] = sum over the stored entries of row i of M times x, in stored order
+// (one wave per row)
+hipError_t launch_ordered_spmv(const int* ptr, const int* idx, const double* val, int nrow, const double* x,
+                               double* y, hipStream_t st);
 hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st);
 // column-major copy of a CSR matrix (stable: rows ascending within a column)
 hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow, int ncol, int nnz, int* cp, int* crow,

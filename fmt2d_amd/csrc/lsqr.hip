@@ -18,7 +18,7 @@
 
 namespace fmt2d {
 
-__device__ inline bool stopped(const LsqrScalars* s) { return s->done != 0; }
+__device__ inline bool stopped(const LsqrScalars* s) { return s != nullptr && s->done != 0; }
 
 // One wave per row: y[i] = sum_k M(i,k) x(idx k), ascending k.  The lanes
 // form the products of 64 entries at a time (coalesced loads, gathers of x)
@@ -423,6 +423,14 @@ hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double
 
 hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st) {
   hipLaunchKernelGGL(gtg_kernel, dim3(nb(nparm, 128)), dim3(128), 0, st, cp, cval, nparm, gtg);
+  return hipGetLastError();
+}
+
+hipError_t launch_ordered_spmv(const int* ptr, const int* idx, const double* val, int nrow, const double* x,
+                               double* y, hipStream_t st) {
+  if (nrow <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nrow), dim3(64), 0, st, ptr, idx, val, nrow, x, y,
+                     (const LsqrScalars*)nullptr);
   return hipGetLastError();
 }
 

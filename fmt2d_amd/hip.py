@@ -113,6 +113,8 @@ def load() -> C.CDLL:
     lib.fmt2d_apply_model_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     lib.fmt2d_model_get.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
     lib.fmt2d_model_put.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
+    lib.fmt2d_frechet_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.fmt2d_param_index.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
     lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp, _ip,
                                       C.POINTER(LineSearchResult)]
     _LIB = lib
@@ -127,7 +129,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
-           "fmt2d_frechet_csr", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
+           "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
            "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
 
 
@@ -244,6 +246,26 @@ class Context:
         _check(self._lib.fmt2d_ray_points(self._h, iray, section, _p(x), _p(z), len(x), C.byref(n)))
         return x[:n.value], z[:n.value]
 
+    def frechet_apply(self, x, transpose: bool = False):
+        """G x (x per parameter) or G^T x (x per ray of the last trace) with the
+        unscaled Frechet rows, sums in the reference's order."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _, n_par = self.inversion_info()
+        nray = self._nray
+        if len(x) != (nray if transpose else n_par):
+            raise ValueError("x has the wrong length")
+        y = np.zeros(n_par if transpose else nray)
+        _check(self._lib.fmt2d_frechet_apply(self._h, int(bool(transpose)), _p(x), _p(y)))
+        return y
+
+    def param_index(self):
+        """Parameter number (1-based, 0 none) of each velocity-grid node, (nz, nx)."""
+        n = C.c_int()
+        _check(self._lib.fmt2d_param_index(self._h, None, 0, C.byref(n)))
+        idx = np.zeros(n.value, dtype=np.int32)
+        _check(self._lib.fmt2d_param_index(self._h, idx.ctypes.data_as(_ip), len(idx), C.byref(n)))
+        return idx.reshape(self.info.nz, self.info.nx)
+
     def frechet_csr(self, index_base: int = 0):
         """(row_ptr, col, val) of the Frechet rows of the last trace."""
         nnz = C.c_int()
@@ -338,7 +360,12 @@ class Context:
 
     def field(self, isrc: int, itf: int):
         """(T, dT/dx, dT/dz) of field itf (1..12) of source isrc, reference node order."""
-        n = self.info.region_nnode[self._lib.fmt2d_tf_region(itf)]
+        # region sizes change when the geometry is rebuilt (model steps,
+        # interface depths put back): ask for the current ones
+        info = GridInfo()
+        _check(self._lib.fmt2d_grid_info_get(self._h, C.byref(info)))
+        self.info = info
+        n = info.region_nnode[self._lib.fmt2d_tf_region(itf)]
         T, gx, gz = np.empty(n), np.empty(n), np.empty(n)
         _check(self._lib.fmt2d_get_field(self._h, isrc, itf, _p(T), _p(gx), _p(gz)))
         return T, gx, gz

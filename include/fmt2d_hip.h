@@ -152,6 +152,18 @@ int fmt2d_ray_points(fmt2d_ctx* ctx, int iray, int section, double* x, double* z
  * row_ptr, then with buffers of cap >= nnz.  index_base 1 gives Fortran
  * indexing for row_ptr and col. */
 int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, double* val, int cap, int* nnz);
+/* y = G x (transpose 0: x has n_par entries, y nray) or y = G^T x
+ * (transpose 1: x nray, y n_par) with the Frechet rows of the last trace as
+ * fmt2d_frechet_csr returns them (unscaled: d t / d slowness of a velocity
+ * node, d t / d depth of an interface node); each sum in the order
+ * sparse_Ax / sparse_ATx accumulate it (inversion2d.f90:2997-3042).  The
+ * adjoint of the receiver times, e.g. for a torch.autograd backward. */
+int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* y);
+/* Parameter number (1-based; 0: not a parameter) of each velocity-grid node
+ * (nz x nx, x fastest) as initialize_inversion2d numbers them
+ * (frechet2d.f90:44-60); interface parameters follow the n_vel_par velocity
+ * parameters.  With node_param == NULL or cap == 0 only *n. */
+int fmt2d_param_index(fmt2d_ctx* ctx, int* node_param, int cap, int* n);
 
 /* Data selection and scaling (lsqr_inversion2d's system assembly). */
 typedef struct {
