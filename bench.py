@@ -21,8 +21,9 @@ Third figure (``programs``): wall seconds of the Fortran host programs
 (fmt2d_amd/fortran/fmt2d_hip and invert2d_hip, process start to exit) on the
 plain case, beside the reference's own fmt2d and invert2d on the same input
 files: the forward program over the 20 OBS (all 12 fields, 7980 Pg rays,
-arrival files), and a two-iteration inversion (fixed damping, with the
-initial and final forwards and every output file).
+arrival files), and a two-iteration inversion (automatic damping as in the
+reference's example scripts, with the initial and final forwards and every
+output file).
 
 Prints ONE JSON line.
 """
@@ -257,7 +258,7 @@ def programs_figure(cpu: bool):
             f.write("1\n1\n1\n1\n0\n")
         with open(os.path.join(d, "invert2d.in"), "w") as f:
             f.write("model.smesh\nmoho.dat\nsed.dat\ngeom.dat\n")
-            for v in [2, 0.1, 0.3, 1.0, 400, 0.05, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0]:
+            for v in [2, 0.0, 0.3, 1.0, 400, 0.03, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0]:
                 f.write(f"{v}\n")
             f.write("1 0.0 0.5 0.3\n-1\n")
         ours = run_program(INVERT2D, d)

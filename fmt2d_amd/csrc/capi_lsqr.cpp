@@ -241,11 +241,12 @@ int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
 
 }  // extern "C"
 
-// One run_lsqr solve of the built system (fmt2d_lsqr; the damping search
-// calls it repeatedly).
-int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
-               fmt2d_lsqr_stats* stats) {
-  if (!ctx || !p || !dm_sum || !dm_out) return fail(FMT2D_EINVAL, "NULL argument");
+// Everything the solves of one system share: checks, column mask, smoothing
+// weights and tables, Gaussian operators, dm_sum; fills S (without the
+// damping weights).  p's damping weights are not used here.
+static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, LsqrSys& S,
+                        size_t& na_out) {
+  if (!ctx || !p || !dm_sum) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
   if (p->smooth_type < 1 || p->smooth_type > 3)
     return fail(FMT2D_EINVAL, "smooth_type must be 1 (Laplacian), 2 (Gaussian) or 3 (hybrid)");
@@ -253,17 +254,15 @@ int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   if (p->smooth_type == 2 && p->w_smooth > 0.0 && !(p->corr_x > 0.0 && p->corr_z > 0.0))
     return fail(FMT2D_EINVAL, "Gaussian smoothing needs corr_x > 0 and corr_z > 0");
   HIPCHK(hipSetDevice(ctx->device));
-  const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = ctx->stream;
   const int nparm = ctx->n_par, n_vel = ctx->n_vel_par, nd = ctx->sys_ndata;
   const double cut = ctx->sys_eps_min * 0.01;
   std::vector<uint8_t> mask(nparm, 0);
-  std::vector<double> ws(nparm), wd(nparm);
+  std::vector<double> ws(nparm);
   for (int j = 0; j < nparm; j++) {
     const bool vel = j < n_vel;
     mask[j] = (vel && ctx->h_gtg[j] < cut) ? 1 : 0;
     ws[j] = vel ? p->w_smooth : p->w_smooth_iface;
-    wd[j] = vel ? p->wdv : p->wdv_iface;
   }
   // Laplacian tables (apply_laplacian_smooth_2d, inversion2d.f90:1161-1249)
   std::vector<int4> lap(std::max(n_vel, 1));
@@ -333,26 +332,18 @@ int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   std::vector<double> dms(dm_sum, dm_sum + nparm);
   HIPCHK(upload(ctx->l_mask, mask, st));
   HIPCHK(upload(ctx->l_ws, ws, st));
-  HIPCHK(upload(ctx->l_wd, wd, st));
   HIPCHK(upload(ctx->l_lapnb, lap, st));
   HIPCHK(upload(ctx->l_ifnb, ifn, st));
   HIPCHK(upload(ctx->l_dmsum, dms, st));
-  const size_t na = (size_t)nd + (p->smooth_type == 2 ? 3 : 2) * (size_t)nparm;
-  if (ctx->l_u.n < na) HIPCHK(ctx->l_u.alloc(na));
-  if (ctx->l_v.n < (size_t)nparm) {
-    HIPCHK(ctx->l_v.alloc(nparm)); HIPCHK(ctx->l_w.alloc(nparm)); HIPCHK(ctx->l_dm.alloc(nparm));
-    HIPCHK(ctx->l_gtu.alloc(nparm)); HIPCHK(ctx->l_sm.alloc(nparm)); HIPCHK(ctx->l_sm2.alloc(nparm));
-  }
-  if (ctx->l_gv.n < (size_t)std::max(nd, 1)) HIPCHK(ctx->l_gv.alloc((size_t)std::max(nd, 1)));
-  if (!ctx->l_sc.p) HIPCHK(ctx->l_sc.alloc(1));
-  LsqrSys S;
+  na_out = (size_t)nd + (p->smooth_type == 2 ? 3 : 2) * (size_t)nparm;
+  S = LsqrSys{};
   S.ndata = nd; S.nparm = nparm; S.n_vel = n_vel;
   S.rp = ctx->s_rp.p; S.col = ctx->s_col.p; S.val = ctx->s_val.p;
   S.cp = ctx->s_cp.p; S.crow = ctx->s_crow.p; S.cval = ctx->s_cval.p;
-  S.res = ctx->s_res.p; S.dm_sum = ctx->l_dmsum.p; S.mask = ctx->l_mask.p; S.ws = ctx->l_ws.p; S.wd = ctx->l_wd.p;
+  S.res = ctx->s_res.p; S.dm_sum = ctx->l_dmsum.p; S.mask = ctx->l_mask.p; S.ws = ctx->l_ws.p; S.wd = nullptr;
   S.lap_nb = ctx->l_lapnb.p; S.if_nb = ctx->l_ifnb.p;
   S.smooth_on = (p->w_smooth > 0.0 || p->w_smooth_iface > 0.0) ? 1 : 0;
-  S.damp_on = (p->wdv > 0.0 || p->wdv_iface > 0.0) ? 1 : 0;
+  S.damp_on = 0;
   S.stype = p->smooth_type;
   S.w_smooth = p->w_smooth;
   S.w_smooth_iface = p->w_smooth_iface;
@@ -367,19 +358,78 @@ int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum,
   }
   if (want_i) { S.rif_rp = ctx->gi_rp.p; S.rif_col = ctx->gi_col.p; S.rif_val = ctx->gi_val.p; }
   S.gsi = (p->smooth_type == 2 && want_i) ? 1 : 0;
-  LsqrWork W{ctx->l_u.p, ctx->l_v.p,  ctx->l_w.p,   ctx->l_dm.p, ctx->l_gv.p,
-             ctx->l_gtu.p, ctx->l_sm.p, ctx->l_sm2.p, ctx->l_sc.p};
-  LsqrScalars out{};
-  HIPCHK(lsqr_run(S, W, p->max_iter, 1.0e-4, 8, st, &out));
-  HIPCHK(hipMemcpy(dm_out, ctx->l_dm.p, sizeof(double) * nparm, hipMemcpyDeviceToHost));
-  if (stats) {
-    stats->iterations = out.iters;
-    stats->rnorm = out.rnorm;
-    stats->bnorm = out.bnorm;
-    stats->test2 = out.test2;
-    stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return FMT2D_OK;
+}
+
+// Work buffers of solve slot q (0 or 1) for na augmented rows.
+static int lsqr_work(fmt2d_ctx* ctx, int q, size_t na, LsqrWork& W) {
+  const int nparm = ctx->n_par, nd = ctx->sys_ndata;
+  DBuf<double>* b[8] = {&ctx->l_u, &ctx->l_v, &ctx->l_w, &ctx->l_dm, &ctx->l_gv, &ctx->l_gtu, &ctx->l_sm, &ctx->l_sm2};
+  DBuf<double>* b2[8] = {&ctx->l2_u, &ctx->l2_v, &ctx->l2_w, &ctx->l2_dm, &ctx->l2_gv, &ctx->l2_gtu, &ctx->l2_sm,
+                         &ctx->l2_sm2};
+  DBuf<double>** B = q == 0 ? b : b2;
+  DBuf<LsqrScalars>& sc = q == 0 ? ctx->l_sc : ctx->l2_sc;
+  const size_t need[8] = {na, (size_t)nparm, (size_t)nparm, (size_t)nparm, (size_t)std::max(nd, 1),
+                          (size_t)nparm, (size_t)nparm, (size_t)nparm};
+  for (int k = 0; k < 8; k++)
+    if (B[k]->n < need[k]) HIPCHK(B[k]->alloc(need[k]));
+  if (!sc.p) HIPCHK(sc.alloc(1));
+  W = LsqrWork{B[0]->p, B[1]->p, B[2]->p, B[3]->p, B[4]->p, B[5]->p, B[6]->p, B[7]->p, sc.p};
+  return FMT2D_OK;
+}
+
+// n (1 or 2) solves of the built system with damping weights (wdv[q],
+// wdv_if[q]), side by side on two streams.
+int lsqr_solve_many(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, int n, const double* wdv,
+                    const double* wdv_if, double* const* dm_out, fmt2d_lsqr_stats* stats) {
+  if (n < 1 || n > 2) return fail(FMT2D_EINVAL, "one or two solves");
+  for (int q = 0; q < n; q++)
+    if (!dm_out[q]) return fail(FMT2D_EINVAL, "NULL argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  LsqrSys S0;
+  size_t na = 0;
+  int rc = lsqr_prepare(ctx, p, dm_sum, S0, na);
+  if (rc) return rc;
+  const int nparm = ctx->n_par, n_vel = ctx->n_vel_par;
+  if (n == 2 && !ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  LsqrSys S[2];
+  LsqrWork W[2];
+  hipStream_t st[2] = {ctx->stream, ctx->stream2};
+  for (int q = 0; q < n; q++) {
+    std::vector<double> wd(nparm);
+    for (int j = 0; j < nparm; j++) wd[j] = j < n_vel ? wdv[q] : wdv_if[q];
+    HIPCHK(upload(q == 0 ? ctx->l_wd : ctx->l2_wd, wd, ctx->stream));
+    rc = lsqr_work(ctx, q, na, W[q]);
+    if (rc) return rc;
+    S[q] = S0;
+    S[q].wd = q == 0 ? ctx->l_wd.p : ctx->l2_wd.p;
+    S[q].damp_on = (wdv[q] > 0.0 || wdv_if[q] > 0.0) ? 1 : 0;
+  }
+  // the second stream reads what the first one uploaded
+  if (n == 2) HIPCHK(hipStreamSynchronize(ctx->stream));
+  LsqrScalars out[2] = {};
+  HIPCHK(lsqr_run_many(n, S, W, p->max_iter, 1.0e-4, 8, st, out));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int q = 0; q < n; q++) {
+    HIPCHK(hipMemcpy(dm_out[q], W[q].dm, sizeof(double) * nparm, hipMemcpyDeviceToHost));
+    if (stats) {
+      stats[q].iterations = out[q].iters;
+      stats[q].rnorm = out[q].rnorm;
+      stats[q].bnorm = out[q].bnorm;
+      stats[q].test2 = out[q].test2;
+      stats[q].ms = ms;
+    }
   }
   return FMT2D_OK;
+}
+
+// One run_lsqr solve of the built system (fmt2d_lsqr; the damping search
+// calls it repeatedly).
+int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
+               fmt2d_lsqr_stats* stats) {
+  if (!p || !dm_out) return fail(FMT2D_EINVAL, "NULL argument");
+  double* outs[1] = {dm_out};
+  return lsqr_solve_many(ctx, p, dm_sum, 1, &p->wdv, &p->wdv_iface, outs, stats);
 }
 
 

@@ -131,14 +131,44 @@ int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* lp, const fmt2d
   fmt2d_lsqr_params p = *lp;
   int solves = 0, iters = 0;
   std::vector<double> dm(nparm), trial(nparm);
+  // The reference's closing solve repeats the search's last trial (same
+  // system, same two weights, :726-731): the solve is deterministic, so the
+  // last result is reused instead of solving again.
+  std::vector<double> last_out;
+  double last_wv = 0.0, last_wi = 0.0;
+  int last_iters = 0;
   auto solve = [&](double wdv, double wdv_if, std::vector<double>& out) {
+    if (!last_out.empty() && wdv == last_wv && wdv_if == last_wi) {
+      out = last_out;
+      iters = last_iters;
+      return 0;
+    }
     p.wdv = wdv;
     p.wdv_iface = wdv_if;
     fmt2d_lsqr_stats st{};
     const int rc = lsqr_solve(ctx, &p, dms.data(), out.data(), &st);
+    if (rc) return rc;
     solves++;
     iters = st.iterations;
-    return rc;
+    last_out = out;
+    last_wv = wdv;
+    last_wi = wdv_if;
+    last_iters = iters;
+    return 0;
+  };
+  // two independent trials (the first two of a secant search) side by side
+  auto solve2 = [&](const double* wv, const double* wi, std::vector<double>& o0, std::vector<double>& o1) {
+    double* outs[2] = {o0.data(), o1.data()};
+    fmt2d_lsqr_stats st[2] = {};
+    const int rc = lsqr_solve_many(ctx, &p, dms.data(), 2, wv, wi, outs, st);
+    if (rc) return rc;
+    solves += 2;
+    iters = st[1].iterations;
+    last_out = o1;
+    last_wv = wv[1];
+    last_wi = wi[1];
+    last_iters = iters;
+    return 0;
   };
   double target_dv = d->target_dv > 0.0 ? d->target_dv : 0.03;
   double target_dd = d->target_dd > 0.0 ? d->target_dd : 0.5;
@@ -173,10 +203,18 @@ int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* lp, const fmt2d
       };
       const double wd1 = 1.0, wd2 = 100.0;
       double fl, fv;
-      int r = eval(wd1, fl);
+      // f at wd1 and at wd2: independent solves, run together
+      std::vector<double> t2(nparm);
+      const double wv[2] = {iface ? 0.0 : wd1, iface ? 0.0 : wd2};
+      const double wi[2] = {iface ? wd1 : wd_best_if, iface ? wd2 : wd_best_if};
+      int r = solve2(wv, wi, trial, t2);
       if (r) return r;
-      r = eval(wd2, fv);
-      if (r) return r;
+      auto favg = [&](std::vector<double>& out) {
+        out = plus(out, dms);
+        return (iface ? avg_dmd(out, mvs, n_vel, nparm) : avg_dmv(out, gtg, eps, n_vel)) - target;
+      };
+      fl = favg(trial);
+      fv = favg(t2);
       double rts, xl;
       if (fabs(fl) < fabs(fv)) {
         rts = wd1; xl = wd2;

@@ -156,6 +156,10 @@ struct fmt2d_ctx {
   DBuf<int4> l_lapnb;
   DBuf<int2> l_ifnb;
   DBuf<LsqrScalars> l_sc;
+  // second solve running beside the first (damping searches)
+  DBuf<double> l2_u, l2_v, l2_w, l2_dm, l2_gv, l2_gtu, l2_sm, l2_sm2, l2_wd;
+  DBuf<LsqrScalars> l2_sc;
+  hipStream_t stream2 = nullptr;
   // inversion parameterisation (frechet.in): velocity grid and interfaces
   int inv_vel = 1;
   std::vector<int> inv_ifaces;
@@ -170,6 +174,7 @@ struct fmt2d_ctx {
     for (auto e : ev) (void)hipEventDestroy(e);
     if (h_count) (void)hipHostFree(h_count);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -185,3 +190,6 @@ int frechet_build(fmt2d_ctx* ctx);
 int model_upload(fmt2d_ctx* ctx);
 int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats);
+// one or two solves of the built system (damping weights per solve) side by side
+int lsqr_solve_many(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, int n, const double* wdv,
+                    const double* wdv_if, double* const* dm_out, fmt2d_lsqr_stats* stats);

@@ -47,6 +47,10 @@ struct LsqrWork {
 
 hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
                     LsqrScalars* host_out);
+// n independent solves (own systems, work buffers and streams) launched
+// iteration by iteration side by side; host_out[q] receives solve q's scalars
+hipError_t lsqr_run_many(int n, const LsqrSys* S, LsqrWork* W, int max_iter, double atol, int poll,
+                         const hipStream_t* st, LsqrScalars* host_out);
 hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double* src_val, const int* keep_row,
                              const double* wt, const int* rp, int ndata, int base, const double* mvs, int nparm,
                              int* col, double* val, hipStream_t st);

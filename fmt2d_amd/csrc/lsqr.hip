@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <vector>
+
 #include "lsqr.h"
 
 namespace fmt2d {
@@ -318,13 +320,14 @@ __global__ void test_kernel(LsqrScalars* s, int iter, int max_iter, double atol)
 
 static inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
-                    LsqrScalars* host_out) {
+// The kernels of the start of a solve: u = [r; -w L dm_sum; -wd dm_sum]
+// (Gaussian: [r; -w R_h dm_sum; -w R_v dm_sum; -wd dm_sum]), beta, u /= beta,
+// v = A^T u masked, alpha, v /= alpha, w = v.
+static hipError_t launch_start(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, hipStream_t st) {
   const bool gauss = S.stype == 2;
   const long long nd = S.ndata, np = S.nparm, na = nd + (gauss ? 3 : 2) * np;
   LsqrScalars* sc = W.sc;
   (void)hipMemsetAsync(sc, 0, sizeof(LsqrScalars), st);
-  // u = [r; -w L dm_sum; -wd dm_sum] (Gaussian: [r; -w R_h dm_sum; -w R_v dm_sum; -wd dm_sum])
   if (gauss)
     hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, S.dm_sum, W.u + nd, W.u + nd + np, 1,
                        sc);
@@ -334,7 +337,6 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 0, sc);
-  // v = A^T u, masked; alpha; v /= alpha; w = v
   hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
                      W.gtu, sc);
   if (!gauss && S.smooth_on)
@@ -344,43 +346,81 @@ hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, in
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 1, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 0, sc);
   hipLaunchKernelGGL(w_init_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  for (int it = 1; it <= max_iter; it++) {
-    hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
-    if (nd > 0)
-      hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nd), dim3(64), 0, st, S.rp, S.col, S.val, S.ndata, W.v,
-                         W.gv, sc);
-    if (gauss)
-      hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, W.v, W.sm, W.sm2, 0, sc);
-    else if (S.smooth_on)
-      hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
-    hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.sm2, W.v, W.u, sc);
-    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
-    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
-    hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
-    hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
-                       W.gtu, sc);
-    if (!gauss && S.smooth_on)
-      hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
-    hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
-    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
-    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);
-    hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 1, sc);
-    hipLaunchKernelGGL(w_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm, sc);
-    hipLaunchKernelGGL(test_kernel, dim3(1), dim3(64), 0, st, sc, it, max_iter, atol);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (it % poll == 0 || it == max_iter) {
-      e = hipMemcpyAsync(host_out, sc, sizeof(LsqrScalars), hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) return e;
-      if (host_out->done) break;
-    }
-  }
-  e = hipMemcpyAsync(host_out, sc, sizeof(LsqrScalars), hipMemcpyDeviceToHost, st);
+  return hipGetLastError();
+}
+
+// The kernels of LSQR iteration `it` (each returns at once after the stop).
+static hipError_t launch_iteration(const LsqrSys& S, LsqrWork& W, int it, int max_iter, double atol,
+                                   hipStream_t st) {
+  const bool gauss = S.stype == 2;
+  const long long nd = S.ndata, np = S.nparm, na = nd + (gauss ? 3 : 2) * np;
+  LsqrScalars* sc = W.sc;
+  hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
+  if (nd > 0)
+    hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nd), dim3(64), 0, st, S.rp, S.col, S.val, S.ndata, W.v,
+                       W.gv, sc);
+  if (gauss)
+    hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, W.v, W.sm, W.sm2, 0, sc);
+  else if (S.smooth_on)
+    hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
+  hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.sm2, W.v, W.u, sc);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
+  hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
+  hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
+                     W.gtu, sc);
+  if (!gauss && S.smooth_on)
+    hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
+  hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);
+  hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 1, sc);
+  hipLaunchKernelGGL(w_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm, sc);
+  hipLaunchKernelGGL(test_kernel, dim3(1), dim3(64), 0, st, sc, it, max_iter, atol);
+  return hipGetLastError();
+}
+
+static hipError_t poll_scalars(LsqrWork& W, LsqrScalars* host_out, hipStream_t st) {
+  hipError_t e = hipMemcpyAsync(host_out, W.sc, sizeof(LsqrScalars), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e;
+}
+
+hipError_t lsqr_run_many(int n, const LsqrSys* S, LsqrWork* W, int max_iter, double atol, int poll,
+                         const hipStream_t* st, LsqrScalars* host_out) {
+  std::vector<char> active(n, 1);
+  for (int q = 0; q < n; q++) {
+    const hipError_t e = launch_start(S[q], W[q], max_iter, atol, st[q]);
+    if (e != hipSuccess) return e;
+  }
+  int left = n;
+  for (int it = 1; it <= max_iter && left > 0; it++) {
+    // the solves' kernels go to their own streams, iteration by iteration,
+    // so independent solves run side by side
+    for (int q = 0; q < n; q++) {
+      if (!active[q]) continue;
+      const hipError_t e = launch_iteration(S[q], W[q], it, max_iter, atol, st[q]);
+      if (e != hipSuccess) return e;
+    }
+    if (it % poll == 0 || it == max_iter) {
+      for (int q = 0; q < n; q++) {
+        if (!active[q]) continue;
+        const hipError_t e = poll_scalars(W[q], &host_out[q], st[q]);
+        if (e != hipSuccess) return e;
+        if (host_out[q].done) { active[q] = 0; left--; }
+      }
+    }
+  }
+  for (int q = 0; q < n; q++) {
+    const hipError_t e = poll_scalars(W[q], &host_out[q], st[q]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
+                    LsqrScalars* host_out) {
+  return lsqr_run_many(1, &S, &W, max_iter, atol, poll, &st, host_out);
 }
 
 // ---- system assembly (read_frechet_data2d + scaling, :280-486)
