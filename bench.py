@@ -25,6 +25,12 @@ arrival files), and a two-iteration inversion (automatic damping as in the
 reference's example scripts, with the initial and final forwards and every
 output file).
 
+Fourth figure (``gradient``): seconds for the misfit 1/2 |t - t_obs|^2 and
+its gradient with respect to the velocity grid and the Moho depths through
+fmt2d_amd.autograd (forward, rays, Frechet rows, adjoint product) at
+2001x301, 60 OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x
+3 km cells plus 30 ms seeded noise, starting model without it).
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -272,6 +278,60 @@ def programs_figure(cpu: bool):
     return out
 
 
+def gradient_figure(steps: int):
+    """Misfit and gradient at the adjoint-gradient workload (see module
+    docstring); the forward with rays and the backward timed apart."""
+    import torch
+    from fmt2d_amd import formats, hip, synth
+    from fmt2d_amd.autograd import TravelTimes
+    m_true, m0, g = synth.gradient_case()
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_fmt2d_case(d, m_true, g)
+        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+        sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+        mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+        geom = formats.read_geom(os.path.join(d, "geom.dat"))
+        ctx = hip.Context(0)
+        ctx.set_model(sm, sx, sz, mx, mz)
+        ctx.forward(list(geom.obs_x))
+        t, v = ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+        noise = np.random.default_rng(4).normal(0.0, 0.03, size=len(t))
+        t_obs = torch.tensor(np.where(v, t + noise, np.nan), dtype=torch.float64, device="cuda:0")
+        synth.write_fmt2d_case(d, m0, g)
+        sm0 = formats.read_smesh(os.path.join(d, "model.smesh"))
+        ctx.set_model(sm0, sx, sz, mx, mz)
+        tt = TravelTimes(ctx, geom.obs_x, geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z,
+                         invert_velocity=True, ifaces=[4])
+        vel = torch.tensor(ctx.model(0), dtype=torch.float64, device="cuda:0", requires_grad=True)
+        moho = torch.tensor(ctx.model(4), dtype=torch.float64, device="cuda:0", requires_grad=True)
+        fw, bw = [], []
+        for _ in range(steps):
+            vel.grad = None
+            moho.grad = None
+            t0 = time.perf_counter()
+            tc = tt(vel, moho)
+            ok = torch.isfinite(tc) & torch.isfinite(t_obs)
+            r = torch.where(ok, tc - torch.nan_to_num(t_obs), torch.zeros_like(tc))
+            misfit = 0.5 * torch.sum(r * r)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            misfit.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+        n_ok = int(ok.sum())
+        gn = float(torch.linalg.vector_norm(vel.grad))
+        ctx.close()
+    return {"metric": "misfit + gradient seconds", "value": round(float(np.mean(fw) + np.mean(bw)), 4),
+            "unit": "s", "higher_is_better": False,
+            "workload": "2001x301 grid (0.1 km), 60 OBS, 1000 shots, Pg at every offset and PmP at 3-30 km; "
+                        "gradient with respect to the velocity grid and the Moho depths",
+            "forward_rays_s": round(float(np.mean(fw)), 4), "backward_s": round(float(np.mean(bw)), 4),
+            "rays": len(geom.rec_x), "rays_with_picks_and_times": n_ok, "n_par": tt.n_par,
+            "misfit": float(misfit.detach()), "grad_norm_velocity": gn}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +341,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
     ap.add_argument("--iteration-only", action="store_true",
                     help="only the inversion-iteration figure (for profiling its kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -334,6 +395,7 @@ def main():
 
     it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
     progs = None if args.no_programs else programs_figure(args.cpu_sample > 0)
+    grad = None if args.no_gradient else gradient_figure(max(1, min(args.steps, 2)))
     wall = float(np.sum(walls))
     stages = 5 * len(obs)
     value = stages * args.steps / wall
@@ -372,6 +434,7 @@ def main():
         "cpu_baseline": cpu,
         "iteration": it,
         "programs": progs,
+        "gradient": grad,
     }
     print(json.dumps(out))
 

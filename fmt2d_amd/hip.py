@@ -7,7 +7,9 @@ for another architecture.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
+import sys
 from typing import Optional, Sequence
 
 import numpy as np
@@ -85,6 +87,12 @@ def load() -> C.CDLL:
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise Fmt2dError(f"{LIB_PATH} is missing: build it with `python -m fmt2d_amd.build`")
+    # One HIP runtime per process: torch's ROCm build carries its own
+    # libamdhip64 under the same soname, and whichever is loaded first serves
+    # both; torch does not initialise on another build's runtime, while this
+    # library runs on torch's.  So torch, when installed, is loaded first.
+    if "torch" not in sys.modules and importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     lib.fmt2d_last_error.restype = C.c_char_p
     lib.fmt2d_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]

@@ -202,6 +202,25 @@ def multistage_case(dx: float = 0.1, nobs: int = 60):
     return m, g
 
 
+def gradient_case(dx: float = 0.1, nobs: int = 60):
+    """The adjoint-gradient workload: 0.1 km grid (2001x301), ``nobs`` OBS
+    every 3.3 km, 1000 shots every 0.2 km at 10 m depth, Pg at every offset
+    and PmP at 3-30 km; the true model carries a +-5 % checkerboard of
+    10 km x 3 km cells, the starting model none.
+    Returns (true model, starting model, geometry without picks)."""
+    m_true = make_model(dx=dx, checker_lx=10.0, checker_lz=3.0)
+    m0 = make_model(dx=dx, checker_amp=0.0)
+    obs = np.round(2.0 + 3.3 * np.arange(nobs), 6)
+    shots = np.round(0.1 + 0.2 * np.arange(1000), 6)
+    recs = []
+    for ox in obs:
+        lst = [(float(sx), 0.010, 1, 0.0, 0.03) for sx in shots]
+        lst += [(float(sx), 0.010, 3, 0.0, 0.03) for sx in shots if 3.0 <= abs(sx - ox) <= 30.0]
+        recs.append(lst)
+    g = Geometry(obs_x=obs, obs_z=np.interp(obs, m_true.xpos, m_true.topo), receivers=recs)
+    return m_true, m0, g
+
+
 def phase_windows_geometry(model: Model, obs_x: Sequence[float], shot_x: Sequence[float],
                            shot_z: float = 0.010) -> Geometry:
     """Receiver lines for all five phases with the reference's windows:
