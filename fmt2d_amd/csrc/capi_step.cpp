@@ -64,11 +64,12 @@ int rebuild(fmt2d_ctx* ctx) {
 }
 
 // restore_model_state (:7242-7276); the region velocities follow the grid,
-// so the geometry is rebuilt in every case
-int restore_state(fmt2d_ctx* ctx, const ModelState& s) {
+// so the geometry is rebuilt, unless a model step follows at once: the step
+// reads only the grid velocities and interface depths, and rebuilds itself.
+int restore_state(fmt2d_ctx* ctx, const ModelState& s, bool step_follows = false) {
   ctx->m.vgrid = s.vel;
   for (size_t q = 0; q < ctx->inv_ifaces.size(); q++) ctx->m.if_z[ctx->inv_ifaces[q]] = s.ifz[q];
-  return rebuild(ctx);
+  return step_follows ? FMT2D_OK : rebuild(ctx);
 }
 
 // apply_model_step (:7173-7240) on the host model
@@ -79,8 +80,8 @@ int model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm)
   const int nparm = ctx->n_par;
   if ((int)mvs.size() != nparm) return fail(FMT2D_ESTATE, "no system built for this parameterisation");
   if (step > 0.0) {
-    std::vector<int> inv((size_t)G.vnx * G.vnz);
-    HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
+    const std::vector<int>& inv = ctx->h_inv;
+    if (inv.size() != (size_t)G.vnx * G.vnz) return fail(FMT2D_ESTATE, "no parameterisation set up");
     const std::vector<double>& vi = ctx->v_init.size() == m.vgrid.size() ? ctx->v_init : m.vgrid;
     for (size_t q = 0; q < inv.size(); q++) {
       const int idx = inv[q];
@@ -328,7 +329,7 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   int idx_ok = 1, forwards = 0;
   for (int li = p->first_step; li <= 6; li++) {
     const double s = LADDER[li];
-    int rc = restore_state(ctx, saved);
+    int rc = restore_state(ctx, saved, true);
     if (!rc) rc = model_step(ctx, s, s * p->iface_step_ratio, dm);
     if (!rc) rc = fmt2d_forward(ctx, (int)src_x.size(), src_x.data(), 0);
     if (!rc) rc = fmt2d_trace(ctx, nray, js.data(), jp.data(), jx.data(), jz.data(), tc.data(), valid.data());
@@ -349,7 +350,7 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   }
   int next = p->first_step;
   if (step_len > 0.0) next = idx_ok;
-  int rc = restore_state(ctx, saved);
+  int rc = restore_state(ctx, saved, step_len > 0.0);
   if (!rc && step_len > 0.0) rc = model_step(ctx, step_len, step_len * p->iface_step_ratio, dm);
   if (rc) return rc;
   const double step_if = ctx->inv_ifaces.empty() ? 0.0 : step_len * p->iface_step_ratio;
