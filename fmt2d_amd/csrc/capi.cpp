@@ -275,10 +275,12 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     d.isGZ = ctx->isGZ.p + (size_t)f * nx;
     d.fixed_key = -1;
     if (TF_PREV[itf] == 0) {
-      // point source on the seafloor (fmt2d_main.f90:545-546), cell corners
+      // point source on the seafloor of the set-up (fmt2d_main.f90:545-546,
+      // invert2d_main.f90:549-550: a later seafloor update leaves it), cell
+      // corners
       d.kind = SEED_POINT;
       d.sx = src_x[s];
-      d.sz = m.iface_z_at(2, d.sx);
+      d.sz = ctx->src_z[s];
       d.t0 = 0.0;
       int ix0 = (int)floor((d.sx - m.x0) / m.dx), iz0 = (int)floor((d.sz - m.z0) / m.dz);
       ix0 = std::max(0, std::min(ix0, nx - 2));

@@ -50,7 +50,8 @@ $FC $FFLAGS -o "$OUT/invert2d" mod_fmt2d.o propagate2d.o rays2d.o read_tomo2d.o 
 # checker build with the ray / Frechet-row dump (oracle/raydump.inc): a temp
 # copy of inversion2d.f90 includes the dump routines before
 # compute_frechet_for_ray2d and calls them after every ray of run_forward2d,
-# at the end of run_lsqr and before the iteration's update is accumulated
+# after every stage field of run_forward2d, at the end of run_lsqr and before
+# the iteration's update is accumulated
 cp "$S/inversion2d.f90" "$TMP/inversion2d_dump.f90"
 chmod u+w "$TMP/inversion2d_dump.f90"
 cp "$HERE/raydump.inc" "$TMP/raydump.inc"
@@ -58,6 +59,9 @@ perl -0pi -e 's/\n(subroutine compute_frechet_for_ray2d\(ray\))/\ninclude "raydu
 perl -0pi -e 's/(          t_fr_acc = t_fr_acc \+ real\(t_c3 - t_c2, kind=dp\) \/ real\(t_rate, kind=dp\)\n        end if\n)/$1        call fmt2d_dump_ray(isrc, ircv, receiver(ircv)%ray(iray))\n/' "$TMP/inversion2d_dump.f90"
 perl -0pi -e 's/\n(end subroutine run_lsqr)/\n  call fmt2d_dump_lsqr(ndata, nparm, nnz, n_vel_parms, smooth_type_in, max_lsqr_iter, lsqr_iter, \&\n       wdv, wdv_iface, w_smooth, w_smooth_iface, eps_min, rnorm_out, bnorm_out, test2_out, \&\n       row_ptr, gindx, gval, residual, gtg_diag, dm_sum, dm_out, mvscale, corr_x_in, corr_z_in)\n$1/' "$TMP/inversion2d_dump.f90"
 perl -0pi -e 's/\n(      dmodel_total_sum\(1:nparm\) = dmodel_total_sum\(1:nparm\) \+ dmodel_total\(1:nparm\)\n)/\n      call fmt2d_dump_step(wd_best, wd_best_iface, avg_dmv, avg_dmd, step_len, rms_orig, rms_new, \&\n           ls_prev_idx, nparm, dmodel_total)\n$1/' "$TMP/inversion2d_dump.f90"
+perl -0pi -e 's/(        if \(associated\(tf%time_gradient\)\) then\n          tf%time_gradient\(1:2,1:reg%nnode\) = reg%time_gradient\(1:2,1:reg%nnode\)\n        end if\n)(\n        if \(ipath == 3 \.and\. itf_idx == 2\) then)/$1        call fmt2d_dump_field(isrc, itf, reg)\n$2/' "$TMP/inversion2d_dump.f90"
+nf=$(grep -c "call fmt2d_dump_field" "$TMP/inversion2d_dump.f90")
+if [ "$nf" != 1 ]; then echo "field dump patch applied at $nf sites, expected 1" >&2; exit 1; fi
 ns=$(grep -c "call fmt2d_dump_step" "$TMP/inversion2d_dump.f90")
 if [ "$ns" != 1 ]; then echo "step dump patch applied at $ns sites, expected 1" >&2; exit 1; fi
 nl=$(grep -c "call fmt2d_dump_lsqr" "$TMP/inversion2d_dump.f90")

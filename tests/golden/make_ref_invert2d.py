@@ -1,6 +1,6 @@
 """Writes tests/golden/ref_invert2d.npz: whole runs of the reference's
 invert2d (oracle/_ref/invert2d, built from the reference sources by
-oracle/build_ref.sh) on two cases of fmt2d_amd.synth.inversion_case at
+oracle/build_ref.sh) on three cases of fmt2d_amd.synth.inversion_case at
 1 km grid spacing (201 x 31 nodes):
 
   pg:    Pg only, velocity grid inverted, automatic damping, Laplacian
@@ -8,7 +8,10 @@ oracle/build_ref.sh) on two cases of fmt2d_amd.synth.inversion_case at
   joint: all five phases, velocity grid + sediment base + Moho inverted,
          the default alternating schedule (iterations 1-3 velocity only,
          4 interfaces only), Gaussian smoothing with interface smoothing,
-         4 iterations.
+         4 iterations;
+  seafloor: all five phases, velocity grid + seafloor inverted jointly,
+         hybrid smoothing, 2 iterations (sources and the in-water test of
+         receivers stay on the seafloor of the set-up while it moves).
 
 Picks are the reference's receiver times on the true model plus seeded
 noise (fmt2d_amd.synth.with_picks).  Kept per case, as bytes: the input
@@ -45,6 +48,9 @@ CASES = {
     "joint": dict(phases=True, ifaces=(3, 4),
                   numbers=[4, 0.0, 0.3, 1.0, 400, 0.05, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0],
                   tail="2 0.2 0.5 0.3\n-1\n8.0\n"),
+    "seafloor": dict(phases=True, ifaces=(2,),
+                     numbers=[2, 0.0, 0.3, 1.0, 400, 0.05, 0.1, 0.3, 1.0, 1.0, 1.0, 1.0, 10.0, 3.0],
+                     tail="3 0.05 0.5 0.3\n0\n8.0\n"),
 }
 INPUTS = ["invert2d.in", "frechet.in", "model.smesh", "moho.dat", "sed.dat", "geom.dat"]
 

@@ -10,7 +10,9 @@ damping and Laplacian smoothing; all five phases with the velocity grid,
 sediment base and Moho inverted under the default alternating schedule
 (three velocity iterations, then an interface iteration whose line search
 fails and reverts), Gaussian smoothing with interface smoothing, and the
-best model restored at the end."""
+best model restored at the end; all five phases with the velocity grid and
+the seafloor inverted jointly (sources and the in-water test of receivers
+stay on the seafloor of the set-up)."""
 import os
 import subprocess
 
@@ -32,7 +34,7 @@ def first_difference(a: bytes, b: bytes) -> str:
     return f"line counts {len(la)} vs {len(lb)}"
 
 
-@pytest.mark.parametrize("case", ["pg", "joint"])
+@pytest.mark.parametrize("case", ["pg", "joint", "seafloor"])
 def test_invert2d_files_match_reference(tmp_path, case):
     if not os.path.exists(EXE):
         pytest.fail("invert2d_hip not built (python -m fmt2d_amd.build)")
