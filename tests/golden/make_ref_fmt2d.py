@@ -10,7 +10,9 @@ and 145 km:
   phases: shots every 1 km with explicit phases 1..5 in the reference's
           windows plus the buried receivers with Pg, PsedP and PmP;
   rays:   as auto with shots every 20 km, ray files written
-          (write_rays = T).
+          (write_rays = T);
+  fourier: explicit phases 1..5 every 2 km on the model whose sediment base
+          and Moho carry seeded random Fourier modes (fourier_seed 3).
 
 Kept per case, as bytes: the input files and every file the run wrote
 (arrivals, observed times, entry points, ray files).  Run from the
@@ -42,6 +44,11 @@ def buried(m, ray_types):
 
 
 def case_geometry(name, m):
+    if name == "fourier":
+        g = synth.phase_windows_geometry(m, OBS, np.arange(1.0, 200.0, 2.0))
+        for lst in g.receivers:
+            lst.extend(buried(m, (1, 2, 3)))
+        return g
     if name in ("auto", "rays"):
         shots = np.arange(1.0, 200.0, 2.0) if name == "auto" else np.arange(2.5, 200.0, 20.0)
         g = synth.make_geometry(OBS, shots, m, ray_types=(0,))
@@ -59,9 +66,10 @@ def main():
     exe = os.path.join(ROOT, "oracle", "_ref", "fmt2d")
     if not os.path.exists(exe):
         raise SystemExit("build the reference first: bash oracle/build_ref.sh")
-    m = synth.make_model(dx=DX)
+    models = {"fourier": synth.make_model(dx=DX, fourier_seed=3)}
     out = {}
-    for name, rays in (("auto", False), ("phases", False), ("rays", True)):
+    for name, rays in (("auto", False), ("phases", False), ("rays", True), ("fourier", False)):
+        m = models.get(name) or synth.make_model(dx=DX)
         g = case_geometry(name, m)
         with tempfile.TemporaryDirectory() as d:
             synth.write_fmt2d_case(d, m, g, write_rays=rays)

@@ -7,7 +7,9 @@ same input files in, every file the reference wrote compared byte for byte
 Cases on the checkerboard model at 1 km spacing, three OBS: receivers at
 10 m depth (in the water: entry search, direct water wave for Pg) and 0.3 km
 below the seafloor; phases assigned by the program from the offset, given
-explicitly (1..5), and a sparse line with the ray files written."""
+explicitly (1..5), and a sparse line with the ray files written; explicit
+phases on a model whose sediment base and Moho carry random Fourier
+modes."""
 import os
 import subprocess
 
@@ -29,7 +31,7 @@ def first_difference(a: bytes, b: bytes) -> str:
     return f"line counts {len(la)} vs {len(lb)}"
 
 
-@pytest.mark.parametrize("case", ["auto", "phases", "rays"])
+@pytest.mark.parametrize("case", ["auto", "phases", "rays", "fourier"])
 def test_fmt2d_files_match_reference(tmp_path, case):
     if not os.path.exists(EXE):
         pytest.fail("fmt2d_hip not built (python -m fmt2d_amd.build)")
