@@ -67,6 +67,7 @@ int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, cons
   // are taken again at the next set-up
   ctx->v_init.clear();
   ctx->topo_init.clear();
+  ctx->mesh_ok = false;
   std::string e = build_model(ctx->m, nx_s, nz_s, xpos, topo, zrel, vel, vwater, nsed, sed_x, sed_z, nmoho, moho_x,
                               moho_z);
   if (!e.empty()) return fail(FMT2D_EINVAL, e);

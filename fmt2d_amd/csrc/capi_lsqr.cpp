@@ -86,8 +86,8 @@ RowsCsr iface_gauss_rows(const fmt2d_ctx* ctx, double corr, const std::vector<do
   const int nparm = (int)mvs.size();
   std::vector<std::vector<std::pair<int, double>>> rows(nparm);
   for (int k : ctx->inv_ifaces) {
-    const int start = ctx->hgeom.if_start[k], nxi = ctx->hgeom.if_nnode[k];
-    const std::vector<double>& x = ctx->m.if_x[k];
+    const int start = ctx->mesh.if_start[k], nxi = ctx->mesh.if_nnode[k];
+    const std::vector<double>& x = ctx->mesh.ifx[k];
     const double dx = x[1] - x[0];
     const int kmin = -(int)(3.0 * corr / dx) - 1, kmax = (int)(3.0 * corr / dx) + 1;
     for (int ix = 1; ix <= nxi; ix++) {
@@ -137,14 +137,15 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
   if (ctx->nray < 1) return fail(FMT2D_ESTATE, "no rays traced");
   if (p->alt_mode < 0 || p->alt_mode > 2) return fail(FMT2D_EINVAL, "alt_mode must be 0, 1 or 2");
   HIPCHK(hipSetDevice(ctx->device));
-  const int rc0 = frechet_build(ctx);
+  const int rc0 = system_rows_build(ctx);
   if (rc0) return rc0;
   hipStream_t st = ctx->stream;
   const HostModel& m = ctx->m;
-  const int nray = ctx->nray, nparm = ctx->n_par, n_vel = ctx->n_vel_par;
+  const InvMesh& M = ctx->mesh;
+  const int nray = ctx->nray, nparm = M.n_par, n_vel = M.n_vel;
   // rows kept: valid ray, observed pick, |obs - calc| <= crit_chi
   std::vector<int> rp_all((size_t)nray + 1);
-  HIPCHK(hipMemcpy(rp_all.data(), ctx->g_rp.p, sizeof(int) * (nray + 1), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rp_all.data(), ctx->gp_rp.p, sizeof(int) * (nray + 1), hipMemcpyDeviceToHost));
   std::vector<int> keep, rp(1, 0);
   std::vector<double> wt, res;
   for (int r = 0; r < nray; r++) {
@@ -173,16 +174,19 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
     ctx->sys_crit_chi = p->crit_chi;
     ctx->sys_alt_mode = p->alt_mode;
   }
-  // parameter scaling: 1/max(v_init,1) on the velocity grid, refl_weight on
-  // interface nodes, zero for the parameters the alternating mode fixes
+  // parameter scaling: 1/max(v_init,1) at the velocity mesh nodes,
+  // refl_weight on interface nodes, zero for the parameters the alternating
+  // mode fixes
   std::vector<double> mvs(nparm, 1.0);
   {
-    std::vector<int> inv((size_t)ctx->hgeom.vnx * ctx->hgeom.vnz);
-    HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
     if (ctx->v_init.size() != m.vgrid.size()) ctx->v_init = m.vgrid;
     const std::vector<double>& vi = ctx->v_init;
-    for (size_t q = 0; q < inv.size(); q++)
-      if (inv[q] > 0) mvs[inv[q] - 1] = 1.0 / std::max(vi[q], 1.0);
+    const size_t fnx = (size_t)ctx->hgeom.vnx, sv = (size_t)M.vstride;
+    for (int b = 0; b < M.vnz; b++)
+      for (int a = 0; a < M.vnx; a++) {
+        const int k = M.inv[(size_t)b * M.vnx + a];
+        if (k > 0) mvs[k - 1] = 1.0 / std::max(vi[b * sv * fnx + a * sv], 1.0);
+      }
     for (int j = n_vel; j < nparm; j++) mvs[j] = p->refl_weight;
     if (p->alt_mode == 1) for (int j = n_vel; j < nparm; j++) mvs[j] = 0.0;
     // the smoothing operators carry the velocity scaling even when the
@@ -201,7 +205,7 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
   HIPCHK(upload(ctx->s_mvs, mvs, st));
   HIPCHK(ctx->s_col.alloc((size_t)std::max(nz_, 1)));
   HIPCHK(ctx->s_val.alloc((size_t)std::max(nz_, 1)));
-  HIPCHK(launch_scale_rows(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, dkeep.p, dwt.p, ctx->s_rp.p, nd, 0, ctx->s_mvs.p,
+  HIPCHK(launch_scale_rows(ctx->gp_rp.p, ctx->gp_col.p, ctx->gp_val.p, dkeep.p, dwt.p, ctx->s_rp.p, nd, 0, ctx->s_mvs.p,
                            nparm, ctx->s_col.p, ctx->s_val.p, st));
   HIPCHK(ctx->s_cp.alloc((size_t)nparm + 1));
   HIPCHK(ctx->s_crow.alloc((size_t)std::max(nz_, 1)));
@@ -228,7 +232,7 @@ int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual) {
   if (!ctx) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
   HIPCHK(hipSetDevice(ctx->device));
-  if (gtg_diag) memcpy(gtg_diag, ctx->h_gtg.data(), sizeof(double) * ctx->n_par);
+  if (gtg_diag) memcpy(gtg_diag, ctx->h_gtg.data(), sizeof(double) * ctx->mesh.n_par);
   if (residual && ctx->sys_ndata > 0)
     HIPCHK(hipMemcpy(residual, ctx->s_res.p, sizeof(double) * ctx->sys_ndata, hipMemcpyDeviceToHost));
   return FMT2D_OK;
@@ -255,7 +259,8 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
     return fail(FMT2D_EINVAL, "Gaussian smoothing needs corr_x > 0 and corr_z > 0");
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int nparm = ctx->n_par, n_vel = ctx->n_vel_par, nd = ctx->sys_ndata;
+  const InvMesh& M = ctx->mesh;
+  const int nparm = M.n_par, n_vel = M.n_vel, nd = ctx->sys_ndata;
   const double cut = ctx->sys_eps_min * 0.01;
   std::vector<uint8_t> mask(nparm, 0);
   std::vector<double> ws(nparm);
@@ -268,9 +273,8 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
   std::vector<int4> lap(std::max(n_vel, 1));
   std::vector<int2> ifn(std::max(nparm - n_vel, 1), make_int2(0, 0));
   {
-    const int vnx = ctx->hgeom.vnx, vnz = ctx->hgeom.vnz;
-    std::vector<int> inv((size_t)vnx * vnz);
-    HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
+    const int vnx = M.vnx, vnz = M.vnz;
+    const std::vector<int>& inv = M.inv;
     auto at = [&](int ix, int iz) { return inv[(size_t)iz * vnx + ix]; };
     for (int iz = 0; iz < vnz; iz++)
       for (int ix = 0; ix < vnx; ix++) {
@@ -284,7 +288,7 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
         lap[idx - 1] = nb;
       }
     for (int k : ctx->inv_ifaces) {
-      const int start = ctx->hgeom.if_start[k], nxi = ctx->hgeom.if_nnode[k];
+      const int start = M.if_start[k], nxi = M.if_nnode[k];
       for (int ix = 1; ix <= nxi; ix++) {
         const int idx = start + ix;  // 1-based
         if (idx < 1 || idx > nparm || idx <= n_vel) continue;
@@ -302,12 +306,8 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
   const bool want_g = p->smooth_type == 2 && p->w_smooth > 0.0;
   const bool want_i = (p->smooth_type == 2 || p->smooth_type == 3) && p->w_smooth_iface > 0.0 && p->corr_x_iface > 0.0;
   if (want_g && !(ctx->gs_nparm == nparm && ctx->gs_key[0] == p->corr_x && ctx->gs_key[1] == p->corr_z)) {
-    std::vector<int> inv((size_t)ctx->hgeom.vnx * ctx->hgeom.vnz);
-    HIPCHK(hipMemcpy(inv.data(), ctx->rinv.p, sizeof(int) * inv.size(), hipMemcpyDeviceToHost));
-    std::vector<double> vz(ctx->hgeom.vnz);
-    for (int j = 0; j < ctx->hgeom.vnz; j++) vz[j] = (double)j * ctx->m.dz;
     RowsCsr rh, rv;
-    gauss_rows(ctx->m.vx, vz, inv, ctx->hgeom.vnx, ctx->hgeom.vnz, p->corr_x, p->corr_z, ctx->h_mvs_smooth, rh, rv);
+    gauss_rows(M.vx, M.vz, M.inv, M.vnx, M.vnz, p->corr_x, p->corr_z, ctx->h_mvs_smooth, rh, rv);
     std::vector<int> cp, row;
     std::vector<double> cv;
     HIPCHK(upload(ctx->gh_rp, rh.rp, st)); HIPCHK(upload(ctx->gh_col, rh.col, st)); HIPCHK(upload(ctx->gh_val, rh.val, st));
@@ -363,7 +363,7 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
 
 // Work buffers of solve slot q (0 or 1) for na augmented rows.
 static int lsqr_work(fmt2d_ctx* ctx, int q, size_t na, LsqrWork& W) {
-  const int nparm = ctx->n_par, nd = ctx->sys_ndata;
+  const int nparm = ctx->mesh.n_par, nd = ctx->sys_ndata;
   DBuf<double>* b[8] = {&ctx->l_u, &ctx->l_v, &ctx->l_w, &ctx->l_dm, &ctx->l_gv, &ctx->l_gtu, &ctx->l_sm, &ctx->l_sm2};
   DBuf<double>* b2[8] = {&ctx->l2_u, &ctx->l2_v, &ctx->l2_w, &ctx->l2_dm, &ctx->l2_gv, &ctx->l2_gtu, &ctx->l2_sm,
                          &ctx->l2_sm2};
@@ -390,7 +390,7 @@ int lsqr_solve_many(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm
   size_t na = 0;
   int rc = lsqr_prepare(ctx, p, dm_sum, S0, na);
   if (rc) return rc;
-  const int nparm = ctx->n_par, n_vel = ctx->n_vel_par;
+  const int nparm = ctx->mesh.n_par, n_vel = ctx->mesh.n_vel;
   if (n == 2 && !ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
   LsqrSys S[2];
   LsqrWork W[2];

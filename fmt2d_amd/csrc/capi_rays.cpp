@@ -20,6 +20,8 @@ const int PATH_NSEC[6] = {0, 1, 2, 2, 4, 4};
 // the resampled velocity grid's active nodes (below the seafloor,
 // invert2d_main.f90:346-363 and :461-475), numbered iz-outer / ix-inner, then
 // each inverted interface's input nodes as one block.
+int build_mesh(fmt2d_ctx* ctx);
+
 int build_ray_geom(fmt2d_ctx* ctx) {
   HostModel& m = ctx->m;
   hipStream_t st = ctx->stream;
@@ -95,7 +97,94 @@ int build_ray_geom(fmt2d_ctx* ctx) {
   std::vector<RayGeomDev> one(1, G);
   HIPCHK(upload(ctx->rgeom, one, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (!ctx->mesh_ok) {
+    const int rc = build_mesh(ctx);
+    if (rc) return rc;
+  }
   ctx->rgeom_ok = true;
+  return FMT2D_OK;
+}
+
+// The inversion mesh and its prolongation P onto the Frechet columns.  The
+// numbering rules are those of the grid parameters: velocity nodes that are
+// parameters at set-up, iz-outer / ix-inner, then each inverted interface's
+// nodes as one block.  At strides 1 the mesh is the grid and P = I.
+int build_mesh(fmt2d_ctx* ctx) {
+  InvMesh& M = ctx->mesh;
+  const RayGeomDev& G = ctx->hgeom;
+  const HostModel& m = ctx->m;
+  const int sv = M.vstride, si = M.istride, fnx = G.vnx, fnz = G.vnz;
+  if ((fnx - 1) % sv != 0 || (fnz - 1) % sv != 0)
+    return fail(FMT2D_EINVAL, "velocity mesh stride " + std::to_string(sv) + " does not divide the grid's " +
+                                  std::to_string(fnx - 1) + " x " + std::to_string(fnz - 1) + " cells");
+  M.vnx = (fnx - 1) / sv + 1;
+  M.vnz = (fnz - 1) / sv + 1;
+  M.vx.resize(M.vnx);
+  M.vz.resize(M.vnz);
+  for (int a = 0; a < M.vnx; a++) M.vx[a] = m.vx[(size_t)a * sv];
+  for (int b = 0; b < M.vnz; b++) M.vz[b] = (double)(b * sv) * m.dz;
+  const std::vector<int>& finv = ctx->h_inv;
+  M.inv.assign((size_t)M.vnx * M.vnz, 0);
+  int np = 0;
+  for (int b = 0; b < M.vnz; b++)
+    for (int a = 0; a < M.vnx; a++)
+      if (finv[(size_t)b * sv * fnx + (size_t)a * sv] > 0) M.inv[(size_t)b * M.vnx + a] = ++np;
+  M.n_vel = np;
+  int group = ctx->inv_vel ? 1 : 0, last = np;
+  for (int k = 0; k <= NIF; k++) { M.if_start[k] = -1; M.if_nnode[k] = 0; M.ifx[k].clear(); }
+  for (int k : ctx->inv_ifaces) {
+    const int nn = G.if_nnode[k];
+    if ((nn - 1) % si != 0)
+      return fail(FMT2D_EINVAL, "interface mesh stride " + std::to_string(si) + " does not divide interface " +
+                                    std::to_string(k) + "'s " + std::to_string(nn - 1) + " node spacings");
+    group++;
+    const int start = group == 1 ? 0 : last;
+    const int nc = (nn - 1) / si + 1;
+    M.ifx[k].resize(nc);
+    for (int c = 0; c < nc; c++) M.ifx[k][c] = m.if_x[k][(size_t)c * si];
+    M.if_start[k] = start;
+    M.if_nnode[k] = nc;
+    last = start + nc;
+    np += nc;
+  }
+  M.n_par = np;
+  // P, one row per Frechet column in column order
+  M.p_rp.assign(1, 0);
+  M.p_col.clear();
+  M.p_w.clear();
+  auto put = [&](int k, double w) {
+    if (w != 0.0) { M.p_col.push_back(k); M.p_w.push_back(w); }
+  };
+  for (int iz = 0; iz < fnz; iz++)
+    for (int ix = 0; ix < fnx; ix++) {
+      if (finv[(size_t)iz * fnx + ix] <= 0) continue;
+      const int a = ix / sv, b = iz / sv;
+      const double wx = (double)(ix % sv) / (double)sv, wz = (double)(iz % sv) / (double)sv;
+      const int a1 = std::min(a + 1, M.vnx - 1), b1 = std::min(b + 1, M.vnz - 1);
+      auto par = [&](int aa, int bb) { return M.inv[(size_t)bb * M.vnx + aa] - 1; };
+      if (par(a, b) >= 0) put(par(a, b), (1.0 - wx) * (1.0 - wz));
+      if (wx != 0.0 && par(a1, b) >= 0) put(par(a1, b), wx * (1.0 - wz));
+      if (wz != 0.0 && par(a, b1) >= 0) put(par(a, b1), (1.0 - wx) * wz);
+      if (wx != 0.0 && wz != 0.0 && par(a1, b1) >= 0) put(par(a1, b1), wx * wz);
+      M.p_rp.push_back((int)M.p_col.size());
+    }
+  for (int k : ctx->inv_ifaces) {
+    for (int ix = 0; ix < G.if_nnode[k]; ix++) {
+      const int c = ix / si;
+      const double w = (double)(ix % si) / (double)si;
+      put(M.if_start[k] + c, 1.0 - w);
+      if (w != 0.0) put(M.if_start[k] + c + 1, w);
+      M.p_rp.push_back((int)M.p_col.size());
+    }
+  }
+  if ((int)M.p_rp.size() != ctx->n_par + 1) return fail(FMT2D_EINVAL, "prolongation does not cover the Frechet columns");
+  hipStream_t st = ctx->stream;
+  HIPCHK(upload(ctx->p_rp, M.p_rp, st));
+  HIPCHK(upload(ctx->p_col, M.p_col, st));
+  HIPCHK(upload(ctx->p_w, M.p_w, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->gp_nnz = -1;
+  ctx->mesh_ok = true;
   return FMT2D_OK;
 }
 
@@ -140,6 +229,53 @@ int frechet_build(fmt2d_ctx* ctx) {
   ctx->g_val.p = co.val; ctx->g_val.n = (size_t)std::max(co.nnz, 1);
   ctx->g_nnz = co.nnz;
   ctx->gt_ok = false;
+  ctx->gp_nnz = -1;
+  return FMT2D_OK;
+}
+
+// G P on the device: every Frechet entry times its column's P row, written
+// in row order (entries in stored order, P rows in order) and summed per
+// (row, mesh parameter) front to back by the same stable CSR assembly.  At
+// strides 1 each entry is multiplied by 1.0 alone, so G P = G bit for bit.
+int system_rows_build(fmt2d_ctx* ctx) {
+  const int rc0 = frechet_build(ctx);
+  if (rc0) return rc0;
+  if (ctx->gp_nnz >= 0) return FMT2D_OK;
+  if (!ctx->mesh_ok)
+    return fail(FMT2D_ESTATE, "the parameterisation changed after the last trace: run a forward and trace again");
+  hipStream_t st = ctx->stream;
+  const int nray = ctx->nray;
+  DBuf<long long> cnt, coff;
+  HIPCHK(cnt.alloc(nray));
+  HIPCHK(launch_prolong_rows(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, ctx->p_rp.p, ctx->p_col.p, ctx->p_w.p,
+                             nullptr, cnt.p, nullptr, nullptr, st));
+  std::vector<long long> hc(nray), ho(nray);
+  HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  long long total = 0;
+  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
+  HIPCHK(upload(coff, ho, st));
+  DBuf<unsigned long long> keys;
+  DBuf<double> vals;
+  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
+  HIPCHK(launch_prolong_rows(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, ctx->p_rp.p, ctx->p_col.p, ctx->p_w.p,
+                             coff.p, nullptr, keys.p, vals.p, st));
+  int rbits = 1;
+  while ((1LL << rbits) < nray) rbits++;
+  CsrOut co;
+  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, 0, co, st);
+  if (e != hipSuccess) {
+    if (co.rowptr) (void)hipFree(co.rowptr);
+    if (co.col) (void)hipFree(co.col);
+    if (co.val) (void)hipFree(co.val);
+    return fail(FMT2D_EHIP, std::string("G P assembly: ") + hipGetErrorString(e));
+  }
+  ctx->gp_rp.release(); ctx->gp_col.release(); ctx->gp_val.release();
+  ctx->gp_rp.p = co.rowptr; ctx->gp_rp.n = (size_t)nray + 1;
+  ctx->gp_col.p = co.col; ctx->gp_col.n = (size_t)std::max(co.nnz, 1);
+  ctx->gp_val.p = co.val; ctx->gp_val.n = (size_t)std::max(co.nnz, 1);
+  ctx->gp_nnz = co.nnz;
   return FMT2D_OK;
 }
 
@@ -200,6 +336,9 @@ int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const 
   ctx->inv_vel = invert_velocity ? 1 : 0;
   ctx->inv_ifaces = ids;
   ctx->rgeom_ok = false;
+  ctx->mesh.vstride = 1;
+  ctx->mesh.istride = 1;
+  ctx->mesh_ok = false;
   // velocity_init of the inversion: parameter scaling uses the model of the
   // set-up, not the updated one (inversion2d.f90:186-191, 451-463)
   ctx->v_init = ctx->m.vgrid;
@@ -209,13 +348,47 @@ int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const 
   return FMT2D_OK;
 }
 
+int fmt2d_set_inversion_mesh(fmt2d_ctx* ctx, int vel_stride, int iface_stride) {
+  if (!ctx) return fail(FMT2D_EINVAL, "NULL argument");
+  if (vel_stride < 1 || iface_stride < 1) return fail(FMT2D_EINVAL, "mesh strides must be >= 1");
+  if (ctx->have_model) {
+    const HostModel& m = ctx->m;
+    if ((m.vx.size() - 1) % vel_stride != 0 || (m.nz - 1) % vel_stride != 0)
+      return fail(FMT2D_EINVAL, "velocity mesh stride " + std::to_string(vel_stride) + " does not divide the grid's " +
+                                    std::to_string(m.vx.size() - 1) + " x " + std::to_string(m.nz - 1) + " cells");
+    for (int k : ctx->inv_ifaces)
+      if ((m.if_x[k].size() - 1) % iface_stride != 0)
+        return fail(FMT2D_EINVAL, "interface mesh stride " + std::to_string(iface_stride) +
+                                      " does not divide interface " + std::to_string(k) + "'s node spacings");
+  }
+  ctx->mesh.vstride = vel_stride;
+  ctx->mesh.istride = iface_stride;
+  ctx->mesh_ok = false;
+  ctx->rgeom_ok = false;
+  ctx->gs_nparm = -1;
+  ctx->gsi_nparm = -1;
+  ctx->sys_ndata = -1;
+  return FMT2D_OK;
+}
+
 int fmt2d_inversion_info(fmt2d_ctx* ctx, int* n_vel_par, int* n_par) {
   if (!ctx || !n_vel_par || !n_par) return fail(FMT2D_EINVAL, "NULL argument");
   if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
   HIPCHK(hipSetDevice(ctx->device));
   if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
-  *n_vel_par = ctx->n_vel_par;
-  *n_par = ctx->n_par;
+  *n_vel_par = ctx->mesh.n_vel;
+  *n_par = ctx->mesh.n_par;
+  return FMT2D_OK;
+}
+
+int fmt2d_mesh_info(fmt2d_ctx* ctx, int* vnx, int* vnz, int* n_frechet) {
+  if (!ctx || !vnx || !vnz || !n_frechet) return fail(FMT2D_EINVAL, "NULL argument");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
+  *vnx = ctx->mesh.vnx;
+  *vnz = ctx->mesh.vnz;
+  *n_frechet = ctx->n_par;
   return FMT2D_OK;
 }
 
@@ -256,6 +429,7 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   ctx->hpoff[nray] = off;
   ctx->nray = nray;
   ctx->g_nnz = -1;
+  ctx->gp_nnz = -1;
   ctx->sys_ndata = -1;
   if (nray == 0) return FMT2D_OK;
   HIPCHK(upload(ctx->rjobs, ctx->hjobs, st));
@@ -277,6 +451,7 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
     ctx->h_tcalc[r] = out[r].t; ctx->h_valid[r] = out[r].valid;
   }
   ctx->g_nnz = -1;
+  ctx->gp_nnz = -1;
   ctx->sys_ndata = -1;
   return FMT2D_OK;
 }

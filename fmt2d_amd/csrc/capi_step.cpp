@@ -57,6 +57,8 @@ ModelState save_state(const fmt2d_ctx* ctx) {
   return s;
 }
 
+inline double step_scale(double s, double mvs, double dm) { return s * mvs * dm; }
+
 int rebuild(fmt2d_ctx* ctx) {
   const std::string e = rebuild_geometry(ctx->m);
   if (!e.empty()) return fail(FMT2D_EINVAL, e);
@@ -72,13 +74,29 @@ int restore_state(fmt2d_ctx* ctx, const ModelState& s, bool step_follows = false
   return step_follows ? FMT2D_OK : rebuild(ctx);
 }
 
-// apply_model_step (:7173-7240) on the host model
+// apply_model_step (:7173-7240) on the host model.  The scaled update of
+// each mesh parameter, step * mvscale * dm, reaches a grid node through its
+// row of P (weights in row order); at strides 1 that row is one weight 1.0,
+// the reference's update itself.
 int model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm) {
   HostModel& m = ctx->m;
   const RayGeomDev& G = ctx->hgeom;
+  const InvMesh& M = ctx->mesh;
   const std::vector<double>& mvs = ctx->h_mvs_model;
-  const int nparm = ctx->n_par;
+  const int nparm = M.n_par;
   if ((int)mvs.size() != nparm) return fail(FMT2D_ESTATE, "no system built for this parameterisation");
+  if ((int)M.p_rp.size() != ctx->n_par + 1) return fail(FMT2D_ESTATE, "no inversion mesh built");
+  auto prolong = [&](int col, double s) {   // Frechet column col (0-based)
+    double acc = 0.0;
+    bool first = true;
+    for (int e = M.p_rp[col]; e < M.p_rp[col + 1]; e++) {
+      const int k = M.p_col[e];
+      const double t = step_scale(s, mvs[k], dm[k]) * M.p_w[e];
+      acc = first ? t : acc + t;
+      first = false;
+    }
+    return acc;
+  };
   if (step > 0.0) {
     const std::vector<int>& inv = ctx->h_inv;
     if (inv.size() != (size_t)G.vnx * G.vnz) return fail(FMT2D_ESTATE, "no parameterisation set up");
@@ -87,7 +105,7 @@ int model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm)
       const int idx = inv[q];
       if (idx <= 0) continue;
       const double cur_s = 1.0 / m.vgrid[q];
-      const double trial_s = cur_s + step * mvs[idx - 1] * dm[idx - 1];
+      const double trial_s = cur_s + prolong(idx - 1, step);
       double tv = 2.0;
       if (trial_s > 0.0) tv = 1.0 / trial_s;
       if (tv < 2.0) tv = 2.0;
@@ -105,8 +123,8 @@ int model_step(fmt2d_ctx* ctx, double step, double step_iface, const double* dm)
       const int start = G.if_start[k], nxi = G.if_nnode[k];
       for (int ix = 1; ix <= nxi; ix++) {
         const int idx = start + ix;
-        if (idx < 1 || idx > nparm) continue;
-        double dz = step_iface * mvs[idx - 1] * dm[idx - 1];
+        if (idx < 1 || idx > ctx->n_par) continue;
+        double dz = prolong(idx - 1, step_iface);
         if (dz > 0.5) dz = 0.5;
         if (dz < -0.5) dz = -0.5;
         m.if_z[k][ix - 1] = m.if_z[k][ix - 1] + dz;
@@ -124,7 +142,7 @@ int fmt2d_damped_update(fmt2d_ctx* ctx, const fmt2d_lsqr_params* lp, const fmt2d
                         const double* dm_sum, double* dm_out, fmt2d_damping_result* res) {
   if (!ctx || !lp || !d || !dm_sum || !dm_out || !res) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
-  const int nparm = ctx->n_par, n_vel = ctx->n_vel_par, n_if = nparm - n_vel;
+  const int nparm = ctx->mesh.n_par, n_vel = ctx->mesh.n_vel, n_if = nparm - n_vel;
   const std::vector<double>& gtg = ctx->h_gtg;
   const double eps = ctx->sys_eps_min;
   const std::vector<double>& mvs = ctx->h_mvs_smooth;
@@ -311,7 +329,7 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   if (ctx->sys_ndata < 0) return fail(FMT2D_ESTATE, "no system built");
   if (p->first_step < 1 || p->first_step > 6) return fail(FMT2D_EINVAL, "first_step must be 1..6");
   HIPCHK(hipSetDevice(ctx->device));
-  const int nparm = ctx->n_par, n_vel = ctx->n_vel_par, nray = ctx->nray;
+  const int nparm = ctx->mesh.n_par, n_vel = ctx->mesh.n_vel, nray = ctx->nray;
   const double rms_orig = ctx->sys_rms;
   // what a trial forward needs: the sources and the rays of this iteration
   const std::vector<double> src_x = ctx->src_x;

@@ -56,6 +56,40 @@ __global__ void run_scatter_kernel(const unsigned long long* __restrict__ k, con
 template <typename T>
 static hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, (n ? n : 1) * sizeof(T)); }
 
+__global__ void prolong_rows_kernel(const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ val, int nrow, const int* __restrict__ p_rp,
+                                    const int* __restrict__ p_col, const double* __restrict__ p_w,
+                                    const long long* __restrict__ off, long long* cnt, unsigned long long* keys,
+                                    double* vals) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  if (!keys) {
+    long long n = 0;
+    for (int k = rp[r]; k < rp[r + 1]; k++) n += p_rp[col[k] + 1] - p_rp[col[k]];
+    cnt[r] = n;
+    return;
+  }
+  long long o = off[r];
+  const unsigned long long hi = (unsigned long long)r << 32;
+  for (int k = rp[r]; k < rp[r + 1]; k++) {
+    const int j = col[k];
+    const double g = val[k];
+    for (int e = p_rp[j]; e < p_rp[j + 1]; e++, o++) {
+      keys[o] = hi | (unsigned long long)(unsigned)p_col[e];
+      vals[o] = g * p_w[e];
+    }
+  }
+}
+
+hipError_t launch_prolong_rows(const int* rp, const int* col, const double* val, int nrow, const int* p_rp,
+                               const int* p_col, const double* p_w, const long long* off, long long* cnt,
+                               unsigned long long* keys, double* vals, hipStream_t st) {
+  if (nrow <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prolong_rows_kernel, dim3((nrow + 127) / 128), dim3(128), 0, st, rp, col, val, nrow, p_rp,
+                     p_col, p_w, off, cnt, keys, vals);
+  return hipGetLastError();
+}
+
 #define CK(x) do { hipError_t _e = (x); if (_e != hipSuccess) { err = _e; goto done; } } while (0)
 
 hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nray, int key_bits, int base,

@@ -57,6 +57,25 @@ const int LEVEL_N[4] = {4, 4, 2, 2};
 
 }  // namespace
 
+// The inversion mesh: every vstride-th grid column and row carries a
+// velocity parameter, every istride-th node of an inverted interface a depth
+// parameter (strides 1: the grid itself, the reference's parameterisation).
+// P prolongs mesh parameters to Frechet columns: bilinear in the velocity
+// cell, linear along an interface; mesh nodes that are not parameters (in
+// the water at set-up) contribute nothing.  The system matrix is G P and a
+// model step moves the grid by P dm.
+struct InvMesh {
+  int vstride = 1, istride = 1;
+  int vnx = 0, vnz = 0;                // mesh velocity nodes
+  std::vector<double> vx, vz;          // their x and depth below the grid top
+  std::vector<int> inv;                // (vnz, vnx) parameter number, 0: none
+  int n_vel = 0, n_par = 0;
+  int if_start[NIF + 1] = {}, if_nnode[NIF + 1] = {};
+  std::vector<double> ifx[NIF + 1];    // mesh interface node x
+  std::vector<int> p_rp, p_col;        // P: row per Frechet column, 0-based mesh parameters
+  std::vector<double> p_w;
+};
+
 struct fmt2d_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -160,10 +179,19 @@ struct fmt2d_ctx {
   DBuf<double> l2_u, l2_v, l2_w, l2_dm, l2_gv, l2_gtu, l2_sm, l2_sm2, l2_wd;
   DBuf<LsqrScalars> l2_sc;
   hipStream_t stream2 = nullptr;
-  // inversion parameterisation (frechet.in): velocity grid and interfaces
+  // inversion parameterisation (frechet.in): velocity grid and interfaces;
+  // n_vel_par / n_par count the Frechet columns (grid nodes, interface
+  // input nodes), mesh the parameters the system solves for
   int inv_vel = 1;
   std::vector<int> inv_ifaces;
   int n_vel_par = 0, n_par = 0;
+  InvMesh mesh;
+  bool mesh_ok = false;                // mesh and P built for the parameterisation
+  DBuf<int> p_rp, p_col;               // prolongation P by Frechet column (0-based)
+  DBuf<double> p_w;
+  DBuf<int> gp_rp, gp_col;             // G P of the last trace (0-based columns)
+  DBuf<double> gp_val;
+  int gp_nnz = -1;                     // -1: not formed for the current trace
   int* h_count = nullptr;              // pinned
   std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
   fmt2d_forward_stats stats{};
@@ -187,6 +215,8 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 
 // Frechet rows of the last trace on the device (capi_rays.cpp)
 int frechet_build(fmt2d_ctx* ctx);
+// G P: the Frechet rows over the mesh parameters (capi_rays.cpp)
+int system_rows_build(fmt2d_ctx* ctx);
 int model_upload(fmt2d_ctx* ctx);
 int lsqr_solve(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats);

@@ -82,6 +82,12 @@ module fmt2d_hip
       integer(c_int), value :: invert_velocity, n_iface
       integer(c_int), intent(in) :: iface_ids(*)
     end function
+    integer(c_int) function fmt2d_set_inversion_mesh(ctx, vel_stride, iface_stride) &
+        bind(C, name="fmt2d_set_inversion_mesh")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: vel_stride, iface_stride
+    end function
     integer(c_int) function fmt2d_inversion_info(ctx, n_vel_par, n_par) bind(C, name="fmt2d_inversion_info")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -32,6 +32,9 @@ program invert2d_hip
   real(dp) :: lam_damp_iface_in, lam_smooth_iface_in, refl_weight_in
   real(dp) :: phase_wt_pmp_in, phase_wt_pgss_in, phase_wt_psed_in, corr_x_in, corr_z_in
   real(dp) :: target_dd_in, iface_step_ratio_in, max_rel_pert_in, corr_x_iface_in, iface_dx_target_in
+  ! inversion mesh spacings (km; 0: the propagation grid), an optional last
+  ! line of this build's own
+  real(dp) :: mesh_dx_in, mesh_dx_iface_in
 
   ! model and acquisition
   integer :: nx, nz, nsed, nmoho, nobs, nrec, nray
@@ -78,6 +81,12 @@ program invert2d_hip
   if (do_inversion) then
     call fmt2d_check(fmt2d_set_inversion(ctx, merge(1_c_int, 0_c_int, inv_vel), int(n_inv_iface, c_int), &
          iface_list), 'fmt2d_set_inversion')
+    if (mesh_dx_in > 0.0_dp .or. mesh_dx_iface_in > 0.0_dp) then
+      call fmt2d_check(fmt2d_set_inversion_mesh(ctx, mesh_stride(mesh_dx_in), mesh_stride(mesh_dx_iface_in)), &
+           'fmt2d_set_inversion_mesh')
+      write(*,'(A,2I4)') '  Inversion mesh strides (velocity, interfaces): ', &
+           mesh_stride(mesh_dx_in), mesh_stride(mesh_dx_iface_in)
+    end if
     call fmt2d_check(fmt2d_inversion_info(ctx, n_vel_par, n_par), 'fmt2d_inversion_info')
     write(*,'(A,I8)') '  Inversion parameters: n_inv_parms = ', n_par
     if (n_par > 0) call invert()
@@ -90,9 +99,18 @@ program invert2d_hip
 
 contains
 
+  ! grid nodes per inversion mesh spacing (1: the grid)
+  integer(c_int) function mesh_stride(dx_mesh)
+    real(dp), intent(in) :: dx_mesh
+    mesh_stride = 1
+    if (dx_mesh > 0.0_dp) mesh_stride = max(1, nint(dx_mesh / ginfo%dx))
+  end function mesh_stride
+
   ! invert2d.in: four file names, fourteen numbers, then the optional
   ! smooth_type / target_dd / iface_step_ratio / max_rel_pert line, the
   ! alternating mode, corr_x_iface and iface_dx_target (invert2d_main.f90:60-96).
+  ! A further optional line, "mesh_dx mesh_dx_iface", sets an inversion mesh
+  ! coarser than the propagation grid (multiples of the grid spacing).
   subroutine read_invert2d_in()
     open(1, file='invert2d.in', status='old', action='read')
     read(1, '(A)') smesh_file
@@ -120,10 +138,13 @@ contains
     alt_inv_mode_in = -1
     corr_x_iface_in = corr_x_in
     iface_dx_target_in = 0.0_dp
+    mesh_dx_in = 0.0_dp
+    mesh_dx_iface_in = 0.0_dp
     read(1, *, end=10) smooth_type, target_dd_in, iface_step_ratio_in, max_rel_pert_in
     read(1, *, end=10) alt_inv_mode_in
     read(1, *, end=10) corr_x_iface_in
     read(1, *, end=10) iface_dx_target_in
+    read(1, *, end=10) mesh_dx_in, mesh_dx_iface_in
 10  continue
     close(1)
     if (iface_step_ratio_in <= 0.0_dp) iface_step_ratio_in = 0.5_dp

@@ -109,6 +109,8 @@ def load() -> C.CDLL:
     lib.fmt2d_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     lib.fmt2d_set_inversion.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
     lib.fmt2d_inversion_info.argtypes = [C.c_void_p, _ip, _ip]
+    lib.fmt2d_set_inversion_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
     lib.fmt2d_ray_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip]
@@ -136,7 +138,8 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_set_tuning",
            "fmt2d_tf_region",
            "fmt2d_get_field",
-           "fmt2d_set_inversion", "fmt2d_inversion_info", "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
+           "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_inversion_info", "fmt2d_mesh_info",
+           "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
            "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
            "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
 
@@ -218,10 +221,22 @@ class Context:
         _check(self._lib.fmt2d_set_inversion(self._h, int(bool(invert_velocity)), len(ids),
                                              ids.ctypes.data_as(_ip) if len(ids) else None))
 
+    def set_inversion_mesh(self, vel_stride: int = 1, iface_stride: int = 1):
+        """Inversion mesh: velocity parameters at every vel_stride-th grid
+        column and row, interface depths at every iface_stride-th node."""
+        _check(self._lib.fmt2d_set_inversion_mesh(self._h, int(vel_stride), int(iface_stride)))
+
     def inversion_info(self):
+        """(velocity parameters, all parameters) of the inversion mesh."""
         a, b = C.c_int(), C.c_int()
         _check(self._lib.fmt2d_inversion_info(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def mesh_info(self):
+        """(mesh velocity columns, rows, number of Frechet columns)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.fmt2d_mesh_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def trace(self, src, path, rx, rz):
         """Rays of the last forward: returns (receiver times, valid flags)."""
@@ -258,7 +273,7 @@ class Context:
         """G x (x per parameter) or G^T x (x per ray of the last trace) with the
         unscaled Frechet rows, sums in the reference's order."""
         x = np.ascontiguousarray(x, dtype=np.float64)
-        _, n_par = self.inversion_info()
+        n_par = self.mesh_info()[2]
         nray = self._nray
         if len(x) != (nray if transpose else n_par):
             raise ValueError("x has the wrong length")
@@ -341,8 +356,13 @@ class Context:
         v = np.ascontiguousarray(values, dtype=np.float64).ravel()
         _check(self._lib.fmt2d_model_put(self._h, int(what), _p(v), len(v)))
 
+    def _check_dm(self, dm):
+        if len(dm) != self.inversion_info()[1]:
+            raise ValueError("dm needs one entry per inversion parameter")
+
     def apply_model_step(self, step, step_iface, dm):
         dm = np.ascontiguousarray(dm, dtype=np.float64)
+        self._check_dm(dm)
         _check(self._lib.fmt2d_apply_model_step(self._h, float(step), float(step_iface), _p(dm)))
 
     def line_search(self, t_obs, dm, crit_chi=1.0, iface_step_ratio=0.5, first_step=1, valid=None):
@@ -353,6 +373,7 @@ class Context:
         lost."""
         t_obs = np.ascontiguousarray(t_obs, dtype=np.float64)
         dm = np.array(dm, dtype=np.float64)
+        self._check_dm(dm)
         p = LineSearchParams(crit_chi, iface_step_ratio, first_step)
         r = LineSearchResult()
         vp = None

@@ -129,10 +129,31 @@ int fmt2d_get_field(fmt2d_ctx* ctx, int isrc, int itf, double* T, double* dTdx, 
 
 /* Inversion parameterisation (frechet.in): invert the velocity grid and/or
  * the listed interfaces (2 seafloor, 3 sediment base, 4 Moho).  Default:
- * velocity only. */
+ * velocity only.  Resets the inversion mesh to the grid. */
 int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids);
-/* Number of velocity parameters and of all parameters (n_inv_parms). */
+/* Inversion mesh coarser than the propagation grid (no counterpart in the
+ * reference, whose inversion grid is the propagation grid,
+ * fmt2d_main.f90:200-205): velocity parameters at every vel_stride-th grid
+ * column and row, interface depth parameters at every iface_stride-th node
+ * of each inverted interface (the reference reads an interface spacing,
+ * iface_dx_target, invert2d_main.f90:89-93, and leaves it unused).  Mesh
+ * nodes below the set-up seafloor are parameters, numbered as the grid's.
+ * The prolongation P (bilinear over a velocity mesh cell, linear along an
+ * interface; nodes that are not parameters contribute zero) maps mesh
+ * parameters to the Frechet columns: fmt2d_build_system, the LSQR solves,
+ * fmt2d_damped_update and fmt2d_line_search work on G P, and a model step
+ * moves the grid by P dm.  Strides 1, 1 (the default) are the reference's
+ * parameterisation, with G P = G bit for bit.  The strides must divide the
+ * grid's cell counts and each inverted interface's node spacings. */
+int fmt2d_set_inversion_mesh(fmt2d_ctx* ctx, int vel_stride, int iface_stride);
+/* Number of velocity parameters and of all parameters (n_inv_parms) of the
+ * inversion mesh: the length of the model updates (dm) of
+ * fmt2d_damped_update, fmt2d_lsqr and fmt2d_line_search.  Equal to the
+ * Frechet column counts when the mesh is the grid. */
 int fmt2d_inversion_info(fmt2d_ctx* ctx, int* n_vel_par, int* n_par);
+/* Velocity nodes of the inversion mesh (vnx x vnz) and the number of
+ * Frechet columns (grid velocity parameters + interface input nodes). */
+int fmt2d_mesh_info(fmt2d_ctx* ctx, int* vnx, int* vnz, int* n_frechet);
 /* Traces nray rays on the fields of the last fmt2d_forward.  Ray r runs
  * from receiver (rx[r], rz[r]) back to source src[r] (0-based index into the
  * last forward's sources) along path path[r] (1 Pg, 2 PsedP, 3 PmP, 4 Pg_ss,
@@ -148,12 +169,12 @@ int fmt2d_ray_info(fmt2d_ctx* ctx, int iray, double* entry_xz, double* water_tim
  * only *npts is returned. */
 int fmt2d_ray_points(fmt2d_ctx* ctx, int iray, int section, double* x, double* z, int cap, int* npts);
 /* Frechet rows of the last trace as CSR (one row per ray, empty for invalid
- * rays; columns are parameter numbers).  Call with col == NULL for *nnz and
+ * rays; columns are the grid's parameter numbers, fmt2d_param_index).  Call with col == NULL for *nnz and
  * row_ptr, then with buffers of cap >= nnz.  index_base 1 gives Fortran
  * indexing for row_ptr and col. */
 int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, double* val, int cap, int* nnz);
-/* y = G x (transpose 0: x has n_par entries, y nray) or y = G^T x
- * (transpose 1: x nray, y n_par) with the Frechet rows of the last trace as
+/* y = G x (transpose 0: x has n_frechet entries, y nray) or y = G^T x
+ * (transpose 1: x nray, y n_frechet) with the Frechet rows of the last trace as
  * fmt2d_frechet_csr returns them (unscaled: d t / d slowness of a velocity
  * node, d t / d depth of an interface node); each sum in the order
  * sparse_Ax / sparse_ATx accumulate it (inversion2d.f90:2997-3042).  The
