@@ -31,6 +31,14 @@ fmt2d_amd.autograd (forward, rays, Frechet rows, adjoint product) at
 2001x301, 60 OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x
 3 km cells plus 30 ms seeded noise, starting model without it).
 
+Fifth figure (``joint``, with ``--joint N``): seconds per iteration of a
+joint velocity + interface inversion on an inversion mesh coarser than the
+propagation grid: 2001x301, 200 OBS every 1 km, shots every 0.2 km, five
+phases (434k rays); velocity parameters on a 0.5 km mesh (401x61), sediment
+base and Moho depths every 1 km (2x201); N iterations of the alternating
+schedule with hybrid smoothing, damping 0.1 and the line search.  The
+reference has no such parameterisation, so no CPU figure stands beside it.
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -332,6 +340,73 @@ def gradient_figure(steps: int):
             "misfit": float(misfit.detach()), "grad_norm_velocity": gn}
 
 
+def joint_figure(iters: int):
+    """Joint velocity + interface inversion on an inversion mesh coarser than
+    the grid (see module docstring): the iterations of the alternating
+    schedule, each phase timed."""
+    from fmt2d_amd import formats, hip, synth
+    obs_x = np.round(np.arange(0.5, 200.0, 1.0), 6)
+    m_true, m0, g = synth.inversion_case(dx=0.1, obs_x=obs_x, shot_dx=0.2, phases=True)
+    with tempfile.TemporaryDirectory() as d:
+        def load(m):
+            synth.write_fmt2d_case(d, m, g)
+            return (formats.read_smesh(os.path.join(d, "model.smesh")), *formats.read_refl(os.path.join(d, "sed.dat")),
+                    *formats.read_refl(os.path.join(d, "moho.dat")))
+        true_in = load(m_true)
+        geom = formats.read_geom(os.path.join(d, "geom.dat"))
+        start_in = load(m0)
+    rays = (geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+    ctx = hip.Context(0)
+    ctx.set_model(*true_in)
+    ctx.forward(list(geom.obs_x))
+    t_true, v_true = ctx.trace(*rays)
+    noise = np.random.default_rng(5).normal(0.0, 0.05, size=len(t_true))
+    t_obs = np.where(v_true, t_true + noise, -1.0)
+    ctx.set_model(*start_in)
+    ctx.set_inversion(True, [3, 4])
+    ctx.set_inversion_mesh(5, 10)
+    n_vel, n_par = ctx.inversion_info()
+    vnx, vnz, n_fre = ctx.mesh_info()
+    dm_sum = np.zeros(n_par)
+    first = 1
+    log, phases = [], {"forward": [], "rays": [], "system": [], "lsqr": [], "line_search": []}
+    for it in range(1, iters + 1):
+        mode = 2 if it % 4 == 0 else 1        # three velocity iterations, then one interface iteration
+        t0 = time.perf_counter()
+        ctx.forward(list(geom.obs_x))
+        t1 = time.perf_counter()
+        t, valid = ctx.trace(*rays)
+        t2 = time.perf_counter()
+        nd, nnz, _ = ctx.build_system(t_obs, crit_chi=1.0, refl_weight=10.0, alt_mode=mode)
+        t3 = time.perf_counter()
+        dm, dres = ctx.damped_update(dm_sum, lam_damp=0.1, lam_damp_iface=0.1, smooth_type=3, w_smooth=0.3,
+                                     w_smooth_iface=0.3, max_iter=400, corr_x=10.0, corr_z=3.0, corr_x_iface=8.0)
+        t4 = time.perf_counter()
+        alive = np.ascontiguousarray(valid, dtype=np.int32)
+        dm, r = ctx.line_search(t_obs, dm, crit_chi=1.0, first_step=first, valid=alive)
+        t5 = time.perf_counter()
+        first = r.next_first_step
+        dm_sum += dm
+        for k, v in zip(phases, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
+            phases[k].append(v)
+        log.append({"iteration": it, "mode": "velocity" if mode == 1 else "interfaces", "ndata": nd, "nnz": nnz,
+                    "rms": round(r.rms_orig, 6), "rms_new": round(r.rms_new, 6), "step": r.step,
+                    "lsqr_iterations": dres.iterations, "line_search_forwards": r.forwards,
+                    "seconds": round(t5 - t0, 3)})
+        print(f"joint iteration {it}: {log[-1]}", file=sys.stderr, flush=True)
+    ctx.close()
+    tot = [sum(v) for v in zip(*phases.values())]
+    return {"metric": "joint inversion seconds per iteration", "value": round(float(np.mean(tot)), 3), "unit": "s",
+            "higher_is_better": False,
+            "workload": "2001x301 grid (0.1 km), 200 OBS every 1 km, 1000 shots every 0.2 km, five phases in the "
+                        "reference's windows; velocity on a 0.5 km mesh (401x61) and sediment base + Moho every "
+                        "1 km (2x201); alternating schedule (three velocity iterations, one interface iteration), "
+                        "hybrid smoothing, damping 0.1, LSQR at most 400 iterations, line search",
+            "rays": int(len(t_true)), "picks": int((t_obs >= 0).sum()), "mesh": [vnx, vnz], "n_par": n_par,
+            "n_frechet_columns": n_fre,
+            "phase_seconds": {k: round(float(np.mean(v)), 3) for k, v in phases.items()}, "log": log}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -342,6 +417,10 @@ def main():
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
+    ap.add_argument("--joint", type=int, default=0,
+                    help="also run this many iterations of the joint inversion on the 0.5 km / 1 km mesh "
+                         "(2001x301, 200 OBS, five phases; a few minutes)")
+    ap.add_argument("--joint-only", action="store_true", help="only the joint-inversion figure")
     ap.add_argument("--iteration-only", action="store_true",
                     help="only the inversion-iteration figure (for profiling its kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -361,6 +440,10 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.joint_only:
+        print(json.dumps({"joint": joint_figure(max(args.joint, 1))}))
+        return
 
     if args.iteration_only:
         print(json.dumps({"iteration": inversion_iteration(max(1, min(args.steps, 3)), args.warmup,
@@ -396,6 +479,7 @@ def main():
     it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
     progs = None if args.no_programs else programs_figure(args.cpu_sample > 0)
     grad = None if args.no_gradient else gradient_figure(max(1, min(args.steps, 2)))
+    joint = joint_figure(args.joint) if args.joint > 0 else None
     wall = float(np.sum(walls))
     stages = 5 * len(obs)
     value = stages * args.steps / wall
@@ -435,6 +519,7 @@ def main():
         "iteration": it,
         "programs": progs,
         "gradient": grad,
+        "joint": joint,
     }
     print(json.dumps(out))
 

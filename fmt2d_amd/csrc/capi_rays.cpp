@@ -190,6 +190,92 @@ int build_mesh(fmt2d_ctx* ctx) {
 
 }  // namespace
 
+// Sparse rows assembled in chunks of at most ctx->frechet_chunk summands:
+// emit(r0, nr, offsets, keys, vals) writes the (local row << 32 | column,
+// value) summands of rows r0 .. r0+nr-1, build_csr sorts and sums each
+// chunk, and the chunks' rows are placed one after another.  A row lies in
+// one chunk, so its sums do not depend on the chunking; the chunk bounds the
+// device memory of the sort (about 40 bytes per summand).
+template <class Emit>
+static int assemble_rows(fmt2d_ctx* ctx, int nrow, const std::vector<long long>& cnt, Emit emit, const char* what,
+                         DBuf<int>& rp_out, DBuf<int>& col_out, DBuf<double>& val_out, int& nnz_out) {
+  hipStream_t st = ctx->stream;
+  const long long cap = std::max<long long>(ctx->frechet_chunk, 1);
+  std::vector<int> bounds(1, 0);
+  long long acc = 0;
+  for (int r = 0; r < nrow; r++) {
+    if (acc > 0 && acc + cnt[r] > cap) { bounds.push_back(r); acc = 0; }
+    acc += cnt[r];
+  }
+  bounds.push_back(nrow);
+  std::vector<CsrOut> parts;
+  auto free_parts = [&]() {
+    for (CsrOut& c : parts) {
+      if (c.rowptr) (void)hipFree(c.rowptr);
+      if (c.col) (void)hipFree(c.col);
+      if (c.val) (void)hipFree(c.val);
+    }
+    parts.clear();
+  };
+  for (size_t c = 0; c + 1 < bounds.size(); c++) {
+    const int r0 = bounds[c], nr = bounds[c + 1] - r0;
+    std::vector<long long> ho(std::max(nr, 1), 0);
+    long long total = 0;
+    for (int r = 0; r < nr; r++) { ho[r] = total; total += cnt[r0 + r]; }
+    DBuf<long long> coff;
+    DBuf<unsigned long long> keys;
+    DBuf<double> vals;
+    hipError_t e = upload(coff, ho, st);
+    if (e == hipSuccess) e = keys.alloc((size_t)std::max<long long>(total, 1));
+    if (e == hipSuccess) e = vals.alloc((size_t)std::max<long long>(total, 1));
+    if (e == hipSuccess) e = emit(r0, nr, coff.p, keys.p, vals.p);
+    int rbits = 1;
+    while ((1LL << rbits) < nr) rbits++;
+    CsrOut co;
+    if (e == hipSuccess) e = build_csr(keys.p, vals.p, total, nr, 32 + rbits, 0, co, st);
+    parts.push_back(co);
+    if (e != hipSuccess) {
+      free_parts();
+      return fail(FMT2D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+  }
+  rp_out.release(); col_out.release(); val_out.release();
+  if (parts.size() == 1) {
+    const CsrOut& co = parts[0];
+    rp_out.p = co.rowptr; rp_out.n = (size_t)nrow + 1;
+    col_out.p = co.col; col_out.n = (size_t)std::max(co.nnz, 1);
+    val_out.p = co.val; val_out.n = (size_t)std::max(co.nnz, 1);
+    nnz_out = co.nnz;
+    parts.clear();
+    return FMT2D_OK;
+  }
+  long long nnz = 0;
+  for (const CsrOut& co : parts) nnz += co.nnz;
+  if (nnz > 2147483647LL) { free_parts(); return fail(FMT2D_EINVAL, std::string(what) + ": more than 2^31 entries"); }
+  std::vector<int> rp((size_t)nrow + 1, 0), part_rp;
+  hipError_t e = col_out.alloc((size_t)std::max<long long>(nnz, 1));
+  if (e == hipSuccess) e = val_out.alloc((size_t)std::max<long long>(nnz, 1));
+  long long off = 0;
+  for (size_t c = 0; c < parts.size() && e == hipSuccess; c++) {
+    const int r0 = bounds[c], nr = bounds[c + 1] - r0;
+    part_rp.resize((size_t)nr + 1);
+    e = hipMemcpyAsync(part_rp.data(), parts[c].rowptr, sizeof(int) * (nr + 1), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    for (int r = 0; r <= nr && e == hipSuccess; r++) rp[(size_t)r0 + r] = (int)(off + part_rp[r]);
+    if (e == hipSuccess && parts[c].nnz > 0)
+      e = hipMemcpyAsync(col_out.p + off, parts[c].col, sizeof(int) * parts[c].nnz, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && parts[c].nnz > 0)
+      e = hipMemcpyAsync(val_out.p + off, parts[c].val, sizeof(double) * parts[c].nnz, hipMemcpyDeviceToDevice, st);
+    off += parts[c].nnz;
+  }
+  if (e == hipSuccess) e = upload(rp_out, rp, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  free_parts();
+  if (e != hipSuccess) return fail(FMT2D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  nnz_out = (int)nnz;
+  return FMT2D_OK;
+}
+
 // Frechet rows of the last trace, assembled on the device once per trace.
 int frechet_build(fmt2d_ctx* ctx) {
   if (ctx->g_nnz >= 0) return FMT2D_OK;
@@ -197,37 +283,21 @@ int frechet_build(fmt2d_ctx* ctx) {
   hipStream_t st = ctx->stream;
   const int nray = ctx->nray;
   // pass 1: contributions per ray
-  DBuf<long long> cnt, coff;
+  DBuf<long long> cnt;
   HIPCHK(cnt.alloc(nray));
   HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
                                 ctx->rout.p, nullptr, cnt.p, nullptr, nullptr, st));
-  std::vector<long long> hc(nray), ho(nray);
+  std::vector<long long> hc(nray);
   HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  long long total = 0;
-  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
-  HIPCHK(upload(coff, ho, st));
-  DBuf<unsigned long long> keys;
-  DBuf<double> vals;
-  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nray, 0, ctx->rpoff.p, ctx->rpts.p,
-                                ctx->rout.p, coff.p, nullptr, keys.p, vals.p, st));
-  int rbits = 1;
-  while ((1LL << rbits) < nray) rbits++;
-  CsrOut co;
-  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, 0, co, st);
-  if (e != hipSuccess) {
-    if (co.rowptr) (void)hipFree(co.rowptr);
-    if (co.col) (void)hipFree(co.col);
-    if (co.val) (void)hipFree(co.val);
-    return fail(FMT2D_EHIP, std::string("CSR assembly: ") + hipGetErrorString(e));
-  }
-  ctx->g_rp.release(); ctx->g_col.release(); ctx->g_val.release();
-  ctx->g_rp.p = co.rowptr; ctx->g_rp.n = (size_t)nray + 1;
-  ctx->g_col.p = co.col; ctx->g_col.n = (size_t)std::max(co.nnz, 1);
-  ctx->g_val.p = co.val; ctx->g_val.n = (size_t)std::max(co.nnz, 1);
-  ctx->g_nnz = co.nnz;
+  auto emit = [&](int r0, int nr, const long long* coff, unsigned long long* keys, double* vals) {
+    return launch_frechet_contrib(ctx->rgeom.p, ctx->rfields.p, ctx->rjobs.p, nr, r0, ctx->rpoff.p, ctx->rpts.p,
+                                  ctx->rout.p, coff, nullptr, keys, vals, st);
+  };
+  int nnz = 0;
+  const int rc = assemble_rows(ctx, nray, hc, emit, "CSR assembly", ctx->g_rp, ctx->g_col, ctx->g_val, nnz);
+  if (rc) return rc;
+  ctx->g_nnz = nnz;
   ctx->gt_ok = false;
   ctx->gp_nnz = -1;
   return FMT2D_OK;
@@ -245,37 +315,21 @@ int system_rows_build(fmt2d_ctx* ctx) {
     return fail(FMT2D_ESTATE, "the parameterisation changed after the last trace: run a forward and trace again");
   hipStream_t st = ctx->stream;
   const int nray = ctx->nray;
-  DBuf<long long> cnt, coff;
+  DBuf<long long> cnt;
   HIPCHK(cnt.alloc(nray));
   HIPCHK(launch_prolong_rows(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, ctx->p_rp.p, ctx->p_col.p, ctx->p_w.p,
                              nullptr, cnt.p, nullptr, nullptr, st));
-  std::vector<long long> hc(nray), ho(nray);
+  std::vector<long long> hc(nray);
   HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(long long) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  long long total = 0;
-  for (int r = 0; r < nray; r++) { ho[r] = total; total += hc[r]; }
-  HIPCHK(upload(coff, ho, st));
-  DBuf<unsigned long long> keys;
-  DBuf<double> vals;
-  HIPCHK(keys.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(vals.alloc((size_t)std::max<long long>(total, 1)));
-  HIPCHK(launch_prolong_rows(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, ctx->p_rp.p, ctx->p_col.p, ctx->p_w.p,
-                             coff.p, nullptr, keys.p, vals.p, st));
-  int rbits = 1;
-  while ((1LL << rbits) < nray) rbits++;
-  CsrOut co;
-  const hipError_t e = build_csr(keys.p, vals.p, total, nray, 32 + rbits, 0, co, st);
-  if (e != hipSuccess) {
-    if (co.rowptr) (void)hipFree(co.rowptr);
-    if (co.col) (void)hipFree(co.col);
-    if (co.val) (void)hipFree(co.val);
-    return fail(FMT2D_EHIP, std::string("G P assembly: ") + hipGetErrorString(e));
-  }
-  ctx->gp_rp.release(); ctx->gp_col.release(); ctx->gp_val.release();
-  ctx->gp_rp.p = co.rowptr; ctx->gp_rp.n = (size_t)nray + 1;
-  ctx->gp_col.p = co.col; ctx->gp_col.n = (size_t)std::max(co.nnz, 1);
-  ctx->gp_val.p = co.val; ctx->gp_val.n = (size_t)std::max(co.nnz, 1);
-  ctx->gp_nnz = co.nnz;
+  auto emit = [&](int r0, int nr, const long long* coff, unsigned long long* keys, double* vals) {
+    return launch_prolong_rows(ctx->g_rp.p + r0, ctx->g_col.p, ctx->g_val.p, nr, ctx->p_rp.p, ctx->p_col.p,
+                               ctx->p_w.p, coff, nullptr, keys, vals, st);
+  };
+  int nnz = 0;
+  const int rc = assemble_rows(ctx, nray, hc, emit, "G P assembly", ctx->gp_rp, ctx->gp_col, ctx->gp_val, nnz);
+  if (rc) return rc;
+  ctx->gp_nnz = nnz;
   return FMT2D_OK;
 }
 
@@ -345,6 +399,15 @@ int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const 
   ctx->topo_init = ctx->m.if_z[2];
   ctx->gs_nparm = -1;
   ctx->gsi_nparm = -1;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_frechet_chunk(fmt2d_ctx* ctx, long long max_summands) {
+  if (!ctx || max_summands < 1) return fail(FMT2D_EINVAL, "the chunk needs at least one summand");
+  ctx->frechet_chunk = max_summands;
+  ctx->g_nnz = -1;
+  ctx->gp_nnz = -1;
+  ctx->gt_ok = false;
   return FMT2D_OK;
 }
 

@@ -150,6 +150,7 @@ struct fmt2d_ctx {
   DBuf<double> gt_val;
   bool gt_ok = false;
   int g_nnz = -1;                      // -1: not built for the current trace
+  long long frechet_chunk = 1LL << 28; // summands sorted at once by the row assembly
   // tomography system (selected, weighted, scaled rows) and its column copy
   int sys_ndata = -1, sys_nnz = 0;
   double sys_eps_min = 0.0;

@@ -110,6 +110,7 @@ def load() -> C.CDLL:
     lib.fmt2d_set_inversion.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
     lib.fmt2d_inversion_info.argtypes = [C.c_void_p, _ip, _ip]
     lib.fmt2d_set_inversion_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.fmt2d_set_frechet_chunk.argtypes = [C.c_void_p, C.c_longlong]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
@@ -138,7 +139,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_set_tuning",
            "fmt2d_tf_region",
            "fmt2d_get_field",
-           "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_inversion_info", "fmt2d_mesh_info",
+           "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
            "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
            "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
            "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
@@ -225,6 +226,10 @@ class Context:
         """Inversion mesh: velocity parameters at every vel_stride-th grid
         column and row, interface depths at every iface_stride-th node."""
         _check(self._lib.fmt2d_set_inversion_mesh(self._h, int(vel_stride), int(iface_stride)))
+
+    def set_frechet_chunk(self, max_summands: int = 1 << 28):
+        """Contributions sorted at once when the Frechet rows are assembled."""
+        _check(self._lib.fmt2d_set_frechet_chunk(self._h, int(max_summands)))
 
     def inversion_info(self):
         """(velocity parameters, all parameters) of the inversion mesh."""

@@ -131,6 +131,13 @@ int fmt2d_get_field(fmt2d_ctx* ctx, int isrc, int itf, double* T, double* dTdx, 
  * the listed interfaces (2 seafloor, 3 sediment base, 4 Moho).  Default:
  * velocity only.  Resets the inversion mesh to the grid. */
 int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const int* iface_ids);
+/* Device memory bound of the Frechet-row and G P assembly: at most
+ * max_summands (ray, node) contributions are sorted at once (about 40 bytes
+ * each; default 2^28); rows are assembled in chunks of whole rays, so the
+ * result does not depend on it.  The rows of the last trace are assembled
+ * again on next use.  (No counterpart in the reference, which accumulates
+ * each ray into a dense row, inversion2d.f90:4573-4577.) */
+int fmt2d_set_frechet_chunk(fmt2d_ctx* ctx, long long max_summands);
 /* Inversion mesh coarser than the propagation grid (no counterpart in the
  * reference, whose inversion grid is the propagation grid,
  * fmt2d_main.f90:200-205): velocity parameters at every vel_stride-th grid
