@@ -375,6 +375,7 @@ def joint_figure(iters: int):
         t0 = time.perf_counter()
         ctx.forward(list(geom.obs_x))
         t1 = time.perf_counter()
+        reused = ctx.stats().launches == 0     # the previous line search left this model's fields
         t, valid = ctx.trace(*rays)
         t2 = time.perf_counter()
         nd, nnz, _ = ctx.build_system(t_obs, crit_chi=1.0, refl_weight=10.0, alt_mode=mode)
@@ -392,6 +393,7 @@ def joint_figure(iters: int):
         log.append({"iteration": it, "mode": "velocity" if mode == 1 else "interfaces", "ndata": nd, "nnz": nnz,
                     "rms": round(r.rms_orig, 6), "rms_new": round(r.rms_new, 6), "step": r.step,
                     "lsqr_iterations": dres.iterations, "line_search_forwards": r.forwards,
+                    "forward_reused": bool(reused),
                     "seconds": round(t5 - t0, 3)})
         print(f"joint iteration {it}: {log[-1]}", file=sys.stderr, flush=True)
     ctx.close()

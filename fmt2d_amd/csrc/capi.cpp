@@ -79,6 +79,7 @@ int fmt2d_set_model(fmt2d_ctx* ctx, int nx_s, int nz_s, const double* xpos, cons
 // Device copies of the host model (grid, regions, intersections, PmP
 // head-wave map); after fmt2d_set_model and after a model step.
 int model_upload(fmt2d_ctx* ctx) {
+  ctx->fields_current = false;
   HostModel& m = ctx->m;
   hipStream_t st = ctx->stream;
   const int nx = m.nx, nz = m.nz;
@@ -214,6 +215,25 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (nsrc < 1) return fail(FMT2D_EINVAL, "nsrc must be >= 1");
   if (flags & ~3) return fail(FMT2D_EINVAL, "unknown forward flag");
   HIPCHK(hipSetDevice(ctx->device));
+  // A line search that accepts a step ends on the forward of exactly the
+  // model it commits; the same forward again would give the same fields bit
+  // for bit, so it returns at once (rays of the old trace are dropped as
+  // after any forward).
+  if (ctx->fields_current && flags == ctx->fields_flags && nsrc == ctx->nsrc &&
+      std::equal(src_x, src_x + nsrc, ctx->src_x.begin())) {
+    ctx->nray = 0;
+    ctx->g_nnz = -1;
+    ctx->gp_nnz = -1;
+    ctx->sys_ndata = -1;
+    const int nfield = ctx->stats.nfield;
+    const long long region_nodes = ctx->stats.region_nodes;
+    ctx->stats = fmt2d_forward_stats{};
+    ctx->stats.nsrc = nsrc;
+    ctx->stats.nfield = nfield;
+    ctx->stats.region_nodes = region_nodes;
+    return FMT2D_OK;
+  }
+  ctx->fields_current = false;
   // the forward program's variant has no head-wave post-pass
   const bool program = (flags & FMT2D_FWD_PROGRAM) != 0;
   if (program) flags |= FMT2D_FWD_NO_HEADWAVE;
@@ -379,7 +399,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         }
         HIPCHK(hipEventRecord(ctx->ev[ei], st));
         HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, ctx->tin.p,
-                          ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->colors,
+                          ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->max_sweeps_heavy,
+                          ctx->colors,
                           ctx->acc.p, ctx->tmax.p + launches, (int)ntt, st));
         HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
         cur = nxt;
@@ -495,6 +516,12 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
   ctx->colors = colors;
   ctx->window_tiles = window_tiles;
   ctx->max_sweeps = max_sweeps;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy) {
+  if (!ctx || max_sweeps_heavy < 1) return fail(FMT2D_EINVAL, "the sweep cap must be >= 1");
+  ctx->max_sweeps_heavy = max_sweeps_heavy;
   return FMT2D_OK;
 }
 

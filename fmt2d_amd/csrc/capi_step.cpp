@@ -345,10 +345,12 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   static const double LADDER[7] = {0.0, 1.0, 0.75, 0.5, 0.25, 0.125, 0.0625};
   double step_len = 0.0, rms_best = rms_orig;
   int idx_ok = 1, forwards = 0;
+  ModelState trial_model;
   for (int li = p->first_step; li <= 6; li++) {
     const double s = LADDER[li];
     int rc = restore_state(ctx, saved, true);
     if (!rc) rc = model_step(ctx, s, s * p->iface_step_ratio, dm);
+    if (!rc) trial_model = save_state(ctx);
     if (!rc) rc = fmt2d_forward(ctx, (int)src_x.size(), src_x.data(), 0);
     if (!rc) rc = fmt2d_trace(ctx, nray, js.data(), jp.data(), jx.data(), jz.data(), tc.data(), valid.data());
     if (rc) return rc;
@@ -371,6 +373,19 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
   int rc = restore_state(ctx, saved, step_len > 0.0);
   if (!rc && step_len > 0.0) rc = model_step(ctx, step_len, step_len * p->iface_step_ratio, dm);
   if (rc) return rc;
+  // an accepted step is the last trial: when the committed model equals that
+  // trial's byte for byte, its fields on the device are the model's fields
+  if (step_len > 0.0) {
+    const ModelState now = save_state(ctx);
+    bool same = now.vel.size() == trial_model.vel.size() &&
+                memcmp(now.vel.data(), trial_model.vel.data(), sizeof(double) * now.vel.size()) == 0 &&
+                now.ifz.size() == trial_model.ifz.size();
+    for (size_t k = 0; same && k < now.ifz.size(); k++)
+      same = now.ifz[k].size() == trial_model.ifz[k].size() &&
+             memcmp(now.ifz[k].data(), trial_model.ifz[k].data(), sizeof(double) * now.ifz[k].size()) == 0;
+    ctx->fields_current = same;
+    ctx->fields_flags = 0;
+  }
   const double step_if = ctx->inv_ifaces.empty() ? 0.0 : step_len * p->iface_step_ratio;
   for (int j = 0; j < nparm; j++)
     dm[j] = (j < n_vel || ctx->inv_ifaces.empty()) ? step_len * dm[j] : step_if * dm[j];

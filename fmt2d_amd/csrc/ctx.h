@@ -92,6 +92,11 @@ struct fmt2d_ctx {
   DBuf<double> isnp[NIF + 1][2];  // normals of the forward program's reflection
   DBuf<IsecDev> isecs_prog;       // isecs with those normals
   bool fwd_program = false;       // last forward ran as the forward program (FMT2D_FWD_PROGRAM)
+  // the fields on the device are those of the current model for the last
+  // forward's sources and flags (set by a line search that ends on its
+  // accepted trial; cleared by every model change and forward)
+  bool fields_current = false;
+  int fields_flags = 0;
   DBuf<int> hw_slot;
   DBuf<double> hw_vhw;
   DBuf<HeadwaveMap> hw_map;
@@ -112,6 +117,7 @@ struct fmt2d_ctx {
   DBuf<uint8_t> theavy;               // per (region, tile): cut by an interface (slow visits)
   double window_tiles = 3.0;          // window in tile crossing times
   int max_sweeps = 8;                 // a visit over the cap stays active and continues later
+  int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;

@@ -70,8 +70,8 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
                            hipStream_t st);
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
-                      int due_cap, hipStream_t st);
+                      int* count_next, int max_sweeps, int max_sweeps_heavy, int colors, unsigned long long* acc,
+                      int* launch_max, int due_cap, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

@@ -577,13 +577,16 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
                                                        const int* __restrict__ count_due, int ntx, int ntz,
                                                        unsigned int* in_set, unsigned long long* tkey,
                                                        int2* list_next, int* count_next, int max_sweeps,
-                                                       int colors, unsigned long long* acc, int* launch_max,
-                                                       int due_cap) {
+                                                       int max_sweeps_heavy, int colors, unsigned long long* acc,
+                                                       int* launch_max, int due_cap) {
   __shared__ TileLds L;
   const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
-    fim_tile(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, max_sweeps, colors, acc, launch_max, L);
+    // a tile cut by an interface gets its own sweep cap: its visits are the
+    // slow ones, and a capped visit continues in a later launch
+    fim_tile(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, k < nh ? max_sweeps_heavy : max_sweeps,
+             colors, acc, launch_max, L);
     __syncthreads();
   }
 }
@@ -1035,10 +1038,10 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
 
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int colors, unsigned long long* acc, int* launch_max,
-                      int due_cap, hipStream_t st) {
+                      int* count_next, int max_sweeps, int max_sweeps_heavy, int colors, unsigned long long* acc,
+                      int* launch_max, int due_cap, hipStream_t st) {
   hipLaunchKernelGGL(fim_tiles_kernel, dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
-                     tkey, list_next, count_next, max_sweeps, colors, acc, launch_max, due_cap);
+                     tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max, due_cap);
   return hipGetLastError();
 }
 

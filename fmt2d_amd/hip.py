@@ -111,6 +111,7 @@ def load() -> C.CDLL:
     lib.fmt2d_inversion_info.argtypes = [C.c_void_p, _ip, _ip]
     lib.fmt2d_set_inversion_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_frechet_chunk.argtypes = [C.c_void_p, C.c_longlong]
+    lib.fmt2d_set_heavy_sweeps.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
@@ -136,7 +137,7 @@ def load() -> C.CDLL:
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist",
-           "fmt2d_set_tuning",
+           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps",
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
@@ -195,6 +196,9 @@ class Context:
 
     def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 8, colors: int = 2):
         _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
+
+    def set_heavy_sweeps(self, max_sweeps_heavy: int = 8):
+        _check(self._lib.fmt2d_set_heavy_sweeps(self._h, int(max_sweeps_heavy)))
 
     def stats(self) -> ForwardStats:
         st = ForwardStats()

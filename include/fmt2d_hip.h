@@ -77,7 +77,8 @@ typedef struct {
 
 typedef struct {
   int nsrc, nfield;         /* sources and resident fields of the last forward */
-  int launches;             /* tile-solver launches over all stage levels */
+  int launches;             /* tile-solver launches over all stage levels (0: the
+                               fields left by a line search were the answer) */
   long long tiles;          /* tiles processed over all launches */
   long long region_nodes;   /* sum of region node counts over the fields */
   double ms_total;          /* wall time of the forward */
@@ -112,6 +113,11 @@ int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
  * order (4: four-colour Gauss-Seidel, 2: two-colour Gauss-Seidel, 1: Jacobi).
  * Defaults 3.0, 8, 2. */
 int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors);
+/* Sweep cap of one visit of a tile cut by an interface (its visits are the
+ * slow ones; a capped visit continues in a later launch; results are
+ * identical for any cap).  Default 8.  (Solver tuning of this build, no
+ * counterpart in the reference's heap-ordered march.) */
+int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy);
 /* Due-tile count and the most sweeps of one tile visit, per solver launch of
  * the last forward (diagnostics); either array may be NULL. */
 int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, int* n);

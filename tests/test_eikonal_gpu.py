@@ -91,3 +91,23 @@ def test_multistage_fields(ms_fields, isrc):
         nb = int(np.sum((gx[fin] != gxo[fin]) | (gz[fin] != gzo[fin])))
         if nb:
             print(f"tf{itf}: {nb} of {fin.sum()} gradients differ in the last bits (max rel {g.max():.1e})")
+
+
+@pytest.mark.parametrize("tuning", [(2.0, 6, 4, 12), (3.0, 12, 2, 3)])
+def test_schedule_does_not_change_the_fields(fields, tuning):
+    """The tile schedule (window, sweep caps, in-tile sweep order) decides
+    only how the fixed point is reached: every field of the plain case is
+    the same, bit for bit, as with the default schedule."""
+    ctx, _ = fields
+    window, light, colors, heavy = tuning
+    before = [np.concatenate(ctx.field(i, f)).view(np.uint64).copy() for i in range(len(SOURCES)) for f in range(1, 13)]
+    try:
+        ctx.set_tuning(window, light, colors)
+        ctx.set_heavy_sweeps(heavy)
+        ctx.forward(SOURCES)
+        after = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
+    finally:
+        ctx.set_tuning()
+        ctx.set_heavy_sweeps()
+        ctx.forward(SOURCES)
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))

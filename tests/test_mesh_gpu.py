@@ -16,7 +16,9 @@ through the same G P code with P = I.  Here:
   node coordinates and hat functions, columns scaled as the system scales
   them;
 * one iteration (automatic damping, line search) moves the grid's
-  slowness only within the range of P, and lowers the RMS;
+  slowness only within the range of P, and lowers the RMS; the next
+  forward reuses the accepted trial's fields and traces the same rays, bit
+  for bit, as a full solve;
 * invert2d_hip with the mesh line in invert2d.in runs the alternating
   joint inversion and lowers the RMS."""
 import os
@@ -225,6 +227,17 @@ def test_iteration_stays_in_the_range_of_p(case):
                    for (ix, iz) in fine])
     assert np.abs(got - pred)[ok].max() <= 1e-12 * max(np.abs(got).max(), 1e-30)
     assert np.abs(got).max() > 0.0
+    # the accepted trial's fields are the committed model's: the next forward
+    # of the same sources reuses them, and its rays equal those of a full solve
+    ctx.forward(case["obs_x"])
+    assert ctx.stats().launches == 0
+    t_reused, v_reused = ctx.trace(*case["rays"])
+    ctx.model_put(0, ctx.model(0))               # any model change: the fields are solved again
+    ctx.forward(case["obs_x"])
+    assert ctx.stats().launches > 0
+    t_full, v_full = ctx.trace(*case["rays"])
+    assert np.array_equal(t_reused.view(np.uint64), t_full.view(np.uint64))
+    assert np.array_equal(v_reused, v_full)
     for k in IFACES:
         ctx.model_put(k, z[k])
     ctx.model_put(0, v0)
