@@ -31,7 +31,7 @@ fmt2d_amd.autograd (forward, rays, Frechet rows, adjoint product) at
 2001x301, 60 OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x
 3 km cells plus 30 ms seeded noise, starting model without it).
 
-Fifth figure (``joint``, with ``--joint N``): seconds per iteration of a
+Fifth figure (``joint``, ``--joint N``, default 4, 0 skips): seconds per iteration of a
 joint velocity + interface inversion on an inversion mesh coarser than the
 propagation grid: 2001x301, 200 OBS every 1 km, shots every 0.2 km, five
 phases (434k rays); velocity parameters on a 0.5 km mesh (401x61), sediment
@@ -419,9 +419,9 @@ def main():
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
-    ap.add_argument("--joint", type=int, default=0,
-                    help="also run this many iterations of the joint inversion on the 0.5 km / 1 km mesh "
-                         "(2001x301, 200 OBS, five phases; a few minutes)")
+    ap.add_argument("--joint", type=int, default=4,
+                    help="iterations of the joint inversion on the 0.5 km / 1 km mesh (2001x301, 200 OBS, five "
+                         "phases; about a minute for 4); 0 skips it")
     ap.add_argument("--joint-only", action="store_true", help="only the joint-inversion figure")
     ap.add_argument("--iteration-only", action="store_true",
                     help="only the inversion-iteration figure (for profiling its kernels)")
