@@ -291,7 +291,8 @@ class Context:
         return y
 
     def param_index(self):
-        """Parameter number (1-based, 0 none) of each velocity-grid node, (nz, nx)."""
+        """Frechet column (1-based, 0 none) of each velocity-grid node, (nz, nx);
+        the inversion parameters when the inversion mesh is the grid."""
         n = C.c_int()
         _check(self._lib.fmt2d_param_index(self._h, None, 0, C.byref(n)))
         idx = np.zeros(n.value, dtype=np.int32)

@@ -193,10 +193,12 @@ int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, do
  * sparse_Ax / sparse_ATx accumulate it (inversion2d.f90:2997-3042).  The
  * adjoint of the receiver times, e.g. for a torch.autograd backward. */
 int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* y);
-/* Parameter number (1-based; 0: not a parameter) of each velocity-grid node
- * (nz x nx, x fastest) as initialize_inversion2d numbers them
- * (frechet2d.f90:44-60); interface parameters follow the n_vel_par velocity
- * parameters.  With node_param == NULL or cap == 0 only *n. */
+/* Frechet column (1-based; 0: none) of each velocity-grid node (nz x nx, x
+ * fastest) as initialize_inversion2d numbers the parameters
+ * (frechet2d.f90:44-60); the interfaces' input nodes follow.  These are the
+ * inversion parameters when the inversion mesh is the grid (the default;
+ * see fmt2d_set_inversion_mesh).  With node_param == NULL or cap == 0 only
+ * *n. */
 int fmt2d_param_index(fmt2d_ctx* ctx, int* node_param, int cap, int* n);
 
 /* Data selection and scaling (lsqr_inversion2d's system assembly). */
