@@ -349,16 +349,18 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (ctx->tin.n < ntt) {
     HIPCHK(ctx->tin.alloc(ntt));
     HIPCHK(ctx->tkey.alloc(ntt));
-    HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
-    HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
     for (int b = 0; b < 3; b++) HIPCHK(ctx->tlist[b].alloc(ntt));
   }
+  // every tile starts outside the active set with no key: a forward that
+  // returned early (launch cap, HIP error) leaves tiles marked active
+  HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
+  HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
   if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
   if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
   if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3 + 64));
   HIPCHK(hipMemsetAsync(ctx->acc.p, 0, (3 + 64) * sizeof(unsigned long long), st));
-  const int max_launch = 64 * (ntx + ntz) + 4000;
+  const int max_launch = ctx->launch_cap > 0 ? ctx->launch_cap : 64 * (ntx + ntz) + 4000;
   const bool headwave = !(flags & FMT2D_FWD_NO_HEADWAVE);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
   const double window = ctx->window_tiles * 8.0 * std::max(m.dx, m.dz) / 4.0;
@@ -516,6 +518,12 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
   ctx->colors = colors;
   ctx->window_tiles = window_tiles;
   ctx->max_sweeps = max_sweeps;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_launch_cap(fmt2d_ctx* ctx, int max_launches) {
+  if (!ctx || max_launches < 0) return fail(FMT2D_EINVAL, "the launch cap must be >= 0");
+  ctx->launch_cap = max_launches;
   return FMT2D_OK;
 }
 

@@ -353,7 +353,13 @@ int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const d
     if (!rc) trial_model = save_state(ctx);
     if (!rc) rc = fmt2d_forward(ctx, (int)src_x.size(), src_x.data(), 0);
     if (!rc) rc = fmt2d_trace(ctx, nray, js.data(), jp.data(), jx.data(), jz.data(), tc.data(), valid.data());
-    if (rc) return rc;
+    if (rc) {
+      // the caller's model is put back (restore_model_state); the trial's
+      // error is the one reported
+      const std::string msg = fmt2d_last_error();
+      (void)restore_state(ctx, saved);
+      return fail(rc, msg);
+    }
     forwards++;
     // compute_rms_after_forward (:3298-3339)
     int n = 0;

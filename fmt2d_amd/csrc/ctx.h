@@ -119,6 +119,7 @@ struct fmt2d_ctx {
   int max_sweeps = 8;                 // a visit over the cap stays active and continues later
   int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
+  int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;

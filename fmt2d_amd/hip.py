@@ -112,6 +112,7 @@ def load() -> C.CDLL:
     lib.fmt2d_set_inversion_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_frechet_chunk.argtypes = [C.c_void_p, C.c_longlong]
     lib.fmt2d_set_heavy_sweeps.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
@@ -137,7 +138,7 @@ def load() -> C.CDLL:
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist",
-           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps",
+           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap",
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
@@ -164,6 +165,7 @@ class Context:
         _check(self._lib.fmt2d_ctx_create(device, C.byref(h)))
         self._h = h
         self.info: Optional[GridInfo] = None
+        self._nray = 0
 
     def close(self):
         if self._h:
@@ -192,6 +194,7 @@ class Context:
         rules in the next trace)."""
         xs = np.ascontiguousarray(src_x, dtype=np.float64)
         flags = (0 if headwave else FWD_NO_HEADWAVE) | (FWD_PROGRAM if program else 0)
+        self._nray = 0   # a forward drops the rays of the last trace
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), flags))
 
     def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 8, colors: int = 2):
@@ -199,6 +202,9 @@ class Context:
 
     def set_heavy_sweeps(self, max_sweeps_heavy: int = 8):
         _check(self._lib.fmt2d_set_heavy_sweeps(self._h, int(max_sweeps_heavy)))
+
+    def set_launch_cap(self, max_launches: int = 0):
+        _check(self._lib.fmt2d_set_launch_cap(self._h, int(max_launches)))
 
     def stats(self) -> ForwardStats:
         st = ForwardStats()
@@ -254,11 +260,14 @@ class Context:
         rx = np.ascontiguousarray(rx, dtype=np.float64)
         rz = np.ascontiguousarray(rz, dtype=np.float64)
         n = len(src)
-        self._nray = n
+        if not (len(path) == len(rx) == len(rz) == n):
+            raise ValueError("src, path, rx and rz must have the same length")
         t = np.zeros(n)
         v = np.zeros(n, dtype=np.int32)
+        self._nray = 0
         _check(self._lib.fmt2d_trace(self._h, n, src.ctypes.data_as(_ip), path.ctypes.data_as(_ip), _p(rx), _p(rz),
                                      _p(t), v.ctypes.data_as(_ip)))
+        self._nray = n
         return t, v.astype(bool)
 
     def ray_info(self, iray: int):
@@ -314,6 +323,8 @@ class Context:
     def build_system(self, t_obs, crit_chi=1.0, phase_wt_psed=1.0, phase_wt_pmp=1.0, phase_wt_pgss=1.0,
                      refl_weight=10.0, alt_mode=0):
         t = np.ascontiguousarray(t_obs, dtype=np.float64)
+        if len(t) != self._nray:
+            raise ValueError(f"t_obs has {len(t)} entries for {self._nray} rays of the last trace")
         p = SystemParams(crit_chi, phase_wt_psed, phase_wt_pmp, phase_wt_pgss, refl_weight, alt_mode)
         nd, nz, eps = C.c_int(), C.c_int(), C.c_double()
         _check(self._lib.fmt2d_build_system(self._h, _p(t), C.byref(p), C.byref(nd), C.byref(nz), C.byref(eps)))
@@ -382,6 +393,10 @@ class Context:
         when given, it is updated in place with the rays the trial forwards
         lost."""
         t_obs = np.ascontiguousarray(t_obs, dtype=np.float64)
+        if len(t_obs) != self._nray:
+            raise ValueError(f"t_obs has {len(t_obs)} entries for {self._nray} rays of the last trace")
+        if valid is not None and len(valid) != self._nray:
+            raise ValueError(f"valid has {len(valid)} entries for {self._nray} rays of the last trace")
         dm = np.array(dm, dtype=np.float64)
         self._check_dm(dm)
         p = LineSearchParams(crit_chi, iface_step_ratio, first_step)

@@ -202,6 +202,20 @@ def multistage_case(dx: float = 0.1, nobs: int = 60):
     return m, g
 
 
+def batched_case(dx: float = 0.1, nobs: int = 200, seed: int = 3):
+    """The batched 1000-source-stage forward: 0.1 km grid (2001x301), sediment
+    base and Moho perturbed by seeded random Fourier modes (wavelengths
+    10-100 km, sum of |amplitudes| 0.5 km and 1.5 km), ``nobs`` OBS every
+    200/nobs km from 0.5 km (every 1 km for 200), 1000 shots every 0.2 km at
+    10 m depth, receiver lines of all five phases in the reference's
+    windows."""
+    m = make_model(dx=dx, fourier_seed=seed)
+    step = 200.0 / nobs
+    obs = np.round(step / 2.0 + step * np.arange(nobs), 6)
+    shots = np.round(0.1 + 0.2 * np.arange(1000), 6)
+    return m, phase_windows_geometry(m, obs, shots)
+
+
 def gradient_case(dx: float = 0.1, nobs: int = 60):
     """The adjoint-gradient workload: 0.1 km grid (2001x301), ``nobs`` OBS
     every 3.3 km, 1000 shots every 0.2 km at 10 m depth, Pg at every offset

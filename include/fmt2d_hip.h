@@ -118,6 +118,12 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
  * identical for any cap).  Default 8.  (Solver tuning of this build, no
  * counterpart in the reference's heap-ordered march.) */
 int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy);
+/* Solver launches allowed per stage level before fmt2d_forward gives up with
+ * FMT2D_ENOCONV (0, the default: 64 x (tile columns + tile rows) + 4000).  A
+ * forward that gave up leaves the context usable: the next forward starts
+ * from clean tile state.  (Solver control of this build, no counterpart in
+ * the reference's heap-ordered march.) */
+int fmt2d_set_launch_cap(fmt2d_ctx* ctx, int max_launches);
 /* Due-tile count and the most sweeps of one tile visit, per solver launch of
  * the last forward (diagnostics); either array may be NULL. */
 int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, int* n);
@@ -303,7 +309,9 @@ typedef struct {
  * forward loses stays out of the RMS of the later trials, as ray%valid is
  * only set again at the start of the next iteration (:283-288).  valid
  * (nray, may be NULL): in, the validity of the iteration's rays (NULL: that
- * of the last trace); out, and-ed with every trial forward's. */
+ * of the last trace); out, and-ed with every trial forward's.  When a trial's
+ * model step, forward or trace fails, the model the caller passed in is put
+ * back (restore_model_state, :7242-7276) before the error returns. */
 int fmt2d_line_search(fmt2d_ctx* ctx, const fmt2d_line_search_params* p, const double* t_obs, double* dm,
                       int* valid, fmt2d_line_search_result* res);
 

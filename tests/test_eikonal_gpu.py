@@ -111,3 +111,21 @@ def test_schedule_does_not_change_the_fields(fields, tuning):
         ctx.set_heavy_sweeps()
         ctx.forward(SOURCES)
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_forward_after_launch_cap_matches_fresh(fields):
+    """A forward that gives up at the launch cap (FMT2D_ENOCONV) leaves tiles
+    marked active on the device; the next forward must start from clean tile
+    state and give the same fields, bit for bit, as before."""
+    from fmt2d_amd import hip
+    ctx, _ = fields
+    before = [np.concatenate(ctx.field(i, f)).view(np.uint64).copy() for i in range(len(SOURCES)) for f in range(1, 13)]
+    try:
+        ctx.set_launch_cap(3)
+        with pytest.raises(hip.Fmt2dError, match="did not converge"):
+            ctx.forward(SOURCES)
+    finally:
+        ctx.set_launch_cap(0)
+    ctx.forward(SOURCES)
+    after = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))

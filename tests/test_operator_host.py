@@ -1,0 +1,75 @@
+"""The device's node-update operator (fmt2d_amd/csrc/eik_update.h) and the
+host model build (fmt2d_amd/csrc/model.cpp), compiled for the CPU with g++
+and iterated serially by tests/native/eik_cpu_check.cpp, against the
+reference's own stage-1 fields (tests/golden/ref_fields_dx2.npz: a point
+source in region 5, propagate2d.f90:33-127 after
+initialize_region_from_source2d, :2307-2411).  This keeps a CPU-side check
+on the exact operator the GPU kernels run: a change to eik_update.h that
+moves a single bit of a traveltime or gradient fails here without a GPU.
+
+The GPU tile schedule (emulated), plain Jacobi sweeps and in-place
+Gauss-Seidel sweeps over the whole grid must all reach the
+reference's field bit for bit (the fixed point does not depend on the
+schedule)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import load_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "ref_fields_dx2.npz")
+
+
+@pytest.fixture(scope="module")
+def operator_lib(tmp_path_factory):
+    d = tmp_path_factory.mktemp("eikcpu")
+    so = str(d / "libeikcpu.so")
+    srcs = [os.path.join(ROOT, "tests", "native", "eik_cpu_check.cpp"),
+            os.path.join(ROOT, "fmt2d_amd", "csrc", "model.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", so] + srcs,
+                   check=True)
+    lib = C.CDLL(so)
+    dp = C.POINTER(C.c_double)
+    lib.cpu_model_create.restype = C.c_void_p
+    lib.cpu_model_create.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, C.c_int, dp, dp, C.c_int, dp, dp,
+                                     C.c_char_p, C.c_int]
+    lib.cpu_region_nnode.argtypes = [C.c_void_p, C.c_int]
+    lib.cpu_point_source.argtypes = [C.c_void_p, C.c_int, C.c_double, dp, dp, dp, C.c_int]
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@pytest.mark.parametrize("maxit", [0, -2000, 2000], ids=["tile_schedule", "jacobi", "gauss_seidel"])
+def test_operator_reaches_reference_field(operator_lib, tmp_path, maxit):
+    g = np.load(GOLD)
+    for name in ("model.smesh", "moho.dat", "sed.dat"):
+        with open(tmp_path / name, "w") as f:
+            f.write(str(g["file_" + name.replace(".", "_")]))
+    sm, sx, sz, mx, mz = load_case(str(tmp_path))
+    vel = np.ascontiguousarray(sm.vel.reshape(-1))
+    xs, tp, zr = (np.ascontiguousarray(a, dtype=np.float64) for a in (sm.xpos, sm.topo, sm.zrel))
+    sx, sz, mx, mz = (np.ascontiguousarray(a, dtype=np.float64) for a in (sx, sz, mx, mz))
+    err = C.create_string_buffer(256)
+    h = operator_lib.cpu_model_create(len(xs), len(zr), _p(xs), _p(tp), _p(zr), _p(vel), sm.vwater, len(sx), _p(sx),
+                                      _p(sz), len(mx), _p(mx), _p(mz), err, 256)
+    assert h, err.value.decode()
+    n = operator_lib.cpu_region_nnode(h, 5)
+    for i, s in enumerate(g["sources"]):
+        T, gx, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+        operator_lib.cpu_point_source(h, 5, float(s), _p(T), _p(gx), _p(gz), maxit)
+        To = g[f"s{i}_tf1_T"]
+        # unreached nodes: the reference's huge_time (1e20) and this build's
+        # (1e20 as a float constant) differ in representation only
+        fin = To < 1e19
+        assert np.array_equal(fin, T < 1e19), f"source {s}: reached-node sets differ"
+        assert np.array_equal(T[fin].view(np.int64), To[fin].view(np.int64)), \
+            f"source {s}: {int(np.sum(T[fin] != To[fin]))} traveltimes differ"
+        assert np.array_equal(gx[fin], g[f"s{i}_tf1_gx"][fin]), f"source {s}: dT/dx"
+        assert np.array_equal(gz[fin], g[f"s{i}_tf1_gz"][fin]), f"source {s}: dT/dz"
