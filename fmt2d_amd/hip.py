@@ -127,6 +127,7 @@ def load() -> C.CDLL:
     lib.fmt2d_model_get.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
     lib.fmt2d_model_put.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_frechet_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.fmt2d_frechet_apply_noalloc.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.fmt2d_param_index.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
     lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp, _ip,
                                       C.POINTER(LineSearchResult)]
@@ -143,7 +144,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
            "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
-           "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
+           "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_frechet_apply_noalloc", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
            "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
 
 
@@ -297,6 +298,18 @@ class Context:
             raise ValueError("x has the wrong length")
         y = np.zeros(n_par if transpose else nray)
         _check(self._lib.fmt2d_frechet_apply(self._h, int(bool(transpose)), _p(x), _p(y)))
+        return y
+
+    def frechet_apply_noalloc(self, x, y, transpose: bool = False):
+        """frechet_apply into the caller's y without device allocation (the
+        rows must have been used once by frechet_apply on this trace)."""
+        n_par = self.mesh_info()[2]
+        nray = self._nray
+        if not (x.dtype == np.float64 and y.dtype == np.float64 and x.flags.c_contiguous and y.flags.c_contiguous):
+            raise TypeError("x and y must be contiguous float64 arrays")
+        if len(x) != (nray if transpose else n_par) or len(y) != (n_par if transpose else nray):
+            raise ValueError("x or y has the wrong length")
+        _check(self._lib.fmt2d_frechet_apply_noalloc(self._h, int(bool(transpose)), _p(x), _p(y)))
         return y
 
     def param_index(self):

@@ -199,6 +199,16 @@ int fmt2d_frechet_csr(fmt2d_ctx* ctx, int index_base, int* row_ptr, int* col, do
  * sparse_Ax / sparse_ATx accumulate it (inversion2d.f90:2997-3042).  The
  * adjoint of the receiver times, e.g. for a torch.autograd backward. */
 int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* y);
+/* fmt2d_frechet_apply without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops).  x and y are staged in a fixed 16 MiB
+ * device array of the library (one per device, held until the process
+ * ends): x and y together must fit in 2^21 doubles (FMT2D_EINVAL beyond).
+ * The rows (and, for transpose 1, their column copy) must already be built
+ * by one fmt2d_frechet_apply call on the same trace (FMT2D_ESTATE
+ * otherwise).  Results equal fmt2d_frechet_apply's bit for bit.  Calls that
+ * overlap on one device share that array and race. */
+int fmt2d_frechet_apply_noalloc(fmt2d_ctx* ctx, int transpose, const double* x, double* y);
 /* Frechet column (1-based; 0: none) of each velocity-grid node (nz x nx, x
  * fastest) as initialize_inversion2d numbers the parameters
  * (frechet2d.f90:44-60); the interfaces' input nodes follow.  These are the
