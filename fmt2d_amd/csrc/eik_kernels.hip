@@ -23,6 +23,7 @@
 
 #include "eik_kernels.h"
 #include "eik_update.h"
+#include "eik_regular.h"
 
 namespace fmt2d {
 
@@ -436,6 +437,11 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
   const int g_color = (gi & 1) | ((gj & 1) << 1);
   double g_t0 = FMT_HUGE_TIME, g_gx0 = 0.0, g_gz0 = 0.0;
   if (g_act) seed_grid(d, R, gi, gj, g_t0, g_gx0, g_gz0);
+  // nodes with four regular cells and no coincident interface node take the
+  // fast path (eik_regular.h); what it reads of the model is formed once
+  const bool g_fast = g_act && reg_fast_node(L.flags[wg]);
+  RegStatic g_rs;
+  if (g_fast) g_rs = reg_static(v, gi, gj);
   const int is = lane >> 3, ic = tx0 + (lane & 7);
   bool i_act = false;
   double i_t0 = FMT_HUGE_TIME, i_gx0 = 0.0, i_gz0 = 0.0;
@@ -485,8 +491,19 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
         if (g_act && (colors == 1 || (colors == 2 ? ((gi + gj) & 1) : g_color) == phase) &&
             ((g_pend & g_dep) | (unsigned long long)(g_ipend & g_idep))) {
           g_pend = 0ull; g_ipend = 0u;
-          b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
-          update_grid_node<double>(v, gi, gj, b);
+          if (g_fast) {
+            RegDyn y;
+            for (int q = 0; q < 4; q++) {
+              const int st = (q < 2 ? (q == 0 ? 1 : -1) : (q == 2 ? WIN : -WIN));
+              const int wm = wg + st, ws = wg + 2 * st;  // member, second node (inside the window)
+              y.mT[q] = L.T[wm]; y.mGX[q] = L.GX[wm]; y.mGZ[q] = L.GZ[wm]; y.mTie[q] = L.tie[wm];
+              y.sT[q] = L.T[ws]; y.sGX[q] = L.GX[ws]; y.sGZ[q] = L.GZ[ws];
+            }
+            reg_update(g_rs, y, g_t0, g_gx0, g_gz0, b.T, b.gx, b.gz, b.tie);
+          } else {
+            b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
+            update_grid_node<double>(v, gi, gj, b);
+          }
           upd = __double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
                 __double_as_longlong(b.gx) != __double_as_longlong(L.GX[wg]) ||
                 __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[wg]) || b.tie != (int)L.tie[wg];

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../fmt2d_amd/csrc/eik_update.h"
+#include "../../fmt2d_amd/csrc/eik_regular.h"
 #include "../../fmt2d_amd/csrc/model.h"
 #include "heap_exit_sorted.h"
 
@@ -63,6 +64,30 @@ struct CpuView {
     return key == F->fixed_key;
   }
 };
+
+// Fast-path check (eik_regular.h): when on, every evaluation of a fast-path
+// grid node also runs reg_update on the same state and counts the results
+// that differ from update_grid_node in any bit (T, gradient, equal-time mask).
+static int g_fast_check = 0;
+static long long g_fast_checked = 0, g_fast_bad = 0;
+
+template <class V>
+static void fast_check(const V& v, int i, int j, const Best<RT>& seed, const Best<RT>& b) {
+  if (!g_fast_check || !reg_fast_node(v.flags(i, j))) return;
+  const RegStatic s = reg_static(v, i, j);
+  const RegDyn y = reg_dyn(v, s, i, j);
+  double T, gx, gz;
+  int tie;
+  reg_update(s, y, seed.T, seed.gx, seed.gz, T, gx, gz, tie);
+  g_fast_checked++;
+  auto bits = [](double a) { long long q; memcpy(&q, &a, 8); return q; };
+  if (bits(T) != bits(b.T) || bits(gx) != bits(b.gx) || bits(gz) != bits(b.gz) || tie != b.tie) {
+    if (g_fast_bad < 5)
+      fprintf(stderr, "fast path differs at (%d,%d): T %.17g/%.17g gx %.17g/%.17g gz %.17g/%.17g tie %d/%d\n", i, j, T,
+              b.T, gx, b.gx, gz, b.gz, tie, b.tie);
+    g_fast_bad++;
+  }
+}
 
 struct Holder {
   HostModel m;
@@ -124,7 +149,9 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
         for (size_t k = 0; k < keys.size(); k++) {
           const int g = keys[k], j = g / nx, i = g % nx;
           Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0, 0, 1};
-          if (j < nz) update_grid_node<RT>(sv, i, j, b); else update_iface_node<RT>(sv, j - nz, i, b);
+          const Best<RT> seed = b;
+          if (j < nz) { update_grid_node<RT>(sv, i, j, b); fast_check(sv, i, j, seed, b); }
+          else update_iface_node<RT>(sv, j - nz, i, b);
           const int tie = j < nz ? b.tie : 0;
           if (b.T != loc.T[g] || (GT)b.gx != loc.GX[g] || (GT)b.gz != loc.GZ[g] || tie != loc.TIE[g]) {
             loc.T[g] = b.T; loc.GX[g] = b.gx; loc.GZ[g] = b.gz; loc.TIE[g] = tie; ch = true; dirty[k] = 1;
@@ -156,6 +183,17 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
 }
 
 extern "C" {
+
+void cpu_fast_check_set(int on) {
+  g_fast_check = on;
+  g_fast_checked = 0;
+  g_fast_bad = 0;
+}
+
+void cpu_fast_check_get(long long* checked, long long* bad) {
+  *checked = g_fast_checked;
+  *bad = g_fast_bad;
+}
 
 void* cpu_model_create(int nx_s, int nz_s, const double* xpos, const double* topo, const double* zrel,
                        const double* vel, double vwater, int nsed, const double* sx, const double* sz,
@@ -254,7 +292,9 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
         Best<RT> b{F.T0[g], F.GX0[g], F.GZ0[g], 0, 0, 0, 1};
         if (j < nz) {
           if (!(R.flags[g] & RF_IN) || (int)g == F.fixed_key) continue;
+          const Best<RT> seed = b;
           update_grid_node<RT>(v, i, j, b);
+          fast_check(v, i, j, seed, b);
         } else {
           update_iface_node<RT>(v, j - nz, i, b);
         }

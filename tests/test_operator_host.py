@@ -73,3 +73,46 @@ def test_operator_reaches_reference_field(operator_lib, tmp_path, maxit):
             f"source {s}: {int(np.sum(T[fin] != To[fin]))} traveltimes differ"
         assert np.array_equal(gx[fin], g[f"s{i}_tf1_gx"][fin]), f"source {s}: dT/dx"
         assert np.array_equal(gz[fin], g[f"s{i}_tf1_gz"][fin]), f"source {s}: dT/dz"
+
+
+def _model_handle(lib, d):
+    sm, sx, sz, mx, mz = load_case(d)
+    vel = np.ascontiguousarray(sm.vel.reshape(-1))
+    xs, tp, zr = (np.ascontiguousarray(a, dtype=np.float64) for a in (sm.xpos, sm.topo, sm.zrel))
+    sx, sz, mx, mz = (np.ascontiguousarray(a, dtype=np.float64) for a in (sx, sz, mx, mz))
+    err = C.create_string_buffer(256)
+    h = lib.cpu_model_create(len(xs), len(zr), _p(xs), _p(tp), _p(zr), _p(vel), sm.vwater, len(sx), _p(sx),
+                             _p(sz), len(mx), _p(mx), _p(mz), err, 256)
+    assert h, err.value.decode()
+    return h
+
+
+@pytest.mark.parametrize("maxit", [0, -2000, 2000], ids=["tile_schedule", "jacobi", "gauss_seidel"])
+def test_fast_regular_path_matches_operator(operator_lib, tmp_path, plain_case, maxit):
+    """reg_update (fmt2d_amd/csrc/eik_regular.h), the kernels' fast path for
+    grid nodes with four regular cells, against update_grid_node on every
+    state the iterations pass through (transient and converged, with and
+    without second-order terms and equal-time ties): the same bits for T,
+    both gradient components and the equal-time mask, at every evaluation."""
+    lib = operator_lib
+    lib.cpu_fast_check_set.argtypes = [C.c_int]
+    lib.cpu_fast_check_get.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    g = np.load(GOLD)
+    for name in ("model.smesh", "moho.dat", "sed.dat"):
+        with open(tmp_path / name, "w") as f:
+            f.write(str(g["file_" + name.replace(".", "_")]))
+    cases = [(str(tmp_path), [float(s) for s in g["sources"]]), (plain_case, [5.0, 95.0, 187.5])]
+    total = 0
+    for d, sources in cases:
+        h = _model_handle(lib, d)
+        n = lib.cpu_region_nnode(h, 5)
+        for s in sources:
+            T, gx, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+            lib.cpu_fast_check_set(1)
+            lib.cpu_point_source(h, 5, s, _p(T), _p(gx), _p(gz), maxit if d != plain_case or maxit != -2000 else -400)
+            checked, bad = C.c_longlong(), C.c_longlong()
+            lib.cpu_fast_check_get(C.byref(checked), C.byref(bad))
+            lib.cpu_fast_check_set(0)
+            assert bad.value == 0, f"source {s}: {bad.value} of {checked.value} fast-path evaluations differ"
+            total += checked.value
+    assert total > 100000
