@@ -358,6 +358,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
   if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
+  if (!ctx->stop_flag.p) HIPCHK(ctx->stop_flag.alloc(1));
+  HIPCHK(hipMemsetAsync(ctx->stop_flag.p, 0, sizeof(int), st));
   if (!ctx->acc.p) HIPCHK(ctx->acc.alloc(3 + 64));
   HIPCHK(hipMemsetAsync(ctx->acc.p, 0, (3 + 64) * sizeof(unsigned long long), st));
   const int max_launch = ctx->launch_cap > 0 ? ctx->launch_cap : 64 * (ntx + ntz) + 4000;
@@ -471,7 +473,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       HIPCHK(launch_exit_check(dl, ctx->regs.p, nf, (int)nslot, w, ncand, st));
       ctx->stats.exit_nodes += ncand;
     }
-    HIPCHK(launch_refit(dl, ctx->regs.p, nf, nx, nz, st));
+    HIPCHK(launch_refit(dl, ctx->regs.p, nf, nx, nz, ctx->stop_flag.p, st));
     // water-layer top-row fixes and the PmP head-wave post-pass
     std::vector<int> which, up, hw;
     for (int f = f0; f < f0 + nf; f++) {
@@ -502,6 +504,9 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     fim_ms += ms;
   }
   ctx->stats.ms_fim = fim_ms;
+  int stop = 0;
+  HIPCHK(hipMemcpy(&stop, ctx->stop_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (stop) return fail(FMT2D_EINVAL, "negative determinant too large in regular_update2d (the reference stops here)");
   ctx->stats.dispatches = (int)nev;
   unsigned long long acc[3 + 64];
   HIPCHK(hipMemcpy(acc, ctx->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
@@ -546,6 +551,33 @@ int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, i
 int fmt2d_forward_sweep_hist(fmt2d_ctx* ctx, long long* hist) {
   if (!ctx || !hist) return fail(FMT2D_EINVAL, "NULL argument");
   for (int k = 0; k < 64; k++) hist[k] = ctx->sweep_hist[k];
+  return FMT2D_OK;
+}
+
+int fmt2d_forward_summary(fmt2d_ctx* ctx, fmt2d_field_summary* out, int cap, int* n) {
+  if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
+  if (ctx->nsrc < 1) return fail(FMT2D_ESTATE, "no forward run");
+  *n = ctx->nfield;
+  if (!out || cap == 0) return FMT2D_OK;
+  if (cap < ctx->nfield) return fail(FMT2D_EINVAL, "output buffer too small");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nf = ctx->nfield;
+  DBuf<FieldSummary> d;
+  HIPCHK(d.alloc(nf));
+  HIPCHK(launch_field_summary(ctx->descs.p, ctx->regs.p, nf, ctx->m.nx, ctx->m.nz, d.p, ctx->stream));
+  std::vector<FieldSummary> h(nf);
+  HIPCHK(hipMemcpyAsync(h.data(), d.p, sizeof(FieldSummary) * nf, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < nf; k++) {
+    const FieldSummary& s = h[ctx->field_of[k]];
+    fmt2d_field_summary& o = out[k];
+    o.reached = s.reached;
+    o.hash = s.hash;
+    o.max_resid = s.max_resid;
+    o.mean_resid = s.n_resid > 0 ? s.sum_resid / (double)s.n_resid : 0.0;
+    o.n_resid = s.n_resid;
+    o.n_resid_big = s.n_resid_big;
+  }
   return FMT2D_OK;
 }
 

@@ -121,6 +121,7 @@ struct fmt2d_ctx {
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
+  DBuf<int> stop_flag;                // set when a time is NaN (the reference's stop case)
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;
   // heap-exhaustion stop work arrays

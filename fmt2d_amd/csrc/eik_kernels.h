@@ -94,7 +94,18 @@ hipError_t launch_exit_bins(const FieldDesc* descs, const RegionDev* regs, int n
 // gather + exact check (needs cap >= *total) + truncation of the fields
 hipError_t launch_exit_check(const FieldDesc* descs, const RegionDev* regs, int nfield, int nslot, ExitWork& w,
                              long long ncand_bound, hipStream_t st);
-hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, hipStream_t st);
+// also sets *stop_flag when a grid node's time is NaN (the reference's stop
+// on a negative discriminant too large, propagate2d.f90:745-756)
+hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, int* stop_flag,
+                        hipStream_t st);
+struct FieldSummary {
+  long long reached;         // slots with a time
+  unsigned long long hash;   // order-free hash of the bits of T and the output gradient
+  double max_resid, sum_resid;
+  long long n_resid, n_resid_big;  // nodes in the eikonal residual, residuals above 0.05
+};
+hipError_t launch_field_summary(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
+                                FieldSummary* out, hipStream_t st);
 hipError_t launch_water_fix(const FieldDesc* descs, const RegionDev* regs, const int* which, const int* up, int nsel,
                             int nx, hipStream_t st);
 hipError_t launch_headwave(const FieldDesc* descs, const RegionDev* regs, const int* which, int nsel,

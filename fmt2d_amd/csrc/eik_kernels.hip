@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
 // Post-pass: refit one-sided gradients of grid nodes; everything else keeps
 // the raw gradient.  One thread per (field, slot).
 __global__ void refit_gradients_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
-                                       int nfield, int nx, int nz) {
+                                       int nfield, int nx, int nz, int* stop_flag) {
   const int nslot = (nz + 2) * nx;
   const long long total = (long long)nfield * nslot;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
@@ -622,6 +622,9 @@ __global__ void refit_gradients_kernel(const FieldDesc* __restrict__ descs, cons
     double ox = d.GX[q], oz = d.GZ[q];
     if (q < nz * nx) {
       const int i = q % nx, j = q / nx;
+      // a time the update left as NaN: the reference stops there
+      // ("negative determinant too large", propagate2d.f90:750-753)
+      if (d.T[q] != d.T[q]) atomicOr(stop_flag, 1);
       if ((R.flags[q] & RF_IN) && q != d.fixed_key && d.T[q] < 0.5e20) {
         GView v{&R, &d};
         double t0, a, b;
@@ -701,6 +704,67 @@ __global__ void pmp_headwave_kernel(const FieldDesc* __restrict__ descs, const R
       if (vv > 0.0) { const double s = 1.0 / (vv * vv) - dtdx * dtdx; dtdz = s > 0.0 ? sqrt(s) : 0.0; }
       d.OGX[q] = dtdx; d.OGZ[q] = dtdz;
     }
+  }
+}
+
+// Per-field summary (diagnostics): reached slots, an order-free hash of the
+// bits of T and the output gradient over all slots, and the eikonal residual
+// | |grad T| v - 1 | over fully regular reached grid nodes whose four
+// neighbours are reached.  One block per field.
+__device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) field_summary_kernel(const FieldDesc* __restrict__ descs,
+                                                            const RegionDev* __restrict__ regs, int nx, int nz,
+                                                            FieldSummary* out) {
+  const int f = blockIdx.x;
+  const FieldDesc& d = descs[f];
+  const RegionDev& R = regs[d.region];
+  const int nslot = (nz + 2) * nx;
+  long long reached = 0, nres = 0, nbig = 0;
+  unsigned long long h = 0;
+  double rmax = 0.0, rsum = 0.0;
+  for (int q = threadIdx.x; q < nslot; q += blockDim.x) {
+    const double t = d.T[q];
+    const unsigned long long bt = (unsigned long long)__double_as_longlong(t);
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(d.OGX[q]);
+    const unsigned long long bz = (unsigned long long)__double_as_longlong(d.OGZ[q]);
+    h += mix64(bt ^ mix64(bx ^ mix64(bz ^ (unsigned long long)q)));
+    if (!(t < 0.5e20)) continue;
+    reached++;
+    if (q >= nz * nx) continue;
+    const int i = q % nx, j = q / nx;
+    if (!(R.flags[q] & RF_FULLREG) || i == 0 || j == 0 || i == nx - 1 || j == nz - 1) continue;
+    if (!(d.T[q - 1] < 0.5e20 && d.T[q + 1] < 0.5e20 && d.T[q - nx] < 0.5e20 && d.T[q + nx] < 0.5e20)) continue;
+    const double gx = d.OGX[q], gz = d.OGZ[q];
+    const double r = fabs(sqrt(gx * gx + gz * gz) * R.vel[q] - 1.0);
+    nres++;
+    rsum += r;
+    rmax = fmax(rmax, r);
+    if (r > 0.05) nbig++;
+  }
+  __shared__ long long s_reached[256], s_nres[256], s_nbig[256];
+  __shared__ unsigned long long s_h[256];
+  __shared__ double s_rmax[256], s_rsum[256];
+  const int k = threadIdx.x;
+  s_reached[k] = reached; s_nres[k] = nres; s_nbig[k] = nbig; s_h[k] = h; s_rmax[k] = rmax; s_rsum[k] = rsum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (k < w) {
+      s_reached[k] += s_reached[k + w]; s_nres[k] += s_nres[k + w]; s_nbig[k] += s_nbig[k + w];
+      s_h[k] += s_h[k + w]; s_rmax[k] = fmax(s_rmax[k], s_rmax[k + w]); s_rsum[k] += s_rsum[k + w];
+    }
+    __syncthreads();
+  }
+  if (k == 0) {
+    FieldSummary o;
+    o.reached = s_reached[0]; o.hash = s_h[0]; o.max_resid = s_rmax[0]; o.sum_resid = s_rsum[0];
+    o.n_resid = s_nres[0]; o.n_resid_big = s_nbig[0];
+    out[f] = o;
   }
 }
 
@@ -1062,10 +1126,18 @@ hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2*
   return hipGetLastError();
 }
 
-hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, hipStream_t st) {
+hipError_t launch_field_summary(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
+                                FieldSummary* out, hipStream_t st) {
+  if (nfield <= 0) return hipSuccess;
+  hipLaunchKernelGGL(field_summary_kernel, dim3(nfield), dim3(256), 0, st, descs, regs, nx, nz, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_refit(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz, int* stop_flag,
+                        hipStream_t st) {
   const long long total = (long long)nfield * (nz + 2) * nx;
   int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(refit_gradients_kernel, dim3(blocks), dim3(256), 0, st, descs, regs, nfield, nx, nz);
+  hipLaunchKernelGGL(refit_gradients_kernel, dim3(blocks), dim3(256), 0, st, descs, regs, nfield, nx, nz, stop_flag);
   return hipGetLastError();
 }
 

@@ -131,7 +131,8 @@ FMT_HD bool reg_root(double a, double b, double c, double tmax, double& t1) {
   c = c - 1.0;
   if (!(fabs(a) >= REG_TINY)) return false;
   double det = b * b - 4.0 * a * c;
-  if (det < 0) det = 0;
+  // as regular_cand: beyond 1e20 s the reference stops, here the time is NaN
+  if (det < 0) det = 0.5 * sqrt(-det) / fabs(a) < 1.0e20 ? 0.0 : (double)NAN;
   t1 = 0.5 * (-b + sqrt(det)) / a;
   if (tmax > t1) t1 = tmax;
   return true;
@@ -165,7 +166,7 @@ struct RegBest {
 
 FMT_HD void reg_add(RegBest& b, double T, int tie, int gsel, const RegTerm& tx, double xoff, const RegTerm& tz,
                     double zoff) {
-  if (T < b.T) {
+  if (T < b.T || T != T) {  // a NaN candidate (reg_root's stop case) sticks
     b.T = T; b.tie = tie; b.indep = tie == 0; b.gsel = gsel;
     b.xta = tx.ta; b.xt2 = tx.t2; b.xoff = xoff; b.zta = tz.ta; b.zt2 = tz.t2; b.zoff = zoff;
   } else if (T == b.T) {

@@ -68,7 +68,7 @@ struct MinVisitor {       // fixed-point iteration: keep the minimum
   // evaluated no earlier than the current minimum cannot lower it
   FMT_HD bool want(Real evt) const { return evt < b.T; }
   FMT_HD void add(Real T, Real gx, Real gz, int kind, int susp, int tie) {
-    if (T < b.T) {
+    if (T < b.T || T != T) {  // a NaN candidate (regular_cand's stop case) sticks
       b.T = T; b.gx = gx; b.gz = gz; b.reg_set = (kind == CK_REGULAR); if (kind == CK_REGULAR) b.susp = susp;
       b.tie = tie; b.indep = tie == 0;
     } else if (T == b.T) {
@@ -136,7 +136,10 @@ FMT_HD bool regular_cand(const V& v, int iu, int ju, const int* mx, const int* m
   c = c - (Real)1.0;
   if (!(fabs((double)a) >= 2.2250738585072014e-308)) return false;
   Real det = b * b - (Real)4.0 * a * c;
-  if (det < 0) det = 0;
+  // a negative discriminant is taken as 0, unless the root it would move is
+  // beyond 1e20 s, where the reference stops (:745-756): the time becomes
+  // NaN, which the forward reports as an error
+  if (det < 0) det = (Real)0.5 * sqrt(-det) / fabs(a) < (Real)1.0e20 ? (Real)0 : (Real)NAN;
   Real t1 = (Real)0.5 * (-b + sqrt(det)) / a;
   if (tmax > t1) t1 = tmax;
   T = t1;

@@ -39,6 +39,11 @@ class ForwardStats(C.Structure):
                 ("capped", C.c_longlong), ("max_sweeps", C.c_int), ("dispatches", C.c_int)]
 
 
+class FieldSummary(C.Structure):
+    _fields_ = [("reached", C.c_longlong), ("hash", C.c_ulonglong), ("max_resid", C.c_double),
+                ("mean_resid", C.c_double), ("n_resid", C.c_longlong), ("n_resid_big", C.c_longlong)]
+
+
 class SystemParams(C.Structure):
     _fields_ = [("crit_chi", C.c_double), ("phase_wt_psed", C.c_double), ("phase_wt_pmp", C.c_double),
                 ("phase_wt_pgss", C.c_double), ("refl_weight", C.c_double), ("alt_mode", C.c_int)]
@@ -102,6 +107,7 @@ def load() -> C.CDLL:
     lib.fmt2d_grid_info_get.argtypes = [C.c_void_p, C.POINTER(GridInfo)]
     lib.fmt2d_forward.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_forward_stats_get.argtypes = [C.c_void_p, C.POINTER(ForwardStats)]
+    lib.fmt2d_forward_summary.argtypes = [C.c_void_p, C.POINTER(FieldSummary), C.c_int, _ip]
     lib.fmt2d_forward_trace.argtypes = [C.c_void_p, _ip, _ip, C.c_int, _ip]
     lib.fmt2d_forward_sweep_hist.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     lib.fmt2d_set_tuning.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
@@ -138,7 +144,7 @@ def load() -> C.CDLL:
 # exported symbols declared in include/fmt2d_hip.h
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
-           "fmt2d_forward_sweep_hist",
+           "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
            "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap",
            "fmt2d_tf_region",
            "fmt2d_get_field",
@@ -211,6 +217,17 @@ class Context:
         st = ForwardStats()
         _check(self._lib.fmt2d_forward_stats_get(self._h, C.byref(st)))
         return st
+
+    def summary(self):
+        """Per-field summary of the last forward (fmt2d_forward_summary), as a
+        numpy structured array in the order isrc * 12 + itf - 1."""
+        n = C.c_int()
+        _check(self._lib.fmt2d_forward_summary(self._h, None, 0, C.byref(n)))
+        buf = (FieldSummary * n.value)()
+        _check(self._lib.fmt2d_forward_summary(self._h, buf, n.value, C.byref(n)))
+        dt = np.dtype([("reached", np.int64), ("hash", np.uint64), ("max_resid", np.float64),
+                       ("mean_resid", np.float64), ("n_resid", np.int64), ("n_resid_big", np.int64)])
+        return np.frombuffer(bytes(buf), dtype=dt, count=n.value).copy()
 
     def forward_trace(self):
         """(due tiles, most sweeps of a tile visit) per launch of the last forward."""

@@ -202,6 +202,16 @@ def multistage_case(dx: float = 0.1, nobs: int = 60):
     return m, g
 
 
+def multistage_phases_case(dx: float = 0.1, obs_index: Sequence[int] = (0, 30, 59)):
+    """The full multistage forward's model and shots (``multistage_case``)
+    with receiver lines of all five phases in the reference's windows for the
+    OBS ``obs_index`` of its 60 (x = 2.0 + 3.3 k km)."""
+    m = make_model(dx=dx)
+    obs = np.round(2.0 + 3.3 * np.asarray(obs_index, dtype=float), 6)
+    shots = np.round(0.1 + 0.2 * np.arange(1000), 6)
+    return m, phase_windows_geometry(m, obs, shots)
+
+
 def batched_case(dx: float = 0.1, nobs: int = 200, seed: int = 3):
     """The batched 1000-source-stage forward: 0.1 km grid (2001x301), sediment
     base and Moho perturbed by seeded random Fourier modes (wavelengths

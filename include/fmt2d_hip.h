@@ -131,6 +131,20 @@ int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, i
  * sweeps, hist[63] 64 or more (diagnostics). */
 int fmt2d_forward_sweep_hist(fmt2d_ctx* ctx, long long* hist);
 
+typedef struct {
+  long long reached;            /* slots of the field with a time (< 1e19) */
+  unsigned long long hash;      /* order-free hash of the bits of T, dT/dx, dT/dz over all slots */
+  double max_resid, mean_resid; /* | |grad T| v - 1 | over fully regular reached grid nodes
+                                   whose four neighbours are reached */
+  long long n_resid;            /* nodes in the residual */
+  long long n_resid_big;        /* of them, residuals above 0.05 */
+} fmt2d_field_summary;
+/* Summary of every field of the last forward, computed on the device, in
+ * the order isrc * 12 + itf - 1 (diagnostics of this build: checks over all
+ * fields of a large batch without reading them back; no counterpart in the
+ * reference).  With out == NULL or cap == 0 only *n. */
+int fmt2d_forward_summary(fmt2d_ctx* ctx, fmt2d_field_summary* out, int cap, int* n);
+
 /* Region id (1..6) of time field itf (1..12). */
 int fmt2d_tf_region(int itf);
 

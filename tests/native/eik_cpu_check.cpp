@@ -184,6 +184,9 @@ static int tile_schedule_run(const RegionDev& R, CpuField& F, int nx, int nz, lo
 
 extern "C" {
 
+// reg_root of the fast path (eik_regular.h), for the discriminant checks
+int cpu_reg_root(double a, double b, double c, double tmax, double* t1) { return reg_root(a, b, c, tmax, *t1) ? 1 : 0; }
+
 void cpu_fast_check_set(int on) {
   g_fast_check = on;
   g_fast_checked = 0;

@@ -116,3 +116,20 @@ def test_fast_regular_path_matches_operator(operator_lib, tmp_path, plain_case, 
             assert bad.value == 0, f"source {s}: {bad.value} of {checked.value} fast-path evaluations differ"
             total += checked.value
     assert total > 100000
+
+
+def test_negative_discriminant_rule(operator_lib):
+    """regular_update2d's negative discriminant (propagate2d.f90:745-756):
+    taken as 0 while 0.5 sqrt(|det|)/|a| < 1e20, else the reference stops;
+    the update then yields NaN, which fmt2d_forward reports as an error."""
+    lib = operator_lib
+    lib.cpu_reg_root.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    t = C.c_double()
+    # det = b^2 - 4 a (c - 1) = 4 - 4*2*2 = -12 < 0: clamped, root -b / 2a
+    assert lib.cpu_reg_root(2.0, -2.0, 3.0, 0.0, C.byref(t)) == 1 and t.value == 0.5
+    # tmax above the root wins
+    assert lib.cpu_reg_root(2.0, -2.0, 3.0, 0.75, C.byref(t)) == 1 and t.value == 0.75
+    # 0.5 sqrt(4e-20 * 1e30) / 1e-20 = 1e25 >= 1e20: the reference's stop
+    assert lib.cpu_reg_root(1e-20, 0.0, 1e30, 0.0, C.byref(t)) == 1 and np.isnan(t.value)
+    # a == 0: no update
+    assert lib.cpu_reg_root(0.0, 1.0, 1.0, 0.0, C.byref(t)) == 0

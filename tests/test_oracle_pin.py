@@ -131,3 +131,30 @@ def test_constant_gradient_first_arrival():
     rel = (T[g] - exact) / exact
     far = r > 10.0
     assert abs(np.median(rel[far])) < 6e-3 and np.abs(rel[far]).max() < 3e-2
+
+
+def test_headwave_post_pass_bit_exact(tmp_path):
+    """The PmP head-wave post-pass of run_forward2d (inversion2d.f90:4294-4409)
+    as the oracle restates it (pmp_headwave in oracle/fmm2d_oracle.c): tf5 of
+    two OBS of the 0.5 km model against the reference inversion's own saved
+    field (tests/golden/ref_headwave.npz), every traveltime and gradient bit
+    for bit; without the post-pass the same field differs, so the pin
+    reaches it."""
+    from oracle.pyoracle import OracleModel
+    g = np.load(os.path.join(ROOT, "tests", "golden", "ref_headwave.npz"))
+    for name in ("model.smesh", "moho.dat", "sed.dat"):
+        with open(tmp_path / name, "w") as f:
+            f.write(str(g["file_" + name.replace(".", "_")]))
+    om = OracleModel(*load_case(str(tmp_path)))
+    changed = 0
+    for k, s in enumerate(g["obs"]):
+        with_hw = om.forward_source(float(s), 12, headwave=True)
+        without = om.forward_source(float(s), 12, headwave=False)
+        for itf in (1, 5):
+            T, gx, gz = with_hw[itf - 1]
+            assert np.array_equal(T, g[f"s{k}_tf{itf}_T"], equal_nan=True), (s, itf)
+            assert np.array_equal(gx, g[f"s{k}_tf{itf}_gx"], equal_nan=True), (s, itf)
+            assert np.array_equal(gz, g[f"s{k}_tf{itf}_gz"], equal_nan=True), (s, itf)
+        T0, gx0, gz0 = without[4]
+        changed += int(np.sum((T0 != g[f"s{k}_tf5_T"]) | (gx0 != g[f"s{k}_tf5_gx"]) | (gz0 != g[f"s{k}_tf5_gz"])))
+    assert changed > 0

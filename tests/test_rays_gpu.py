@@ -87,3 +87,94 @@ def test_frechet_rows(traced):
         nbit += int(np.sum(val[a:b] == rval[ra:rb]))
     print(f"Frechet entries {len(rcol)}, bit-identical {nbit}")
     assert nbit == len(rcol)
+
+
+FINE = os.path.join(ROOT, "tests", "golden", "ref_rays_fine.npz")
+
+
+@pytest.fixture(scope="module")
+def traced_fine(tmp_path_factory):
+    """The 0.1 km grid (2001x301), three OBS of the 60-OBS multistage case,
+    all five phases (tests/golden/ref_rays_fine.npz, written by
+    tests/golden/make_ref_rays_fine.py from the reference inversion)."""
+    from fmt2d_amd import formats, hip, synth
+    Z = np.load(FINE)
+    d = str(tmp_path_factory.mktemp("fine"))
+    m, g = synth.multistage_phases_case()
+    synth.write_fmt2d_case(d, m, g)
+    sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+    sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+    mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+    geom = formats.read_geom(os.path.join(d, "geom.dat"))
+    ctx = hip.Context(0)
+    ctx.set_model(sm, sx, sz, mx, mz)
+    ctx.set_inversion(True, [3, 4])
+    print(f"[gpu] forward 0.1 km grid, sources {list(geom.obs_x)}, {len(geom.rec_x)} rays", flush=True)
+    ctx.forward(list(geom.obs_x))
+    t, v = ctx.trace(geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+    rp, col, val = ctx.frechet_csr(index_base=1)
+    yield ctx, Z, t, v, rp, col, val
+    ctx.close()
+
+
+def test_fine_grid_validity_and_times(traced_fine):
+    """Which rays exist at 0.1 km (the 5000-point section cap, head-wave and
+    interface rules) and their receiver times, every ray, bit for bit."""
+    ctx, Z, t, v, *_ = traced_fine
+    assert np.array_equal(v, Z["valid"]), f"{int(np.sum(v != Z['valid']))} rays differ in validity"
+    ok = Z["valid"]
+    assert np.array_equal(t[ok].view(np.int64), Z["t"][ok].view(np.int64)), \
+        f"max |dt| {np.max(np.abs(t[ok] - Z['t'][ok])):.3e}"
+    print(f"rays {len(v)}, valid {int(ok.sum())}; invalid per phase:",
+          {p: int(np.sum(~ok & (Z["path"] == p))) for p in range(1, 6)})
+    assert 0 < ok.sum() < len(ok)
+
+
+def test_fine_grid_points(traced_fine):
+    """Points per section of every valid ray identical; the point lists
+    within 1e-12 km (bit for bit but for the last bit where a step reads the
+    refit gradient of a sea-surface node of a water-layer stage, whose
+    least-squares fit rounds differently in its last bit: 1e-17 km)."""
+    ctx, Z, t, v, *_ = traced_fine
+    for k in np.flatnonzero(Z["valid"]):
+        assert np.array_equal(ctx.ray_info(int(k))["npts"], Z["npts"][k]), f"ray {k}: points per section differ"
+    n, npt, nbit, dmax = 0, 0, 0, 0.0
+    for key in Z.files:
+        if not key.startswith("pts_"):
+            continue
+        _, k, s = key.split("_")
+        x, z = ctx.ray_points(int(k), int(s))
+        got, ref = np.stack([x, z]), Z[key]
+        assert got.shape == ref.shape, f"ray {k} section {s}: {got.shape} points, reference {ref.shape}"
+        d = np.abs(got - ref).max()
+        assert d < 1e-12, f"ray {k} section {s}: points off by {d:.3e} km"
+        dmax = max(dmax, d)
+        npt += ref.shape[1]
+        nbit += int(np.sum(np.all(got == ref, axis=0)))
+        n += 1
+    print(f"{n} sections, {npt} points, bit-identical {nbit}, max |d| {dmax:.1e} km")
+    assert n > 10 and nbit > 0.99 * npt
+
+
+def test_fine_grid_frechet_rows(traced_fine):
+    """Frechet rows at 0.1 km: the sparsity pattern identical, entries within
+    1e-9 relative (bar: 1e-3); rays whose points differ in the last bit (see
+    above) carry that into their weights."""
+    ctx, Z, t, v, rp, col, val = traced_fine
+    rrp, rcol, rval = Z["rowptr"], Z["col"], Z["val"]
+    nrow, nent, nbit, rmax = 0, 0, 0, 0.0
+    for j, k in enumerate(Z["row_rays"]):
+        a, b = rp[k] - 1, rp[k + 1] - 1
+        ra, rb = rrp[j], rrp[j + 1]
+        if not Z["valid"][k]:
+            assert b == a
+            continue
+        assert np.array_equal(col[a:b], rcol[ra:rb]), f"ray {k}: Frechet sparsity differs"
+        rel = np.abs(val[a:b] - rval[ra:rb]) / np.maximum(np.abs(rval[ra:rb]), 1e-30)
+        assert rel.max(initial=0.0) < 1e-9, f"ray {k}: Frechet entry off by {rel.max():.2e} relative"
+        rmax = max(rmax, float(rel.max(initial=0.0)))
+        nent += rb - ra
+        nbit += int(np.sum(val[a:b] == rval[ra:rb]))
+        nrow += 1
+    print(f"{nrow} rows, {nent} entries, bit-identical {nbit}, max relative difference {rmax:.1e}")
+    assert nrow > 10 and nbit > 0.95 * nent
