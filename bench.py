@@ -6,8 +6,11 @@ forward pass of the multistage eikonal solver over all OBS of the workload:
 for every OBS the twelve time fields of the five phases (Pg, PsedP, PmP,
 Pg_ss, Pg_ss2), solved on the device with the inputs (model, geometry)
 already resident.  A source-stage is one (OBS, phase) chain, so a step
-completes 5 x nobs source-stages.  Default workload: the full multistage
-forward on the 0.1 km grid (2001x301 nodes), 60 OBS every 3.3 km.
+completes 5 x nobs source-stages.  Default workload: the batched forward on
+the 0.1 km grid (2001x301 nodes), 200 OBS every 1 km with the sediment base
+and Moho perturbed by seeded Fourier modes: 1000 source-stages (2400 fields)
+in one fmt2d_forward.  ``multistage`` (60 OBS every 3.3 km) is reported
+beside it.
 
 Second figure (``iteration``): seconds per inversion iteration = forward +
 rays + Frechet rows + system assembly + one LSQR solve (fixed damping, update
@@ -27,9 +30,10 @@ output file).
 
 Fourth figure (``gradient``): seconds for the misfit 1/2 |t - t_obs|^2 and
 its gradient with respect to the velocity grid and the Moho depths through
-fmt2d_amd.autograd (forward, rays, Frechet rows, adjoint product) at
-2001x301, 60 OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x
-3 km cells plus 30 ms seeded noise, starting model without it).
+fmt2d_amd.autograd (forward, rays, adjoint-state backward) at 2001x301, 60
+OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x 3 km cells plus
+30 ms seeded noise, starting model without it), with the ray-based
+backward (Frechet rows + G^T r) timed beside it.
 
 Fifth figure (``joint``, ``--joint N``, default 4, 0 skips): seconds per iteration of a
 joint velocity + interface inversion on an inversion mesh coarser than the
@@ -62,6 +66,11 @@ BYTES_PER_NODE = 32  # FP64: read velocity 8, write T 8, write grad T 16
 
 def workload(name: str):
     from fmt2d_amd import synth
+    if name == "batched":
+        m, g = synth.batched_case()
+        return m, g, ("batched forward, 2001x301 grid (0.1 km), 200 OBS every 1 km, sediment base and Moho "
+                      "perturbed by seeded Fourier modes (seed 3), 5 phases / 12 fields per OBS (1000 "
+                      "source-stages, 2400 fields in one fmt2d_forward)")
     if name == "multistage":
         m, g = synth.multistage_case(dx=0.1, nobs=60)
         return m, g, "multistage forward, 2001x301 grid (0.1 km), 60 OBS every 3.3 km, 5 phases / 12 fields per OBS"
@@ -288,7 +297,9 @@ def programs_figure(cpu: bool):
 
 def gradient_figure(steps: int):
     """Misfit and gradient at the adjoint-gradient workload (see module
-    docstring); the forward with rays and the backward timed apart."""
+    docstring) through fmt2d_amd.autograd: the forward with rays, the
+    adjoint-state backward (fmt2d_adjoint_gradient) and, for comparison, the
+    ray-based backward (Frechet rows + G^T r) timed apart."""
     import torch
     from fmt2d_amd import formats, hip, synth
     from fmt2d_amd.autograd import TravelTimes
@@ -308,39 +319,114 @@ def gradient_figure(steps: int):
         synth.write_fmt2d_case(d, m0, g)
         sm0 = formats.read_smesh(os.path.join(d, "model.smesh"))
         ctx.set_model(sm0, sx, sz, mx, mz)
-        tt = TravelTimes(ctx, geom.obs_x, geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z,
-                         invert_velocity=True, ifaces=[4])
-        vel = torch.tensor(ctx.model(0), dtype=torch.float64, device="cuda:0", requires_grad=True)
-        moho = torch.tensor(ctx.model(4), dtype=torch.float64, device="cuda:0", requires_grad=True)
-        fw, bw = [], []
-        for _ in range(steps):
-            vel.grad = None
-            moho.grad = None
-            t0 = time.perf_counter()
-            tc = tt(vel, moho)
-            ok = torch.isfinite(tc) & torch.isfinite(t_obs)
-            r = torch.where(ok, tc - torch.nan_to_num(t_obs), torch.zeros_like(tc))
-            misfit = 0.5 * torch.sum(r * r)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            misfit.backward()
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            fw.append(t1 - t0)
-            bw.append(t2 - t1)
-        n_ok = int(ok.sum())
-        gn = float(torch.linalg.vector_norm(vel.grad))
+        rays = (geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z)
+        res = {}
+        for mode in ("adjoint", "rays"):
+            tt = TravelTimes(ctx, geom.obs_x, *rays, invert_velocity=True, ifaces=[4], backward=mode)
+            vel = torch.tensor(ctx.model(0), dtype=torch.float64, device="cuda:0", requires_grad=True)
+            moho = torch.tensor(ctx.model(4), dtype=torch.float64, device="cuda:0", requires_grad=True)
+            fw, bw, stats = [], [], []
+            for _ in range(steps):
+                vel.grad = None
+                moho.grad = None
+                t0 = time.perf_counter()
+                tc = tt(vel, moho)
+                ok = torch.isfinite(tc) & torch.isfinite(t_obs)
+                r = torch.where(ok, tc - torch.nan_to_num(t_obs), torch.zeros_like(tc))
+                misfit = 0.5 * torch.sum(r * r)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                misfit.backward()
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                fw.append(t1 - t0)
+                bw.append(t2 - t1)
+                stats.append(tt.last_stats)
+            res[mode] = dict(fw=float(np.mean(fw)), bw=float(np.mean(bw)), stats=stats[-1], n_ok=int(ok.sum()),
+                             misfit=float(misfit.detach()), gn=float(torch.linalg.vector_norm(vel.grad)),
+                             gz=float(torch.linalg.vector_norm(moho.grad)))
         ctx.close()
-    return {"metric": "misfit + gradient seconds", "value": round(float(np.mean(fw) + np.mean(bw)), 4),
-            "unit": "s", "higher_is_better": False,
+    a, b = res["adjoint"], res["rays"]
+    st = a["stats"]
+    return {"metric": "misfit + gradient seconds", "value": round(a["fw"] + a["bw"], 4),
+            "unit": "s", "higher_is_better": False, "backward": "adjoint state",
             "workload": "2001x301 grid (0.1 km), 60 OBS, 1000 shots, Pg at every offset and PmP at 3-30 km; "
-                        "gradient with respect to the velocity grid and the Moho depths",
-            "forward_rays_s": round(float(np.mean(fw)), 4), "backward_s": round(float(np.mean(bw)), 4),
-            "rays": len(geom.rec_x), "rays_with_picks_and_times": n_ok, "n_par": tt.n_par,
-            "misfit": float(misfit.detach()), "grad_norm_velocity": gn}
+                        "gradient with respect to every grid node's velocity and every Moho node's depth",
+            "forward_rays_s": round(a["fw"], 4), "backward_s": round(a["bw"], 4),
+            "adjoint": {"fields": st.fields, "rounds": st.rounds, "record_ms": round(st.ms_record, 1),
+                        "sweep_ms": round(st.ms_reverse, 1), "total_ms": round(st.ms_total, 1),
+                        "peak_gib": round(st.peak_bytes / 2**30, 3), "unmatched_nodes": st.leaves_unmatched},
+            "ray_backward_s": round(b["bw"], 4),
+            "ray_backward_note": "Frechet rows of compute_frechet_for_ray2d + G^T r (the reference's linearisation), "
+                                 "inversion parameters only",
+            "rays": len(geom.rec_x), "rays_with_picks_and_times": a["n_ok"],
+            "misfit": a["misfit"], "grad_norm_velocity": a["gn"], "grad_norm_moho": a["gz"]}
 
 
-def joint_figure(iters: int):
+def joint_cpu_sample(m0, g, t_obs, keep_obs=(50, 150)):
+    """The reference inversion (oracle/_ref/invert2d, one core) on a bounded
+    sample of the joint workload: the same starting model, grid and picks for
+    two of its 200 OBS, its own parameterisation (the propagation grid: the
+    reference cannot run the coarser mesh), velocity + sediment base + Moho.
+    Timed: its first forward with Frechet rows (its own "forward breakdown"
+    print), which scales with the OBS count.  The run is stopped there: on
+    the 2001x301 grid the reference's LSQR set-up sorts the 542 k column
+    norms with an O(n^2) exchange sort (sort_array, inversion2d.f90:6297-6312,
+    called at :479), hours at this size, so no LSQR time stands beside it."""
+    from fmt2d_amd import synth
+    exe = os.path.join(ROOT, "oracle", "_ref", "invert2d")
+    if not os.path.exists(exe):
+        return None
+    recs, k = [], 0
+    for i, rl in enumerate(g.receivers):
+        n = len(rl)
+        if i in keep_obs:
+            recs.append([(x, z, rt, float(t_obs[k + j]), u) for j, (x, z, rt, t, u) in enumerate(rl)])
+        k += n
+    gs = synth.Geometry(obs_x=g.obs_x[list(keep_obs)], obs_z=g.obs_z[list(keep_obs)], receivers=recs)
+    line = None
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_fmt2d_case(d, m0, gs)
+        with open(os.path.join(d, "frechet.in"), "w") as f:
+            f.write("1\n1\n1\n1\n2\n3 4\n")
+        with open(os.path.join(d, "invert2d.in"), "w") as f:
+            f.write("model.smesh\nmoho.dat\nsed.dat\ngeom.dat\n")
+            for v in [1, 0.1, 0.3, 1.0, 400, 0.03, 0.1, 0.3, 10.0, 1.0, 1.0, 1.0, 10.0, 3.0]:
+                f.write(f"{v}\n")
+            f.write("3 0.5 0.5 0.3\n0\n8.0\n")
+        cmd = [exe]
+        if os.path.exists("/usr/bin/taskset"):
+            cmd = ["/usr/bin/taskset", "-c", "0", exe]
+        t0 = time.perf_counter()
+        proc = subprocess.Popen(cmd, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+        try:
+            for ln in proc.stdout:
+                if "forward breakdown" in ln:
+                    line = ln
+                    break
+                if time.perf_counter() - t0 > 600:
+                    break
+        finally:
+            proc.kill()
+            proc.wait()
+        wall = time.perf_counter() - t0
+    if line is None:
+        return {"error": "no forward breakdown from the reference within 600 s"}
+    nums = [float(x) for x in line.replace(",", " ").replace("=", " ").split() if x.replace(".", "", 1).isdigit()]
+    fmm, trace, fre = nums[:3]
+    scale = len(g.obs_x) / len(keep_obs)
+    compiler = open(os.path.join(ROOT, "oracle", "_ref", "COMPILER")).read().strip()
+    return {"value": round((fmm + trace + fre) * scale, 2), "unit": "s per forward with Frechet rows (estimate)",
+            "cores": 1, "kind": "reference",
+            "sample": f"reference inversion ({compiler}, 1 core) on {len(keep_obs)} of the {len(g.obs_x)} OBS "
+                      f"(x = {', '.join(str(float(g.obs_x[i])) for i in keep_obs)} km), same grid, model and "
+                      f"picks, its own parameterisation (propagation grid): first forward with Frechet rows "
+                      f"FMM {fmm:.2f} s + rays {trace:.2f} s + Frechet {fre:.2f} s, scaled by {scale:.0f} to all "
+                      f"OBS; stopped after it ({wall:.1f} s): its LSQR set-up sorts 542 k column norms with an "
+                      f"O(n^2) exchange sort (inversion2d.f90:6297-6312), hours at this size"}
+
+
+def joint_figure(iters: int, cpu: bool = False):
     """Joint velocity + interface inversion on an inversion mesh coarser than
     the grid (see module docstring): the iterations of the alternating
     schedule, each phase timed."""
@@ -398,6 +484,21 @@ def joint_figure(iters: int):
         print(f"joint iteration {it}: {log[-1]}", file=sys.stderr, flush=True)
     ctx.close()
     tot = [sum(v) for v in zip(*phases.values())]
+    # LSQR: one solve per iteration (fixed damping); algorithmic bytes of an
+    # iteration: G v and G^T u each read every non-zero (FP64 value + int32
+    # index), plus the vectors of the hybrid system (3 m_aug + 8 n doubles)
+    lsqr_iters = sum(e["lsqr_iterations"] for e in log)
+    lsqr_s = float(np.sum(phases["lsqr"]))
+    nnz_m = float(np.mean([e["nnz"] for e in log]))
+    nd_m = float(np.mean([e["ndata"] for e in log]))
+    m_aug = nd_m + 2 * n_par
+    lsqr_bytes = 2 * nnz_m * 12 + (3 * m_aug + 8 * n_par) * 8
+    lsqr_ms_it = 1e3 * lsqr_s / max(lsqr_iters, 1)
+    lsqr_fig = {"iterations": lsqr_iters, "ms_per_iteration": round(lsqr_ms_it, 3),
+                "algorithmic_bytes_per_iteration": int(lsqr_bytes),
+                "achieved_GBps": round(lsqr_bytes / (lsqr_ms_it * 1e-3) / 1e9, 2),
+                "frac_of_hbm_peak": round(lsqr_bytes / (lsqr_ms_it * 1e-3) / 1e9 / PEAK_HBM_GBS, 5)}
+    cpu_b = joint_cpu_sample(m0, g, t_obs) if cpu else None
     return {"metric": "joint inversion seconds per iteration", "value": round(float(np.mean(tot)), 3), "unit": "s",
             "higher_is_better": False,
             "workload": "2001x301 grid (0.1 km), 200 OBS every 1 km, 1000 shots every 0.2 km, five phases in the "
@@ -406,7 +507,42 @@ def joint_figure(iters: int):
                         "hybrid smoothing, damping 0.1, LSQR at most 400 iterations, line search",
             "rays": int(len(t_true)), "picks": int((t_obs >= 0).sum()), "mesh": [vnx, vnz], "n_par": n_par,
             "n_frechet_columns": n_fre,
-            "phase_seconds": {k: round(float(np.mean(v)), 3) for k, v in phases.items()}, "log": log}
+            "phase_seconds": {k: round(float(np.mean(v)), 3) for k, v in phases.items()}, "lsqr": lsqr_fig,
+            "cpu_baseline": cpu_b, "log": log}
+
+
+def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int):
+    """Source-stages per second of one workload's forward (the headline's
+    measurement; see the module docstring), with the tile-kernel time from
+    HIP events on the solver stream and, when cpu_sample > 0, the reference
+    timed on that many of the workload's OBS."""
+    from fmt2d_amd import hip, formats
+    m, g, desc = workload(workload_name)
+    with tempfile.TemporaryDirectory() as d:
+        write_case(d, m, g)
+        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+        sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+        mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+        ctx = hip.Context(0)
+        ctx.set_model(sm, sx, sz, mx, mz)
+        obs = list(g.obs_x)
+        for _ in range(warmup):
+            ctx.forward(obs)
+        walls, fims, nodes, launches, tiles, disp = [], [], 0, [], [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ctx.forward(obs)
+            walls.append(time.perf_counter() - t0)
+            st = ctx.stats()
+            fims.append(st.ms_fim / 1e3)
+            nodes = st.region_nodes
+            launches.append(st.launches)
+            tiles.append(st.tiles)
+            disp.append(st.dispatches)
+        cpu = cpu_baseline(d, obs, cpu_sample) if cpu_sample > 0 else None
+        ctx.close()
+    return dict(desc=desc, obs=obs, walls=walls, fims=fims, nodes=nodes, launches=launches, tiles=tiles, disp=disp,
+                cpu=cpu)
 
 
 def main():
@@ -414,8 +550,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="multistage", choices=["multistage", "plain"])
-    ap.add_argument("--cpu-sample", type=int, default=4, help="OBS timed on the host CPU (0: skip)")
+    ap.add_argument("--workload", default="batched", choices=["batched", "multistage", "plain"])
+    ap.add_argument("--cpu-sample", type=int, default=12, help="OBS timed on the host CPU (0: skip)")
+    ap.add_argument("--no-multistage", action="store_true", help="skip the 60-OBS multistage figure")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
@@ -444,7 +581,7 @@ def main():
         sys.exit(2)
 
     if args.joint_only:
-        print(json.dumps({"joint": joint_figure(max(args.joint, 1))}))
+        print(json.dumps({"joint": joint_figure(max(args.joint, 1), args.cpu_sample > 0)}))
         return
 
     if args.iteration_only:
@@ -452,36 +589,23 @@ def main():
                                                            args.cpu_sample > 0)}))
         return
 
-    from fmt2d_amd import hip, formats
-    m, g, desc = workload(args.workload)
-    with tempfile.TemporaryDirectory() as d:
-        write_case(d, m, g)
-        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
-        sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
-        mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
-        ctx = hip.Context(0)
-        ctx.set_model(sm, sx, sz, mx, mz)
-        obs = list(g.obs_x)
-        for _ in range(args.warmup):
-            ctx.forward(obs)
-        walls, fims, nodes, launches, tiles, disp = [], [], 0, [], [], []
-        for _ in range(args.steps):
-            t0 = time.perf_counter()
-            ctx.forward(obs)
-            walls.append(time.perf_counter() - t0)
-            st = ctx.stats()
-            fims.append(st.ms_fim / 1e3)
-            nodes = st.region_nodes
-            launches.append(st.launches)
-            tiles.append(st.tiles)
-            disp.append(st.dispatches)
-        cpu = cpu_baseline(d, obs, args.cpu_sample) if args.cpu_sample > 0 else None
-        ctx.close()
-
+    head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample)
+    print(f"headline forward done: {head['walls']}", file=sys.stderr, flush=True)
+    ms = None
+    if args.workload != "multistage" and not args.no_multistage:
+        f = forward_figure("multistage", max(1, min(args.steps, 3)), 1, 0)
+        w = float(np.mean(f["walls"]))
+        ms = {"metric": "eikonal source-stages/s", "value": round(5 * len(f["obs"]) / w, 3),
+              "unit": "source-stages/s", "workload": f["desc"], "ms_per_step": round(1e3 * w, 3),
+              "solver_launches_per_step": int(np.mean(f["launches"])), "tiles_per_step": int(np.mean(f["tiles"])),
+              "fim_kernel_ms_per_step": round(1e3 * float(np.mean(f["fims"])), 3)}
+        print(f"multistage figure done: {ms['value']}", file=sys.stderr, flush=True)
     it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
     progs = None if args.no_programs else programs_figure(args.cpu_sample > 0)
     grad = None if args.no_gradient else gradient_figure(max(1, min(args.steps, 2)))
-    joint = joint_figure(args.joint) if args.joint > 0 else None
+    joint = joint_figure(args.joint, args.cpu_sample > 0) if args.joint > 0 else None
+    walls, fims, nodes, disp = head["walls"], head["fims"], head["nodes"], head["disp"]
+    obs = head["obs"]
     wall = float(np.sum(walls))
     stages = 5 * len(obs)
     value = stages * args.steps / wall
@@ -505,10 +629,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded model generator of the FMT2D model family)",
-        "config": {"workload": desc, "nobs": len(obs), "source_stages_per_step": stages,
+        "config": {"workload": head["desc"], "nobs": len(obs), "source_stages_per_step": stages,
                    "fields_per_step": 12 * len(obs), "region_nodes_per_step": int(nodes),
-                   "solver_launches_per_step": int(np.mean(launches)),
-                   "tiles_per_step": int(np.mean(tiles)), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3),
+                   "solver_launches_per_step": int(np.mean(head["launches"])),
+                   "tiles_per_step": int(np.mean(head["tiles"])), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3),
                    "fim_kernel_dispatches_per_step": int(ndisp),
                    "fim_kernel_us_per_dispatch": round(1e6 * fim_s / ndisp, 2)},
         "roofline": {"bound": "hbm", "kernel": "fim_tiles_kernel", "achieved": round(achieved, 3),
@@ -517,7 +641,8 @@ def main():
                      "algorithmic_bytes": f"{BYTES_PER_NODE} B per region node per field (FP64 velocity "
                                           f"read, T and grad T written) = {alg_bytes} B per step, "
                                           f"{alg_bytes / ndisp:.0f} B per dispatch"},
-        "cpu_baseline": cpu,
+        "cpu_baseline": head["cpu"],
+        "multistage": ms,
         "iteration": it,
         "programs": progs,
         "gradient": grad,

@@ -24,7 +24,7 @@ F_MODULES = ["fmt2d_hip_mod.f90", "fmt2d_io.f90"]
 F_PROGRAMS = {"fmt2d_hip.f90": FMT2D, "invert2d_hip.f90": INVERT2D}
 
 ARCH = os.environ.get("FMT2D_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["eik_kernels.hip", "rays.hip", "csr.hip", "lsqr.hip", "noalloc.hip"]
+HIP_SOURCES = ["eik_kernels.hip", "rays.hip", "csr.hip", "lsqr.hip", "noalloc.hip", "adjoint.hip"]
 HOST_SOURCES = ["capi.cpp", "capi_rays.cpp", "capi_lsqr.cpp", "capi_step.cpp", "model.cpp"]
 # no contraction into fused multiply-adds: the eikonal operator reproduces the
 # reference's FP64 rounding

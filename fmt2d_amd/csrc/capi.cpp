@@ -160,6 +160,7 @@ int model_upload(fmt2d_ctx* ctx) {
     const int n = moho.n;
     std::vector<int> slot(n, -1);
     std::vector<double> vhw(n, 0.0);
+    std::vector<int> vslot(n, -1);  // region-6 slot vhw was read from
     for (int c = 0; c < n; c++) {
       const int mref = R4.ngnode + c + 1;  // 1-based node id
       if (mref < 1 || mref > R6.nnode) continue;
@@ -175,16 +176,20 @@ int model_upload(fmt2d_ctx* ctx) {
             for (int jj = 0; jj <= 1; jj++) {
               const int ixx = i3 + ii, izz = ccz + jj;
               if (ixx < nx && izz < nz && m.node_region[(size_t)izz * nx + ixx] == 6) {
-                if (m.zg[izz] > moho.z[c] + 0.5 * m.dz)
-                  vhw[c] = std::max(vhw[c], R6.vel[(size_t)izz * nx + ixx]);
+                if (m.zg[izz] > moho.z[c] + 0.5 * m.dz && R6.vel[(size_t)izz * nx + ixx] > vhw[c]) {
+                  vhw[c] = R6.vel[(size_t)izz * nx + ixx];
+                  vslot[c] = izz * nx + ixx;
+                }
               }
             }
         }
       } else {
         vhw[c] = R6.vel[q];
+        vslot[c] = q;
       }
     }
     HIPCHK(upload(ctx->hw_slot, slot, st));
+    HIPCHK(upload(ctx->hw_vslot, vslot, st));
     HIPCHK(upload(ctx->hw_vhw, vhw, st));
     HeadwaveMap hm;
     hm.n = n; hm.dxif = moho.x[1] - moho.x[0]; hm.slot = ctx->hw_slot.p; hm.vhw = ctx->hw_vhw.p;
@@ -334,6 +339,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     }
   }
   ctx->stats = fmt2d_forward_stats{};
+  ctx->hw_fields.clear();
   ctx->trace.clear();
   ctx->trace_sweeps.clear();
   ctx->stats.nsrc = nsrc;
@@ -490,7 +496,10 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     if (!hw.empty()) {
       HIPCHK(upload(ctx->sel_which, hw, st));
       if (ctx->hw_scratch.n < (size_t)hw.size() * 2 * nx) HIPCHK(ctx->hw_scratch.alloc((size_t)hw.size() * 2 * nx));
-      HIPCHK(launch_headwave(dl, ctx->regs.p, ctx->sel_which.p, (int)hw.size(), ctx->hw_map.p, ctx->hw_scratch.p, st));
+      if (ctx->hw_save.n < (size_t)hw.size() * nx) HIPCHK(ctx->hw_save.alloc((size_t)hw.size() * nx));
+      HIPCHK(launch_headwave(dl, ctx->regs.p, ctx->sel_which.p, (int)hw.size(), ctx->hw_map.p, ctx->hw_scratch.p,
+                             ctx->hw_save.p, st));
+      for (int w : hw) ctx->hw_fields.push_back(f0 + w);
       HIPCHK(hipStreamSynchronize(st));
     }
   }

@@ -99,6 +99,7 @@ struct fmt2d_ctx {
   int fields_flags = 0;
   DBuf<int> hw_slot;
   DBuf<double> hw_vhw;
+  DBuf<int> hw_vslot;                 // region-6 velocity slot of each hw_vhw entry (-1: none)
   DBuf<HeadwaveMap> hw_map;
   // fields of the last forward
   int nsrc = 0, nfield = 0;
@@ -120,10 +121,13 @@ struct fmt2d_ctx {
   int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
+  size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> stop_flag;                // set when a time is NaN (the reference's stop case)
   DBuf<int> sel_which, sel_up;
   DBuf<double> hw_scratch;
+  DBuf<double> hw_save;               // Moho-node times of the PmP up-going fields before the head-wave pass
+  std::vector<int> hw_fields;         // their field indices, in hw_save's order
   // heap-exhaustion stop work arrays
   DBuf<ExitField> x_ef;
   DBuf<int> x_cnt, x_diff, x_off, x_total;

@@ -108,7 +108,11 @@ hipError_t launch_field_summary(const FieldDesc* descs, const RegionDev* regs, i
                                 FieldSummary* out, hipStream_t st);
 hipError_t launch_water_fix(const FieldDesc* descs, const RegionDev* regs, const int* which, const int* up, int nsel,
                             int nx, hipStream_t st);
+// save (nsel x hm->n, may be NULL): the Moho-node times before the pass
 hipError_t launch_headwave(const FieldDesc* descs, const RegionDev* regs, const int* which, int nsel,
-                           const HeadwaveMap* hm, double* scratch, hipStream_t st);
+                           const HeadwaveMap* hm, double* scratch, double* save, hipStream_t st);
+// exchanges the Moho-node times of fields which[0..nsel) with save
+hipError_t launch_headwave_swap(const FieldDesc* descs, const int* which, int nsel, const HeadwaveMap* hm, int n,
+                                double* save, hipStream_t st);
 
 }  // namespace fmt2d

@@ -59,6 +59,10 @@ FMT_HD int tie_dir(int di, int dj) {
 // time (the time of the member whose pop triggers it = the largest member
 // time) before it is computed, so a visitor can skip it cheaply.
 enum { CK_REGULAR = 1, CK_POINT = 2 };
+// What an offered update is, announced (note) just before its evaluation
+// time (want): for visitors that need to know which update a value came
+// from (the adjoint); the others ignore it.
+enum { CN_REG1 = 1, CN_REG2 = 2, CN_C1 = 3, CN_C2 = 4, CN_PARTNER = 5 };
 
 template <typename Real>
 struct MinVisitor {       // fixed-point iteration: keep the minimum
@@ -67,6 +71,8 @@ struct MinVisitor {       // fixed-point iteration: keep the minimum
   // member time, T1 + dist/v, the acceptance atime >= T(node1)), so an update
   // evaluated no earlier than the current minimum cannot lower it
   FMT_HD bool want(Real evt) const { return evt < b.T; }
+  template <class M>
+  FMT_HD void note(int, const M&, const M&) const {}
   FMT_HD void add(Real T, Real gx, Real gz, int kind, int susp, int tie) {
     if (T < b.T || T != T) {  // a NaN candidate (regular_cand's stop case) sticks
       b.T = T; b.gx = gx; b.gz = gz; b.reg_set = (kind == CK_REGULAR); if (kind == CK_REGULAR) b.susp = susp;
@@ -98,7 +104,7 @@ FMT_HD bool regular_cand(const V& v, int iu, int ju, const int* mx, const int* m
   for (int n = 0; n < nm; n++) {
     const int ia = mx[n], ja = mz[n];
     const Real ta = v.T(ia, ja), va = v.vel(ia, ja);
-    tmax = ta > tmax ? ta : tmax;
+    if (ta > tmax) tmax = ta;
     const bool isx = (ia != iu);
     const int is = isx ? iu + 2 * (ia - iu) : ia;
     const int js = isx ? ja : ju + 2 * (ja - ju);
@@ -171,6 +177,23 @@ FMT_HD void cand1(Real xu, Real zu, Real vu, Real x1, Real z1, Real t1, Real g1x
   vis.add(at, dx / (dist * vu), dz / (dist * vu), CK_POINT, 0, 0);
 }
 
+// One root of time_from_2_nodes2d: the slowness vector p + u q, kept when it
+// points into the wedge of the two members and gives a time no earlier than
+// the later member's (src/propagate2d.f90:903-942).
+template <typename Real, class Vis>
+FMT_HD void cand2_root(const Real& uu, const Real& p0, const Real& p1, const Real& q0, const Real& q1,
+                       const Real& n10, const Real& n11, const Real& n20, const Real& n21, const Real& r10,
+                       const Real& r11, const Real& ta, const Real& va, const Real& vm, Vis& vis) {
+  const Real w0 = p0 + uu * q0, w1 = p1 + uu * q1;
+  if (w0 * n10 + w1 * n11 <= 0 && w0 * n20 + w1 * n21 <= 0) {
+    const Real at = ta - (w0 * r10 + w1 * r11) / vm;
+    if (at >= ta) {
+      const Real wl = sqrt(w0 * w0 + w1 * w1);
+      vis.add(at, w0 / (wl * va), w1 / (wl * va), CK_POINT, 0, 0);
+    }
+  }
+}
+
 // time_from_2_nodes2d (src/propagate2d.f90:831-946)
 template <typename Real, class Vis>
 FMT_HD void cand2(Real xu, Real zu, Real vu,
@@ -213,17 +236,11 @@ FMT_HD void cand2(Real xu, Real zu, Real vu,
   n20 /= l2; n21 /= l2;
   if (n20 * r10 + n21 * r11 < 0) { n20 = -n20; n21 = -n21; }
   const Real vm = (Real)0.5 * (vu + va);
-  for (int k = 0; k < 2; k++) {
-    const Real uu = k == 0 ? u1 : u2;
-    const Real w0 = p0 + uu * q0, w1 = p1 + uu * q1;
-    if (w0 * n10 + w1 * n11 <= 0 && w0 * n20 + w1 * n21 <= 0) {
-      const Real at = ta - (w0 * r10 + w1 * r11) / vm;
-      if (at >= ta) {
-        const Real wl = sqrt(w0 * w0 + w1 * w1);
-        vis.add(at, w0 / (wl * va), w1 / (wl * va), CK_POINT, 0, 0);
-      }
-    }
-  }
+  // the two roots in order, each by a call of its own (no selection between
+  // two local values: the device compiler has mis-selected such a pair for
+  // class-typed Real)
+  cand2_root<Real>(u1, p0, p1, q0, q1, n10, n11, n20, n21, r10, r11, ta, va, vm, vis);
+  cand2_root<Real>(u2, p0, p1, q0, q1, n10, n11, n20, n21, r10, r11, ta, va, vm, vis);
 }
 
 // member of an irregular cell: grid node (kind 0) or interface node (kind 1+s)
@@ -290,11 +307,13 @@ FMT_HD void irregular_cell(const V& v, int cx, int cz, const Member& self, const
     const Member A = slot_member(cx, cz, lowest_slot(ma));
     Real xa, za, ta, gax, gaz, va;
     member_data<Real>(v, A, xa, za, ta, gax, gaz, va);
+    vis.note(CN_C1, A, A);
     if (!v.fixed(A) && vis.want(ta)) cand1<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, tol, vis);
     for (unsigned mc = ma & (ma - 1); mc; mc &= mc - 1) {
       const Member Cm = slot_member(cx, cz, lowest_slot(mc));
       Real xc, zc, tc, gcx, gcz, vc;
       member_data<Real>(v, Cm, xc, zc, tc, gcx, gcz, vc);
+      vis.note(CN_C2, A, Cm);
       if (vis.want(ta > tc ? ta : tc))
         cand2<Real>(xu, zu, vu, xa, za, ta, gax, gaz, va, xc, zc, tc, gcx, gcz, vc, tol, vis);
     }
@@ -331,6 +350,7 @@ FMT_HD void regular_cell(const V& v, int iu, int ju, int cx, int cz, Vis& vis) {
     bool xfirst = hx && (!hz || v.T(xn, ju) <= v.T(iu, zn));
     const int fi = xfirst ? xn : iu, fj = xfirst ? ju : zn;
     Member fm; fm.kind = 0; fm.i = fi; fm.j = fj;
+    vis.note(CN_REG1, fm, fm);
     if (!v.fixed(fm) && vis.want((Real)v.T(fi, fj))) {
       mx[0] = fi; mz[0] = fj;
       if (regular_cand<Real>(v, iu, ju, mx, mz, 1, T, gx, gz, susp)) {
@@ -344,6 +364,9 @@ FMT_HD void regular_cell(const V& v, int iu, int ju, int cx, int cz, Vis& vis) {
     const bool zlater = v.T(iu, zn) >= v.T(xn, ju);
     mx[0] = zlater ? iu : xn; mz[0] = zlater ? zn : ju;
     mx[1] = zlater ? xn : iu; mz[1] = zlater ? ju : zn;
+    Member la; la.kind = 0; la.i = mx[0]; la.j = mz[0];
+    Member ea; ea.kind = 0; ea.i = mx[1]; ea.j = mz[1];
+    vis.note(CN_REG2, la, ea);
     if (vis.want((Real)v.T(mx[0], mz[0])) && regular_cand<Real>(v, iu, ju, mx, mz, 2, T, gx, gz, susp)) {
       int tie;
       if (regular_tie<Real>(v, iu, ju, mx, mz, 2, T, tie)) vis.add(T, gx, gz, CK_REGULAR, susp, tie);
@@ -440,6 +463,8 @@ FMT_HD void grid_candidates(const V& v, int iu, int ju, Vis& vis) {
   own_grid_candidates<Real>(v, iu, ju, excl, vis);
   if (ps >= 0) {
     MinVisitor<Real> p = iface_own_min<Real>(v, ps, iu, iu, ju);
+    Member pm; pm.kind = 1 + ps; pm.i = iu; pm.j = 0;
+    vis.note(CN_PARTNER, pm, pm);
     if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0, 0);
   }
 }
@@ -456,6 +481,8 @@ FMT_HD void iface_candidates(const V& v, int s, int c, Vis& vis) {
   own_iface_candidates<Real>(v, s, c, excl, vis);
   if (pj >= 0) {
     MinVisitor<Real> p = grid_own_min<Real>(v, c, pj, s);
+    Member pm; pm.kind = 0; pm.i = c; pm.j = pj;
+    vis.note(CN_PARTNER, pm, pm);
     if (p.b.T < (Real)FMT_HUGE_HALF && vis.want(p.b.T)) vis.add(p.b.T, p.b.gx, p.b.gz, CK_POINT, 0, 0);
   }
 }
@@ -548,6 +575,8 @@ struct EvtCollector {
     e[k] = t; n++;
     return false;
   }
+  template <class M>
+  FMT_HD void note(int, const M&, const M&) const {}
   FMT_HD void add(Real, Real, Real, int, int, int) {}
 };
 
@@ -557,6 +586,8 @@ struct ReplayVisitor {
   Real evt, T;
   int susp;
   FMT_HD bool want(Real t) const { return t == evt; }
+  template <class M>
+  FMT_HD void note(int, const M&, const M&) const {}
   FMT_HD void add(Real t, Real, Real, int kind, int s, int) {
     if (t < T) { T = t; if (kind == CK_REGULAR) susp = s; }
   }

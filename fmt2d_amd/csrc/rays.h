@@ -81,6 +81,12 @@ struct RayOut {
 
 hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, int nsrc, int nsf, double* tsf,
                                  hipStream_t st);
+// adjoint sources of the receiver times (weights g per ray) in the
+// adjoint fields lam[field_of[src*12 + itf-1]][slot]; active[f] set for
+// fields that receive one
+hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, const RayOut* out,
+                                  const double* g, int nray, const int* field_of, long long nslot, double* lam,
+                                  int* active, hipStream_t st);
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
                         const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
                         RayOut* out, int program, hipStream_t st);

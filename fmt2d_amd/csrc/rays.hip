@@ -810,6 +810,64 @@ hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, in
   return hipGetLastError();
 }
 
+// Adjoint sources of the receiver times.  A valid ray's time is
+// interp_time(last stage field, entry point) + the water leg, and the entry
+// point is the minimiser of a discrete search (its own move does not change
+// the time to first order), so d t / d T(node) is the node's weight in
+// interp_time at the entry point: the bilinear weights of the cell corners
+// (or their nearby-node substitutes), or 1 on the nearest node.  lam[f][q]
+// += g[r] * weight for field f = field_of[src * 12 + itf - 1]; active[f] is
+// set for every field that receives a non-zero source.
+__global__ void adjoint_sources_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
+                                       const RayJob* __restrict__ jobs, const RayOut* __restrict__ out,
+                                       const double* __restrict__ g, int nray, const int* __restrict__ field_of,
+                                       long long nslot, double* lam, int* active) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nray) return;
+  const double gr = g[r];
+  const RayOut& o = out[r];
+  if (!o.valid || o.direct || !(gr != 0.0) || gr != gr) return;
+  const RayGeomDev& G = *Gp;
+  const RayJob J = jobs[r];
+  const int itf = c_path_tf[J.path][c_path_ntf[J.path] - 1];
+  const RayField& F = fields[J.src * 12 + itf - 1];
+  const RayRegionDev& R = G.reg[F.region];
+  const int f = field_of[J.src * 12 + itf - 1];
+  double* L = lam + (long long)f * nslot;
+  const double px = o.ex, pz = o.ez;
+  int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
+  if (ix < 1) ix = 1;
+  if (ix >= G.nx) ix = G.nx - 1;
+  if (iz < 1) iz = 1;
+  if (iz >= G.nz) iz = G.nz - 1;
+  const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
+  int n[4] = {rnode(G, R, ix, iz), rnode(G, R, ix + 1, iz), rnode(G, R, ix, iz + 1), rnode(G, R, ix + 1, iz + 1)};
+  const int cx[4] = {ix, ix + 1, ix, ix + 1}, cz[4] = {iz, iz, iz + 1, iz + 1};
+  bool all = true;
+  for (int k = 0; k < 4; k++) {
+    if (n[k] < 0) n[k] = rnode_nearby(G, R, cx[k], cz[k]);
+    if (n[k] < 0 || !(F.T[n[k]] < RAY_HUGE * 0.5)) all = false;
+  }
+  atomicOr(active + f, 1);
+  if (all) {
+    const double w[4] = {(1.0 - w1) * (1.0 - w2), w1 * (1.0 - w2), (1.0 - w1) * w2, w1 * w2};
+    for (int k = 0; k < 4; k++)
+      if (w[k] != 0.0) atomicAdd(L + n[k], gr * w[k]);
+    return;
+  }
+  const int m = nearest_any(G, R, px, pz, RAY_HUGE);
+  if (m >= 0) atomicAdd(L + m, gr);
+}
+
+hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, const RayOut* out,
+                                  const double* g, int nray, const int* field_of, long long nslot, double* lam,
+                                  int* active, hipStream_t st) {
+  if (nray == 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_sources_kernel, dim3((nray + 127) / 128), dim3(128), 0, st, G, fields, jobs, out, g, nray,
+                     field_of, nslot, lam, active);
+  return hipGetLastError();
+}
+
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
                         const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
                         RayOut* out, int program, hipStream_t st) {

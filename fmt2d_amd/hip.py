@@ -44,6 +44,12 @@ class FieldSummary(C.Structure):
                 ("mean_resid", C.c_double), ("n_resid", C.c_longlong), ("n_resid_big", C.c_longlong)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("fields", C.c_int), ("rounds", C.c_int), ("leaves_unmatched", C.c_longlong),
+                ("dropped_partials", C.c_longlong), ("ms_record", C.c_double), ("ms_reverse", C.c_double),
+                ("ms_total", C.c_double), ("peak_bytes", C.c_longlong)]
+
+
 class SystemParams(C.Structure):
     _fields_ = [("crit_chi", C.c_double), ("phase_wt_psed", C.c_double), ("phase_wt_pmp", C.c_double),
                 ("phase_wt_pgss", C.c_double), ("refl_weight", C.c_double), ("alt_mode", C.c_int)]
@@ -134,6 +140,8 @@ def load() -> C.CDLL:
     lib.fmt2d_model_put.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
     lib.fmt2d_frechet_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.fmt2d_frechet_apply_noalloc.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.fmt2d_adjoint_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(AdjointStats)]
+    lib.fmt2d_set_adjoint_memory.argtypes = [C.c_void_p, C.c_longlong]
     lib.fmt2d_param_index.argtypes = [C.c_void_p, _ip, C.c_int, _ip]
     lib.fmt2d_line_search.argtypes = [C.c_void_p, C.POINTER(LineSearchParams), _dp, _dp, _ip,
                                       C.POINTER(LineSearchResult)]
@@ -150,7 +158,9 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
            "fmt2d_trace", "fmt2d_ray_info", "fmt2d_ray_points",
-           "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_frechet_apply_noalloc", "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
+           "fmt2d_frechet_csr", "fmt2d_frechet_apply", "fmt2d_frechet_apply_noalloc",
+           "fmt2d_adjoint_gradient", "fmt2d_set_adjoint_memory",
+           "fmt2d_param_index", "fmt2d_build_system", "fmt2d_system_get", "fmt2d_lsqr",
            "fmt2d_damped_update", "fmt2d_model_get", "fmt2d_model_put", "fmt2d_apply_model_step", "fmt2d_line_search")
 
 
@@ -328,6 +338,25 @@ class Context:
             raise ValueError("x or y has the wrong length")
         _check(self._lib.fmt2d_frechet_apply_noalloc(self._h, int(bool(transpose)), _p(x), _p(y)))
         return y
+
+    def adjoint_gradient(self, dJdt):
+        """Adjoint-state gradient of a receiver-time misfit (fmt2d_adjoint_gradient):
+        dJdt per ray of the last trace; returns (d J / d velocity on the
+        propagation grid (nz, nx), d J / d depth of the interface nodes
+        (5, nx): interfaces 1..5, AdjointStats)."""
+        g = np.ascontiguousarray(np.nan_to_num(np.asarray(dJdt, dtype=np.float64), nan=0.0))
+        if len(g) != self._nray:
+            raise ValueError(f"dJdt has {len(g)} entries for {self._nray} rays of the last trace")
+        info = self.info
+        gv = np.zeros(info.nz * info.nx)
+        gz = np.zeros(5 * info.nx)
+        st = AdjointStats()
+        _check(self._lib.fmt2d_adjoint_gradient(self._h, _p(g), _p(gv), _p(gz), C.byref(st)))
+        return gv.reshape(info.nz, info.nx), gz.reshape(5, info.nx), st
+
+    def set_adjoint_memory(self, nbytes: int = 0):
+        """Cap on the adjoint sweep's edge records per batch (0: 4 GiB)."""
+        _check(self._lib.fmt2d_set_adjoint_memory(self._h, int(nbytes)))
 
     def param_index(self):
         """Frechet column (1-based, 0 none) of each velocity-grid node, (nz, nx);

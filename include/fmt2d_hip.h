@@ -223,6 +223,45 @@ int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* 
  * otherwise).  Results equal fmt2d_frechet_apply's bit for bit.  Calls that
  * overlap on one device share that array and race. */
 int fmt2d_frechet_apply_noalloc(fmt2d_ctx* ctx, int transpose, const double* x, double* y);
+typedef struct {
+  int fields;                 /* stage fields swept (those some receiver time depends on) */
+  int rounds;                 /* rounds of the reverse sweep over all stage levels */
+  long long leaves_unmatched; /* reached nodes whose recomputed winning update, or head-wave chain, did
+                                 not reproduce the stored time (0 expected): kept as leaves */
+  long long dropped_partials; /* partial derivatives dropped (a value held 8 already; 0 expected) */
+  double ms_record, ms_reverse, ms_total;
+  long long peak_bytes;       /* device memory of the sweep (adjoint fields, edges, counters) */
+} fmt2d_adjoint_stats;
+/* Adjoint-state gradient of a misfit of the receiver times of the last
+ * trace (no counterpart in the reference, which inverts with the ray-based
+ * rows of compute_frechet_for_ray2d, inversion2d.f90:4548-4787): dJdt[r] is
+ * d J / d t_r for every ray (0 or NaN: none).  Every reached node of the
+ * solved fields is one local update (regular_update2d, time_from_1_node /
+ * time_from_2_nodes, propagate2d.f90:605-946, or its seed,
+ * :1982-2411); the adjoint of the receiver times (interpolation weights at
+ * the entry point) is swept back over those updates in reverse dependency
+ * order, stage by stage through the interface re-seeding and the PmP
+ * head-wave pass, onto the region velocities and interface-node depths.
+ * The updates' partial derivatives are recomputed from the resident fields
+ * one stage level at a time (the forward stores no adjoint data; the sweep
+ * holds one level's edges).  grad_vel (nz x nx, x fastest, the layout of
+ * fmt2d_model_get 0): d J / d velocity of each propagation-grid node (the
+ * region and interface velocities are linear in them,
+ * velocities_to_region2d / velocities_to_intersection2d,
+ * propagate2d.f90:1698-1929).  grad_iface (5 x nx, interface 1..5 in turn,
+ * may be NULL): d J / d depth of each interface node through the node
+ * positions the updates and seeds use and the interface velocities sampled
+ * at them, grid velocities and the cut-cell topology held fixed.  At nodes
+ * where two updates tie (coincident grid and interface nodes on the sea
+ * surface) the times are not differentiable; the gradient follows the
+ * update the forward kept.  Needs a forward without FMT2D_FWD_PROGRAM. */
+int fmt2d_adjoint_gradient(fmt2d_ctx* ctx, const double* dJdt, double* grad_vel, double* grad_iface,
+                           fmt2d_adjoint_stats* stats);
+/* Device memory the adjoint sweep may give to one batch of edge records
+ * (about 120 bytes per field slot; the fields of a stage level are swept in
+ * batches that fit).  0: the default, 4 GiB.  No counterpart in the
+ * reference. */
+int fmt2d_set_adjoint_memory(fmt2d_ctx* ctx, long long bytes);
 /* Frechet column (1-based; 0: none) of each velocity-grid node (nz x nx, x
  * fastest) as initialize_inversion2d numbers the parameters
  * (frechet2d.f90:44-60); the interfaces' input nodes follow.  These are the

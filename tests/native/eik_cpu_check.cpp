@@ -410,6 +410,7 @@ int cpu_point_eval(void* hp, int r, double sx, const double* inT, const double* 
 struct PrintVisitor {
   Best<double> b;
   double last_evt = -1;
+  void note(int, const Member&, const Member&) const {}
   bool want(double evt) { last_evt = evt; return true; }
   void add(double T, double gx, double gz, int kind, int susp, int tie) {
     fprintf(stderr, "    cand kind=%d evt=%.17g T=%.17g g=(%.17g,%.17g) susp=%d tie=%d%s\n", kind, last_evt, T, gx, gz, susp, tie,

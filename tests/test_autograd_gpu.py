@@ -1,5 +1,6 @@
-"""fmt2d_amd.autograd on the GPU (all five phases on the 1 km checkerboard
-model, velocity grid + sediment base + Moho as parameters).
+"""fmt2d_amd.autograd with the ray-based backward (backward="rays") on the
+GPU (all five phases on the 1 km checkerboard model, velocity grid +
+sediment base + Moho as parameters).
 
 * The adjoint: fmt2d_frechet_apply (G^T x and G x on the device, each sum
   in stored order) equals sequential sums over the rows that
@@ -43,7 +44,8 @@ def setup(tmp_path_factory):
     print("[gpu] autograd: all phases, 1 km grid, velocity + sediment base + Moho", flush=True)
     ctx = hip.Context(0)
     ctx.set_model(sm, sx, sz, mx, mz)
-    tt = TravelTimes(ctx, geom.obs_x, geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z, True, [3, 4])
+    tt = TravelTimes(ctx, geom.obs_x, geom.obs_id - 1, geom.ray_type, geom.rec_x, geom.rec_z, True, [3, 4],
+                     backward="rays")
     dev = "cuda:0"
     v = torch.tensor(ctx.model(0), dtype=torch.float64, device=dev)
     z3 = torch.tensor(ctx.model(3), dtype=torch.float64, device=dev)

@@ -63,11 +63,24 @@ class LinearCtx:
         assert transpose
         return self.G.T @ np.asarray(x)
 
+    def adjoint_gradient(self, g):
+        """The stand-in's exact gradient in fmt2d_adjoint_gradient's layout:
+        d J / d velocity of every grid node, d J / d depth of interfaces 1..5."""
+        y = self.G.T @ np.asarray(g)
+        order = np.argsort(self.idx[self.idx > 0])
+        gs = np.empty(self.n_vel)
+        gs[:] = y[:self.n_vel][order]
+        gv = np.zeros((NZ, NX))
+        gv[self.idx > 0] = -gs / self.v[self.idx > 0] ** 2
+        gz = np.zeros((5, NX))
+        gz[3] = y[self.n_vel:]
+        return gv, gz, None
 
-def make():
+
+def make(backward="rays", param="velocity"):
     ctx = LinearCtx()
     tt = TravelTimes(ctx, [10.0], np.zeros(NRAY), np.ones(NRAY), np.arange(NRAY, dtype=float), np.zeros(NRAY),
-                     invert_velocity=True, ifaces=[4])
+                     invert_velocity=True, ifaces=[4], backward=backward, param=param)
     return ctx, tt
 
 
@@ -81,6 +94,31 @@ def test_gradcheck_velocity_and_depths():
         return torch.nan_to_num(tt(v_, z_), nan=0.0)
 
     assert torch.autograd.gradcheck(f, (v, z), eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+def test_gradcheck_adjoint_backward_velocity_and_slowness():
+    """The adjoint backward's plumbing: the gradient of every grid node and
+    of the chosen interface's depths, with the velocity or its reciprocal
+    as the input."""
+    rng = np.random.default_rng(2)
+    z = torch.tensor(rng.uniform(1.0, 2.0, size=NIF), dtype=torch.float64, requires_grad=True)
+    for param in ("velocity", "slowness"):
+        ctx, tt = make("adjoint", param)
+        v = rng.uniform(2.0, 5.0, size=(NZ, NX))
+        m = torch.tensor(v if param == "velocity" else 1.0 / v, dtype=torch.float64, requires_grad=True)
+
+        def f(m_, z_):
+            return torch.nan_to_num(tt(m_, z_), nan=0.0)
+
+        assert torch.autograd.gradcheck(f, (m, z), eps=1e-7, atol=1e-8, rtol=1e-6)
+
+
+def test_backward_choices_are_checked():
+    import pytest
+    with pytest.raises(ValueError):
+        make("other")
+    with pytest.raises(ValueError):
+        make("adjoint", "density")
 
 
 def test_failed_ray_and_water_nodes_get_no_gradient():
