@@ -511,11 +511,14 @@ def joint_figure(iters: int, cpu: bool = False):
             "cpu_baseline": cpu_b, "log": log}
 
 
-def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int):
+def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int, autotune: bool = False):
     """Source-stages per second of one workload's forward (the headline's
     measurement; see the module docstring), with the tile-kernel time from
     HIP events on the solver stream and, when cpu_sample > 0, the reference
-    timed on that many of the workload's OBS."""
+    timed on that many of the workload's OBS.  autotune: the tile schedule
+    is first picked by fmt2d_autotune on this model and these OBS (before
+    the warm-up, outside the timed steps); every schedule gives the same
+    fields bit for bit."""
     from fmt2d_amd import hip, formats
     m, g, desc = workload(workload_name)
     with tempfile.TemporaryDirectory() as d:
@@ -526,6 +529,13 @@ def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int)
         ctx = hip.Context(0)
         ctx.set_model(sm, sx, sz, mx, mz)
         obs = list(g.obs_x)
+        sched = None
+        if autotune:
+            r = ctx.autotune(obs, level=0)
+            sched = {"tile": [r.tile_x, r.tile_z], "max_sweeps": r.max_sweeps, "max_sweeps_heavy": r.max_sweeps_heavy,
+                     "colors": r.colors, "window_tiles": r.window_tiles, "tuner_trials": r.trials,
+                     "tuner_ms_default": round(r.ms_start, 1), "tuner_ms_chosen": round(r.ms_best, 1)}
+            print(f"autotune: {sched}", file=sys.stderr, flush=True)
         for _ in range(warmup):
             ctx.forward(obs)
         walls, fims, nodes, launches, tiles, disp = [], [], 0, [], [], []
@@ -542,7 +552,7 @@ def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int)
         cpu = cpu_baseline(d, obs, cpu_sample) if cpu_sample > 0 else None
         ctx.close()
     return dict(desc=desc, obs=obs, walls=walls, fims=fims, nodes=nodes, launches=launches, tiles=tiles, disp=disp,
-                cpu=cpu)
+                cpu=cpu, schedule=sched)
 
 
 def main():
@@ -552,6 +562,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched", choices=["batched", "multistage", "plain"])
     ap.add_argument("--cpu-sample", type=int, default=12, help="OBS timed on the host CPU (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile schedule (8x8 tiles, caps 8/8, window 3, two colours)")
     ap.add_argument("--no-multistage", action="store_true", help="skip the 60-OBS multistage figure")
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
@@ -589,7 +601,7 @@ def main():
                                                            args.cpu_sample > 0)}))
         return
 
-    head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample)
+    head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample, autotune=not args.no_autotune)
     print(f"headline forward done: {head['walls']}", file=sys.stderr, flush=True)
     ms = None
     if args.workload != "multistage" and not args.no_multistage:
@@ -634,7 +646,8 @@ def main():
                    "solver_launches_per_step": int(np.mean(head["launches"])),
                    "tiles_per_step": int(np.mean(head["tiles"])), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3),
                    "fim_kernel_dispatches_per_step": int(ndisp),
-                   "fim_kernel_us_per_dispatch": round(1e6 * fim_s / ndisp, 2)},
+                   "fim_kernel_us_per_dispatch": round(1e6 * fim_s / ndisp, 2),
+                   "schedule": head["schedule"]},
         "roofline": {"bound": "hbm", "kernel": "fim_tiles_kernel", "achieved": round(achieved, 3),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 6),
                      "traffic": traffic,

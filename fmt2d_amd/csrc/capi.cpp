@@ -20,6 +20,38 @@ int fmt2d_fail(int code, const std::string& msg) {
   return code;
 }
 
+// Tiles (of the current shape) with cut cells, coincident nodes or interface
+// nodes of the region: their visits are the slow ones and start first in a
+// launch.
+static int build_heavy_tiles(fmt2d_ctx* ctx) {
+  const HostModel& m = ctx->m;
+  const int nx = m.nx, nz = m.nz, tx = ctx->tile_x, tz = ctx->tile_z;
+  const int ntx = (nx + tx - 1) / tx, ntz = (nz + tz - 1) / tz, ntiles = ntx * ntz;
+  std::vector<uint8_t> heavy((size_t)(NREG + 1) * ntiles, 0);
+  for (int r = 1; r <= NREG; r++) {
+    const HostRegion& R = m.reg[r];
+    uint8_t* h = heavy.data() + (size_t)r * ntiles;
+    for (int j = 0; j < nz; j++)
+      for (int i = 0; i < nx; i++) {
+        const uint16_t fl = R.flags[(size_t)j * nx + i];
+        if ((fl & RF_IN) && (fl & ((RF_IRR0 * 15) | (RF_COI0 * 3)))) h[(j / tz) * ntx + i / tx] = 1;
+      }
+    const int ids[2] = {R.itop, R.ibot};
+    for (int s2 = 0; s2 < 2; s2++) {
+      const HostInterface& is = m.isec[ids[s2]];
+      for (int c = 0; c < nx; c++) {
+        const int cz = is.cc[c];
+        if (cz < 0) continue;
+        for (int j = cz; j <= std::min(cz + 1, nz - 1); j++) h[(j / tz) * ntx + c / tx] = 1;
+      }
+    }
+  }
+  HIPCHK(upload(ctx->theavy, heavy, ctx->stream));
+  ctx->heavy_tx = tx;
+  ctx->heavy_tz = tz;
+  return FMT2D_OK;
+}
+
 extern "C" {
 
 const char* fmt2d_last_error(void) { return g_err.c_str(); }
@@ -106,30 +138,9 @@ int model_upload(fmt2d_ctx* ctx) {
   }
   rd[0] = rd[1];
   HIPCHK(upload(ctx->regs, rd, st));
-  // tiles (8x8) with cut cells, coincident nodes or interface nodes of the
-  // region: their visits are the slow ones and start first in a launch
   {
-    const int ntx = (nx + 7) / 8, ntz = (nz + 7) / 8, ntiles = ntx * ntz;
-    std::vector<uint8_t> heavy((size_t)(NREG + 1) * ntiles, 0);
-    for (int r = 1; r <= NREG; r++) {
-      const HostRegion& R = m.reg[r];
-      uint8_t* h = heavy.data() + (size_t)r * ntiles;
-      for (int j = 0; j < nz; j++)
-        for (int i = 0; i < nx; i++) {
-          const uint16_t fl = R.flags[(size_t)j * nx + i];
-          if ((fl & RF_IN) && (fl & ((RF_IRR0 * 15) | (RF_COI0 * 3)))) h[(j / 8) * ntx + i / 8] = 1;
-        }
-      const int ids[2] = {R.itop, R.ibot};
-      for (int s2 = 0; s2 < 2; s2++) {
-        const HostInterface& is = m.isec[ids[s2]];
-        for (int c = 0; c < nx; c++) {
-          const int cz = is.cc[c];
-          if (cz < 0) continue;
-          for (int j = cz; j <= std::min(cz + 1, nz - 1); j++) h[(j / 8) * ntx + c / 8] = 1;
-        }
-      }
-    }
-    HIPCHK(upload(ctx->theavy, heavy, st));
+    const int rc = build_heavy_tiles(ctx);
+    if (rc) return rc;
   }
   std::vector<IsecDev> idv(NIF + 1);
   for (int k = 1; k <= NIF; k++) {
@@ -348,7 +359,12 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   HIPCHK(upload(ctx->descs, fd, st));
   HIPCHK(upload(ctx->links, lk, st));
   // ---- tile bookkeeping sized for the largest level
-  const int ntx = (nx + 7) / 8, ntz = (nz + 7) / 8, ntiles = ntx * ntz;
+  const int tx = ctx->tile_x, tz = ctx->tile_z;
+  if (ctx->heavy_tx != tx || ctx->heavy_tz != tz) {
+    const int rc = build_heavy_tiles(ctx);
+    if (rc) return rc;
+  }
+  const int ntx = (nx + tx - 1) / tx, ntz = (nz + tz - 1) / tz, ntiles = ntx * ntz;
   int maxlev = 0;
   for (int L = 0; L < 4; L++) maxlev = std::max(maxlev, level_off[L + 1] - level_off[L]);
   const size_t ntt = (size_t)maxlev * ntiles;
@@ -371,7 +387,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   const int max_launch = ctx->launch_cap > 0 ? ctx->launch_cap : 64 * (ntx + ntz) + 4000;
   const bool headwave = !(flags & FMT2D_FWD_NO_HEADWAVE);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
-  const double window = ctx->window_tiles * 8.0 * std::max(m.dx, m.dz) / 4.0;
+  const double window = ctx->window_tiles * std::max(tx * m.dx, tz * m.dz) / 4.0;
   size_t nev = 0;  // event pairs recorded around fim launches
   for (int L = 0; L < 4; L++) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
@@ -382,7 +398,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
     // [1] / [2] sizes of active lists 0 / 1
     HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 4, st));
-    HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
+    HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, tx, tz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
                              ctx->tcount.p + 1, st));
     if (ctx->ttrace.n < (size_t)max_launch + 1) HIPCHK(ctx->ttrace.alloc((size_t)max_launch + 1));
     if (ctx->tmax.n < (size_t)max_launch + 1) HIPCHK(ctx->tmax.alloc((size_t)max_launch + 1));
@@ -408,7 +424,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
           ctx->ev.push_back(e);
         }
         HIPCHK(hipEventRecord(ctx->ev[ei], st));
-        HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, ctx->tin.p,
+        HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
                           ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->max_sweeps_heavy,
                           ctx->colors,
                           ctx->acc.p, ctx->tmax.p + launches, (int)ntt, st));
@@ -427,7 +443,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         for (auto& p : tail) {
           const int itf = order[f0 + p.x] % FMT2D_NTF + 1;
           msg += " tf" + std::to_string(itf) + "/src" + std::to_string(order[f0 + p.x] / FMT2D_NTF) + "/t" +
-                 std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * 8) + "," + std::to_string((p.y / ntx) * 8) + ")";
+                 std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * tx) + "," + std::to_string((p.y / ntx) * tz) + ")";
         }
         return fail(FMT2D_ENOCONV, msg + ")");
       }
@@ -532,6 +548,86 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
   ctx->colors = colors;
   ctx->window_tiles = window_tiles;
   ctx->max_sweeps = max_sweeps;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_tile(fmt2d_ctx* ctx, int tile_x, int tile_z) {
+  if (!ctx) return fail(FMT2D_EINVAL, "NULL context");
+  const bool ok = (tile_x == 8 && tile_z == 8) || (tile_x == 16 && tile_z == 8) || (tile_x == 8 && tile_z == 16);
+  if (!ok) return fail(FMT2D_EINVAL, "tile shape must be 8x8, 16x8 or 8x16");
+  ctx->tile_x = tile_x;
+  ctx->tile_z = tile_z;
+  return FMT2D_OK;
+}
+
+int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int level, fmt2d_tune_result* out) {
+  if (!ctx || !out) return fail(FMT2D_EINVAL, "NULL argument");
+  if (level < 0 || level > 1) return fail(FMT2D_EINVAL, "level must be 0 or 1");
+  struct Cfg {
+    int tx, tz, cap, hcap, colors;
+    double window;
+  };
+  auto apply = [&](const Cfg& c) {
+    ctx->tile_x = c.tx; ctx->tile_z = c.tz;
+    ctx->max_sweeps = c.cap; ctx->max_sweeps_heavy = c.hcap;
+    ctx->colors = c.colors; ctx->window_tiles = c.window;
+  };
+  // one forward per candidate, timed by the forward's own wall clock; every
+  // candidate gives the same fields bit for bit, so the context ends solved
+  int trials = 0;
+  auto time_of = [&](const Cfg& c, double& ms) -> int {
+    apply(c);
+    ctx->fields_current = false;
+    const int rc = fmt2d_forward(ctx, nsrc, src_x, flags);
+    if (rc) return rc;
+    ms = ctx->stats.ms_total;
+    trials++;
+    return FMT2D_OK;
+  };
+  const Cfg start{ctx->tile_x, ctx->tile_z, ctx->max_sweeps, ctx->max_sweeps_heavy, ctx->colors, ctx->window_tiles};
+  Cfg best = start;
+  double best_ms = 0.0;
+  int rc = time_of(best, best_ms);
+  if (rc) { apply(start); return rc; }
+  const double start_ms = best_ms;
+  // coordinate descent: tile shape, light cap, heavy cap, then (level 1)
+  // window and in-tile order
+  auto try_all = [&](std::vector<Cfg> cands) -> int {
+    for (const Cfg& c : cands) {
+      double ms;
+      const int r = time_of(c, ms);
+      if (r) return r;
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    return FMT2D_OK;
+  };
+  std::vector<Cfg> cands;
+  const int shapes[3][2] = {{8, 8}, {16, 8}, {8, 16}};
+  for (auto& sh : shapes) {
+    if (sh[0] == best.tx && sh[1] == best.tz) continue;
+    Cfg c = best; c.tx = sh[0]; c.tz = sh[1];
+    if (c.tx * c.tz > 64) { c.cap = std::max(c.cap, 16); c.hcap = std::max(c.hcap, 16); }
+    cands.push_back(c);
+  }
+  if ((rc = try_all(cands))) { apply(start); return rc; }
+  cands.clear();
+  for (int cap : {8, 12, 16, 24}) if (cap != best.cap) { Cfg c = best; c.cap = cap; cands.push_back(c); }
+  if ((rc = try_all(cands))) { apply(start); return rc; }
+  cands.clear();
+  for (int cap : {6, 8, 12, 16}) if (cap != best.hcap) { Cfg c = best; c.hcap = cap; cands.push_back(c); }
+  if ((rc = try_all(cands))) { apply(start); return rc; }
+  if (level >= 1) {
+    cands.clear();
+    for (double w : {2.0, 3.0, 4.0, 6.0}) if (w != best.window) { Cfg c = best; c.window = w; cands.push_back(c); }
+    for (int col : {2, 4}) if (col != best.colors) { Cfg c = best; c.colors = col; cands.push_back(c); }
+    if ((rc = try_all(cands))) { apply(start); return rc; }
+  }
+  apply(best);
+  out->tile_x = best.tx; out->tile_z = best.tz;
+  out->max_sweeps = best.cap; out->max_sweeps_heavy = best.hcap;
+  out->colors = best.colors; out->window_tiles = best.window;
+  out->ms_best = best_ms; out->ms_start = start_ms;
+  out->trials = trials;
   return FMT2D_OK;
 }
 

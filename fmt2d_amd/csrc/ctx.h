@@ -120,6 +120,8 @@ struct fmt2d_ctx {
   int max_sweeps = 8;                 // a visit over the cap stays active and continues later
   int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
+  int tile_x = 8, tile_z = 8;         // tile shape of the solver (8x8, 16x8, 8x16)
+  int heavy_tx = 0, heavy_tz = 0;     // shape the slow-tile flags (theavy) were built for
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]

@@ -59,8 +59,8 @@ hipError_t launch_fill(const FieldDesc* descs, int nfield, int nslot, hipStream_
 hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const StageLink* links_all, int nfield, int nx,
                                    int nz, const IsecDev* isecs, hipStream_t st);
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
-                             unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
-                             hipStream_t st);
+                             int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
+                             int* count_next, hipStream_t st);
 // per launch: clear the due / next counters and field fronts, then split
 // the active list (its count on the device) into due tiles and carry-over
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
@@ -68,10 +68,11 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
                            unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
                            int* trace_prev, const FieldDesc* descs, const uint8_t* heavy, int due_cap, int grid,
                            hipStream_t st);
+// tile shapes tx x tz: 8x8, 16x8, 8x16 (hipErrorInvalidValue otherwise)
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
-                      int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int max_sweeps_heavy, int colors, unsigned long long* acc,
-                      int* launch_max, int due_cap, hipStream_t st);
+                      int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
+                      int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
+                      unsigned long long* acc, int* launch_max, int due_cap, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

@@ -28,38 +28,99 @@
 
 namespace fmt2d {
 
-// LDS view of one tile window (TILE+2*HALO)^2 plus the interface nodes of
-// its columns.  Accessors clamp to the window; the operator never reaches
-// beyond two nodes from an inner node, which the halo covers.
-constexpr int TILE = 8, HALO = 2, WIN = TILE + 2 * HALO, WIN2 = WIN * WIN;
+// Tile shapes.  A wavefront solves one tile of TX x TZ grid nodes with a
+// 2-node halo staged in LDS.  8 x 8: one node per lane.  16 x 8 and 8 x 16:
+// two nodes per lane, one of each checkerboard colour (a horizontal pair),
+// so every lane has a node in each two-colour phase.
+constexpr int HALO = 2;
 
-struct TileLds {
-  double T[WIN2];
-  double GX[WIN2], GZ[WIN2], vel[WIN2];
-  uint16_t flags[WIN2];
-  uint8_t tie[WIN2];
-  double xg[WIN], zg[WIN];
-  double iT[2 * WIN];
-  double iGX[2 * WIN], iGZ[2 * WIN], ivel[2 * WIN], ifx[2 * WIN], ifz[2 * WIN];
-  int ifcc[2 * WIN];
-  uint8_t ifok[2 * WIN];
+template <int TX, int TZ>
+struct Shape {
+  static constexpr int NN = TX * TZ;      // tile nodes
+  static constexpr int NPL = NN / 64;     // nodes per lane (= 64-bit words of a node mask)
+  static constexpr int WX = TX + 2 * HALO, WZ = TZ + 2 * HALO, W2 = WX * WZ;
+  static_assert(NN == 64 || NN == 128, "tile of 64 or 128 nodes");
+  static_assert(2 * TX <= 32, "interface nodes of a tile fit a 32-bit mask");
+  // tile position (a, b) of node k of a lane
+  __device__ static int node_a(int lane, int k) {
+    if (NPL == 1) return lane % TX;
+    const int b = node_b(lane, k);
+    return 2 * (lane % (TX / 2)) + ((b + k) & 1);
+  }
+  __device__ static int node_b(int lane, int k) {
+    if (NPL == 1) return lane / TX;
+    return lane / (TX / 2);
+  }
+  // node index (lane + 64 k) of tile position (a, b)
+  __device__ static int node_index(int a, int b) {
+    if (NPL == 1) return b * TX + a;
+    return (b * (TX / 2) + (a >> 1)) + 64 * ((a + b) & 1);
+  }
 };
 
+template <int NPL>
+struct NodeMask {
+  unsigned long long w[NPL];
+  __device__ void clear() {
+#pragma unroll
+    for (int k = 0; k < NPL; k++) w[k] = 0ull;
+  }
+  __device__ void fill() {
+#pragma unroll
+    for (int k = 0; k < NPL; k++) w[k] = ~0ull;
+  }
+  __device__ void set(int n) { w[n >> 6] |= 1ull << (n & 63); }
+  __device__ void operator|=(const NodeMask& o) {
+#pragma unroll
+    for (int k = 0; k < NPL; k++) w[k] |= o.w[k];
+  }
+  __device__ bool meets(const NodeMask& o) const {
+    unsigned long long x = 0ull;
+#pragma unroll
+    for (int k = 0; k < NPL; k++) x |= w[k] & o.w[k];
+    return x != 0ull;
+  }
+};
+
+// LDS view of one tile window plus the interface nodes of its columns.
+// Accessors clamp to the window; the operator never reaches beyond two
+// nodes from an inner node, which the halo covers.
+template <int TX, int TZ>
+struct TileLds {
+  using S = Shape<TX, TZ>;
+  double T[S::W2];
+  double GX[S::W2], GZ[S::W2], vel[S::W2];
+  uint16_t flags[S::W2];
+  uint8_t tie[S::W2];
+  double xg[S::WX], zg[S::WZ];
+  double iT[2 * S::WX];
+  double iGX[2 * S::WX], iGZ[2 * S::WX], ivel[2 * S::WX], ifx[2 * S::WX], ifz[2 * S::WX];
+  int ifcc[2 * S::WX];
+  uint8_t ifok[2 * S::WX];
+  // tiles of two nodes per lane: the nodes' seeds and fast-path model data
+  // (one node per lane keeps them in registers)
+  static constexpr int NS = S::NPL > 1 ? S::NN : 1;
+  RegStatic rs[NS];
+  double t0[NS], gx0[NS], gz0[NS];
+};
+
+template <int TX, int TZ>
 struct LView {
+  using S = Shape<TX, TZ>;
   const RegionDev* R;
   const FieldDesc* D;
-  const TileLds* L;
+  const TileLds<TX, TZ>* L;
   int wx0, wz0;
   __device__ int w(int i, int j) const {
     int a = i - wx0, b = j - wz0;
-    a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a);
-    b = b < 0 ? 0 : (b >= WIN ? WIN - 1 : b);
-    return b * WIN + a;
+    a = a < 0 ? 0 : (a >= S::WX ? S::WX - 1 : a);
+    b = b < 0 ? 0 : (b >= S::WZ ? S::WZ - 1 : b);
+    return b * S::WX + a;
   }
   __device__ int wc(int s, int c) const {
     int a = c - wx0;
-    a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a);
-    return s * WIN + a;
+    a = a < 0 ? 0 : (a >= S::WX ? S::WX - 1 : a);
+    return s * S::WX + a;
   }
   __device__ const RegionDev& reg() const { return *R; }
   __device__ bool in(int i, int j) const {
@@ -71,8 +132,12 @@ struct LView {
   __device__ double GZ(int i, int j) const { return L->GZ[w(i, j)]; }
   __device__ int tie(int i, int j) const { return L->tie[w(i, j)]; }
   __device__ double vel(int i, int j) const { return L->vel[w(i, j)]; }
-  __device__ double xg(int i) const { int a = i - wx0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->xg[a]; }
-  __device__ double zg(int j) const { int a = j - wz0; a = a < 0 ? 0 : (a >= WIN ? WIN - 1 : a); return L->zg[a]; }
+  __device__ double xg(int i) const {
+    int a = i - wx0; a = a < 0 ? 0 : (a >= S::WX ? S::WX - 1 : a); return L->xg[a];
+  }
+  __device__ double zg(int j) const {
+    int a = j - wz0; a = a < 0 ? 0 : (a >= S::WZ ? S::WZ - 1 : a); return L->zg[a];
+  }
   __device__ double iT(int s, int c) const { return L->iT[wc(s, c)]; }
   __device__ double iGX(int s, int c) const { return L->iGX[wc(s, c)]; }
   __device__ double iGZ(int s, int c) const { return L->iGZ[wc(s, c)]; }
@@ -198,8 +263,8 @@ __device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key
 // First active tiles: the point-source cell, or every tile within three rows
 // of a start-interface node that carries a time.
 __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
-                                  int nfield, int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey,
-                                  int2* list_next, int* count_next) {
+                                  int nfield, int ntx, int ntz, int tx, int tz, unsigned int* in_set,
+                                  unsigned long long* tkey, int2* list_next, int* count_next) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const RegionDev& R0 = regs[0];
   const int nx = R0.nx, nz = R0.nz;
@@ -216,7 +281,7 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
     for (int i = d.ix0; i <= d.ix0 + 1; i++)
       for (int j = d.iz0; j <= d.iz0 + 1; j++)
         if (i >= 0 && i < nx && j >= 0 && j < nz)
-          add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key_of(d.t0), in_set, tkey, list_next, count_next);
+          add_tile(f, (j / tz) * ntx + i / tx, ntiles, key_of(d.t0), in_set, tkey, list_next, count_next);
     return;
   }
   if (!(d.isT[c] < 0.5e20)) return;
@@ -226,7 +291,7 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
   const unsigned long long key = key_of(d.isT[c]);
   for (int i = max(0, c - 1); i <= min(nx - 1, c + 1); i++)
     for (int j = j0; j <= j1; j++)
-      if ((j % TILE) == 0 || j == j0) add_tile(f, (j / TILE) * ntx + i / TILE, ntiles, key, in_set, tkey, list_next, count_next);
+      if ((j % tz) == 0 || j == j0) add_tile(f, (j / tz) * ntx + i / tx, ntiles, key, in_set, tkey, list_next, count_next);
 }
 
 // Due tiles of a launch: count_due[0] tiles cut by an interface (placed
@@ -295,39 +360,44 @@ __global__ void split_kernel(const int2* __restrict__ list, const int* __restric
   }
 }
 
-// Tile nodes (bit b*TILE + a) of the box [a0,a1] x [b0,b1], clipped.
-__device__ inline unsigned long long tile_box(int a0, int a1, int b0, int b1) {
-  a0 = max(a0, 0); a1 = min(a1, TILE - 1); b0 = max(b0, 0); b1 = min(b1, TILE - 1);
-  if (a0 > a1 || b0 > b1) return 0ull;
-  const unsigned long long row = ((1ull << (a1 - a0 + 1)) - 1ull) << a0;
-  unsigned long long m = 0ull;
-  for (int b = b0; b <= b1; b++) m |= row << (TILE * b);
+// Tile nodes of the box [a0,a1] x [b0,b1], clipped to the tile.
+template <int TX, int TZ>
+__device__ inline NodeMask<Shape<TX, TZ>::NPL> tile_box(int a0, int a1, int b0, int b1) {
+  NodeMask<Shape<TX, TZ>::NPL> m;
+  m.clear();
+  a0 = max(a0, 0); a1 = min(a1, TX - 1); b0 = max(b0, 0); b1 = min(b1, TZ - 1);
+  for (int b = b0; b <= b1; b++)
+    for (int a = a0; a <= a1; a++) m.set(Shape<TX, TZ>::node_index(a, b));
   return m;
 }
 
-// Interface nodes (bit s*TILE + column) of columns [a0,a1], both sides.
+// Interface nodes (bit s*TX + column) of columns [a0,a1], both sides.
+template <int TX>
 __device__ inline unsigned tile_iface_cols(int a0, int a1) {
-  a0 = max(a0, 0); a1 = min(a1, TILE - 1);
+  a0 = max(a0, 0); a1 = min(a1, TX - 1);
   if (a0 > a1) return 0u;
-  const unsigned m = ((1u << (a1 - a0 + 1)) - 1u) << a0;
-  return m | (m << TILE);
+  const unsigned m = (unsigned)(((1ull << (a1 - a0 + 1)) - 1ull) << a0);
+  return m | (m << TX);
 }
 
 // One visit of tile ft: stage the window, sweep to the tile's local fixed
 // point, write back, activate the neighbours whose halo changed.
+template <int TX, int TZ>
 __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
                          int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
-                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, TileLds& L) {
+                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, TileLds<TX, TZ>& L) {
+  using S = Shape<TX, TZ>;
+  constexpr int NPL = S::NPL, WX = S::WX, WZ = S::WZ, W2 = S::W2;
   const int f = ft.x, t = ft.y;
   const FieldDesc& d = descs[f];
   const RegionDev& R = regs[d.region];
   const int nx = R.nx, nz = R.nz, lane = threadIdx.x;
   const int ntiles = ntx * ntz;
-  const int tx0 = (t % ntx) * TILE, tz0 = (t / ntx) * TILE;
+  const int tx0 = (t % ntx) * TX, tz0 = (t / ntx) * TZ;
   const int wx0 = tx0 - HALO, wz0 = tz0 - HALO;
   // ---- stage the window
-  for (int k = lane; k < WIN2; k += 64) {
-    const int gi = wx0 + k % WIN, gj = wz0 + k / WIN;
+  for (int k = lane; k < W2; k += 64) {
+    const int gi = wx0 + k % WX, gj = wz0 + k / WX;
     if (gi >= 0 && gi < nx && gj >= 0 && gj < nz) {
       const int q = gj * nx + gi;
       L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
@@ -336,10 +406,10 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
       L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0; L.tie[k] = 0;
     }
   }
-  if (lane < WIN) { const int gi = wx0 + lane; L.xg[lane] = (gi >= 0 && gi < nx) ? R.xg[gi] : 0.0; }
-  else if (lane < 2 * WIN) { const int gj = wz0 + lane - WIN; L.zg[lane - WIN] = (gj >= 0 && gj < nz) ? R.zg[gj] : 0.0; }
-  if (lane < 2 * WIN) {
-    const int s = lane / WIN, gi = wx0 + lane % WIN;
+  if (lane < WX) { const int gi = wx0 + lane; L.xg[lane] = (gi >= 0 && gi < nx) ? R.xg[gi] : 0.0; }
+  else if (lane < WX + WZ) { const int gj = wz0 + lane - WX; L.zg[lane - WX] = (gj >= 0 && gj < nz) ? R.zg[gj] : 0.0; }
+  if (lane < 2 * WX) {
+    const int s = lane / WX, gi = wx0 + lane % WX;
     if (gi >= 0 && gi < nx) {
       const int q = (nz + s) * nx + gi;
       L.iT[lane] = d.T[q]; L.iGX[lane] = d.GX[q]; L.iGZ[lane] = d.GZ[q];
@@ -351,49 +421,72 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     }
   }
   __syncthreads();
-  LView v{&R, &d, &L, wx0, wz0};
-  // ---- my grid node and (lanes 0..15) my interface node
-  const int gi = tx0 + lane % TILE, gj = tz0 + lane / TILE;
-  const int wg = (gj - wz0) * WIN + (gi - wx0);
-  const bool g_act = gi < nx && gj < nz && (L.flags[wg] & RF_IN) && (gj * nx + gi) != d.fixed_key;
-  const int g_color = (gi & 1) | ((gj & 1) << 1);
+  LView<TX, TZ> v{&R, &d, &L, wx0, wz0};
+  // ---- my grid nodes (one per colour when a lane holds two) and (lanes
+  // below 2 TX) my interface node
+  int gi[NPL], gj[NPL], wg[NPL], g_phase[NPL];
+  bool g_act[NPL], g_fast[NPL], g_dirty[NPL];
+  // seed and fast-path model data of the node (one node per lane: registers)
   double g_t0 = FMT_HUGE_TIME, g_gx0 = 0.0, g_gz0 = 0.0;
-  if (g_act) seed_grid(d, R, gi, gj, g_t0, g_gx0, g_gz0);
-  // nodes with four regular cells and no coincident interface node take the
-  // fast path (eik_regular.h); what it reads of the model is formed once
-  const bool g_fast = g_act && reg_fast_node(L.flags[wg]);
   RegStatic g_rs;
-  if (g_fast) g_rs = reg_static(v, gi, gj);
-  const int is = lane >> 3, ic = tx0 + (lane & 7);
+  NodeMask<NPL> g_dep[NPL], g_pend[NPL];
+  unsigned g_idep[NPL], g_ipend[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; k++) {
+    const int a = S::node_a(lane, k), b = S::node_b(lane, k);
+    gi[k] = tx0 + a; gj[k] = tz0 + b;
+    wg[k] = (b + HALO) * WX + (a + HALO);
+    g_act[k] = gi[k] < nx && gj[k] < nz && (L.flags[wg[k]] & RF_IN) && (gj[k] * nx + gi[k]) != d.fixed_key;
+    g_phase[k] = colors == 1 ? 0 : (colors == 2 ? ((gi[k] + gj[k]) & 1) : ((gi[k] & 1) | ((gj[k] & 1) << 1)));
+    double t0 = FMT_HUGE_TIME, gx0 = 0.0, gz0 = 0.0;
+    if (g_act[k]) seed_grid(d, R, gi[k], gj[k], t0, gx0, gz0);
+    // nodes with four regular cells and no coincident interface node take
+    // the fast path (eik_regular.h); what it reads of the model is formed once
+    g_fast[k] = g_act[k] && reg_fast_node(L.flags[wg[k]]);
+    if constexpr (NPL == 1) {
+      g_t0 = t0; g_gx0 = gx0; g_gz0 = gz0;
+      if (g_fast[k]) g_rs = reg_static(v, gi[k], gj[k]);
+    } else {
+      const int n = lane + 64 * k;
+      L.t0[n] = t0; L.gx0[n] = gx0; L.gz0[n] = gz0;
+      if (g_fast[k]) L.rs[n] = reg_static(v, gi[k], gj[k]);
+    }
+    g_dirty[k] = false;
+    // Nodes whose inputs did not change since their last evaluation give
+    // the same result again and are skipped.  A grid node reads tile nodes
+    // at most two rows/columns away and the interface nodes of its own and
+    // adjacent columns; an interface node reads the grid nodes within two
+    // columns and two rows of its cell (rows cz-2 .. cz+3 with its
+    // coincident partner) and the interface nodes of its column.
+    // Changed-node masks of every phase accumulate until a node is
+    // evaluated again; the first sweep evaluates everything.
+    g_dep[k] = tile_box<TX, TZ>(a - 2, a + 2, b - 2, b + 2);
+    g_idep[k] = tile_iface_cols<TX>(a - 2, a + 2);
+    g_pend[k].fill();
+    g_ipend[k] = 0xFFFFFFFFu;
+  }
+  const int is = lane / TX, ic = tx0 + lane % TX;
   bool i_act = false;
   double i_t0 = FMT_HUGE_TIME, i_gx0 = 0.0, i_gz0 = 0.0;
   int i_cc = -1;
   int i_home = -1;  // row the interface node is solved in
-  if (lane < 16 && ic < nx - 1) {
-    i_cc = L.ifcc[is * WIN + ic - wx0];
-    if (i_cc >= tz0 - 1 && i_cc < tz0 + TILE) i_home = iface_home_row(v, is, ic, i_cc);
-    i_act = i_home >= tz0 && i_home < tz0 + TILE;
+  if (lane < 2 * TX && ic < nx - 1) {
+    i_cc = L.ifcc[is * WX + ic - wx0];
+    if (i_cc >= tz0 - 1 && i_cc < tz0 + TZ) i_home = iface_home_row(v, is, ic, i_cc);
+    i_act = i_home >= tz0 && i_home < tz0 + TZ;
     if (i_act) seed_iface(d, is, ic, i_t0, i_gx0, i_gz0);
   }
-  const int wi = is * WIN + (ic - wx0);
-  bool g_dirty = false, i_dirty = false;
+  const int wi = is * WX + (ic - wx0);
+  bool i_dirty = false;
+  NodeMask<NPL> i_dep, i_pend;
+  i_dep.clear();
+  const int ia = lane % TX;
+  if (i_act) i_dep = tile_box<TX, TZ>(ia - 2, ia + 2, i_cc - tz0 - 2, i_cc - tz0 + 3);
+  const unsigned i_idep = tile_iface_cols<TX>(ia - 2, ia + 2);
+  i_pend.fill();
+  unsigned i_ipend = 0xFFFFFFFFu;
+  constexpr unsigned IMASK = (unsigned)((1ull << (2 * TX)) - 1ull);
   bool more = true;
-  // Nodes whose inputs did not change since their last evaluation give the
-  // same result again and are skipped.  A grid node reads tile nodes at most
-  // two rows/columns away and the interface nodes of its own and adjacent
-  // columns; an interface node reads the grid nodes within two columns and
-  // two rows of its cell (rows cz-2 .. cz+3 with its coincident partner) and
-  // the interface nodes of its column.  Changed-node masks of every phase
-  // (one bit per tile node, lane = node) accumulate until a node is
-  // evaluated again; the first sweep evaluates everything.
-  const int la = lane & (TILE - 1), lb = lane / TILE;
-  const unsigned long long g_dep = tile_box(la - 2, la + 2, lb - 2, lb + 2);
-  const unsigned g_idep = tile_iface_cols(la - 2, la + 2);
-  unsigned long long i_dep = 0ull;
-  if (i_act) i_dep = tile_box(la - 2, la + 2, i_cc - tz0 - 2, i_cc - tz0 + 3);
-  const unsigned i_idep = tile_iface_cols(la - 2, la + 2);
-  unsigned long long g_pend = ~0ull, i_pend = ~0ull;
-  unsigned g_ipend = 0xFFFFu, i_ipend = 0xFFFFu;
   // Sweeps.  colors == 4: Gauss-Seidel in five phases, the four parity
   // colours of the grid nodes (a node's x/z neighbours and cell diagonals
   // have other colours), then the interface nodes.  colors == 2: the two
@@ -407,50 +500,86 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     bool ch = false;
     for (int phase = 0; phase < nphase; phase++) {
       const bool grid_phase = phase < nphase - 1;
-      bool upd = false;
+      bool upd_any = false;
       if (grid_phase) {
-        Best<double> b;
-        if (g_act && (colors == 1 || (colors == 2 ? ((gi + gj) & 1) : g_color) == phase) &&
-            ((g_pend & g_dep) | (unsigned long long)(g_ipend & g_idep))) {
-          g_pend = 0ull; g_ipend = 0u;
-          if (g_fast) {
+        bool upd[NPL];
+        double rT[NPL], rGX[NPL], rGZ[NPL];
+        int rTie[NPL];
+#pragma unroll
+        for (int k = 0; k < NPL; k++) { upd[k] = false; rT[k] = 0.0; rGX[k] = 0.0; rGZ[k] = 0.0; rTie[k] = 0; }
+        // the lane's nodes of this phase, one per round (two only for
+        // Jacobi with two nodes per lane): the update code is emitted once
+        const int rounds = (NPL > 1 && colors == 1) ? NPL : 1;
+#pragma unroll 1
+        for (int r = 0; r < rounds; r++) {
+          int sel = -1, cnt = 0;
+#pragma unroll
+          for (int k = 0; k < NPL; k++)
+            if (g_act[k] && g_phase[k] == phase) { if (cnt == r) sel = k; cnt++; }
+          bool go = false, fast = false;
+          int ci = 0, cj = 0, cw = 0;
+#pragma unroll
+          for (int k = 0; k < NPL; k++)
+            if (k == sel) {
+              go = g_pend[k].meets(g_dep[k]) || (g_ipend[k] & g_idep[k]);
+              if (go) { g_pend[k].clear(); g_ipend[k] = 0u; }
+              fast = g_fast[k]; ci = gi[k]; cj = gj[k]; cw = wg[k];
+            }
+          if (!go) continue;
+          double t0, gx0, gz0;
+          if constexpr (NPL == 1) { t0 = g_t0; gx0 = g_gx0; gz0 = g_gz0; }
+          else { const int n = lane + 64 * sel; t0 = L.t0[n]; gx0 = L.gx0[n]; gz0 = L.gz0[n]; }
+          Best<double> b;
+          if (fast) {
             RegDyn y;
             for (int q = 0; q < 4; q++) {
-              const int st = (q < 2 ? (q == 0 ? 1 : -1) : (q == 2 ? WIN : -WIN));
-              const int wm = wg + st, ws = wg + 2 * st;  // member, second node (inside the window)
+              const int st = (q < 2 ? (q == 0 ? 1 : -1) : (q == 2 ? WX : -WX));
+              const int wm = cw + st, ws = cw + 2 * st;  // member, second node (inside the window)
               y.mT[q] = L.T[wm]; y.mGX[q] = L.GX[wm]; y.mGZ[q] = L.GZ[wm]; y.mTie[q] = L.tie[wm];
               y.sT[q] = L.T[ws]; y.sGX[q] = L.GX[ws]; y.sGZ[q] = L.GZ[ws];
             }
-            reg_update(g_rs, y, g_t0, g_gx0, g_gz0, b.T, b.gx, b.gz, b.tie);
+            if constexpr (NPL == 1) reg_update(g_rs, y, t0, gx0, gz0, b.T, b.gx, b.gz, b.tie);
+            else reg_update(L.rs[lane + 64 * sel], y, t0, gx0, gz0, b.T, b.gx, b.gz, b.tie);
           } else {
-            b.T = g_t0; b.gx = g_gx0; b.gz = g_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
-            update_grid_node<double>(v, gi, gj, b);
+            b.T = t0; b.gx = gx0; b.gz = gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
+            update_grid_node<double>(v, ci, cj, b);
           }
-          upd = __double_as_longlong(b.T) != __double_as_longlong(L.T[wg]) ||
-                __double_as_longlong(b.gx) != __double_as_longlong(L.GX[wg]) ||
-                __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[wg]) || b.tie != (int)L.tie[wg];
+          const bool u = __double_as_longlong(b.T) != __double_as_longlong(L.T[cw]) ||
+                         __double_as_longlong(b.gx) != __double_as_longlong(L.GX[cw]) ||
+                         __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[cw]) || b.tie != (int)L.tie[cw];
+#pragma unroll
+          for (int k = 0; k < NPL; k++)
+            if (k == sel) { upd[k] = u; rT[k] = b.T; rGX[k] = b.gx; rGZ[k] = b.gz; rTie[k] = b.tie; }
         }
         if (colors == 1) __syncthreads();
-        if (upd) {
-          L.T[wg] = b.T; L.GX[wg] = b.gx; L.GZ[wg] = b.gz; L.tie[wg] = (uint8_t)b.tie;
-          ch = true; g_dirty = true;
+        NodeMask<NPL> chg;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+          if (upd[k]) {
+            L.T[wg[k]] = rT[k]; L.GX[wg[k]] = rGX[k]; L.GZ[wg[k]] = rGZ[k]; L.tie[wg[k]] = (uint8_t)rTie[k];
+            ch = true; g_dirty[k] = true;
+          }
+          chg.w[k] = __ballot(upd[k]);
         }
-        const unsigned long long chg = __ballot(upd);
-        g_pend |= chg; i_pend |= chg;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) g_pend[k] |= chg;
+        i_pend |= chg;
       } else {
-        if (i_act && ((i_pend & i_dep) | (unsigned long long)(i_ipend & i_idep))) {
-          i_pend = 0ull; i_ipend = 0u;
+        if (i_act && (i_pend.meets(i_dep) || (i_ipend & i_idep))) {
+          i_pend.clear(); i_ipend = 0u;
           Best<double> b;
           b.T = i_t0; b.gx = i_gx0; b.gz = i_gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
           update_iface_node<double>(v, is, ic, b);
           if (__double_as_longlong(b.T) != __double_as_longlong(L.iT[wi]) ||
               __double_as_longlong(b.gx) != __double_as_longlong(L.iGX[wi]) ||
               __double_as_longlong(b.gz) != __double_as_longlong(L.iGZ[wi])) {
-            L.iT[wi] = b.T; L.iGX[wi] = b.gx; L.iGZ[wi] = b.gz; ch = true; i_dirty = true; upd = true;
+            L.iT[wi] = b.T; L.iGX[wi] = b.gx; L.iGZ[wi] = b.gz; ch = true; i_dirty = true; upd_any = true;
           }
         }
-        const unsigned ichg = (unsigned)(__ballot(upd) & 0xFFFFull);
-        g_ipend |= ichg; i_ipend |= ichg;
+        const unsigned ichg = (unsigned)(__ballot(upd_any) & IMASK);
+#pragma unroll
+        for (int k = 0; k < NPL; k++) g_ipend[k] |= ichg;
+        i_ipend |= ichg;
       }
       __syncthreads();
     }
@@ -468,32 +597,33 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     if (launch_max) atomicMax(launch_max, sweep + 1);
   }
   // ---- write back and activate neighbours whose halo changed
-  if (g_dirty) {
-    const int q = gj * nx + gi;
-    d.T[q] = L.T[wg]; d.GX[q] = L.GX[wg]; d.GZ[q] = L.GZ[wg]; d.TIE[q] = L.tie[wg];
-  }
-  if (i_dirty) {
-    const int q = (nz + is) * nx + ic;
-    d.T[q] = L.iT[wi]; d.GX[q] = L.iGX[wi]; d.GZ[q] = L.iGZ[wi];
-  }
-  // border position of each dirty node (interface nodes at their home row)
-  int li = -1, lj = -1;
-  if (g_dirty) { li = gi - tx0; lj = gj - tz0; }
-  const int li2 = i_dirty ? ic - tx0 : -1, lj2 = i_dirty ? i_home - tz0 : -1;
   auto edge = [&](int a, int b) {
     unsigned m = 0;
     if (a < 0) return m;
-    const bool L0 = a < HALO, R0 = a >= TILE - HALO, T0 = b < HALO, B0 = b >= TILE - HALO;
+    const bool L0 = a < HALO, R0 = a >= TX - HALO, T0 = b < HALO, B0 = b >= TZ - HALO;
     if (L0) m |= 1; if (R0) m |= 2; if (T0) m |= 4; if (B0) m |= 8;
     if (L0 && T0) m |= 16; if (R0 && T0) m |= 32; if (L0 && B0) m |= 64; if (R0 && B0) m |= 128;
     return m;
   };
-  unsigned mask = edge(li, lj) | edge(li2, lj2);
-  if (more) mask |= 256;  // sweep cap reached: run this tile again
+  unsigned mask = 0;
   // activation key: earliest time among this tile's changed nodes
   double kmin = 1.0e300;
-  if (g_dirty) kmin = L.T[wg];
-  if (i_dirty) kmin = fmin(kmin, L.iT[wi]);
+#pragma unroll
+  for (int k = 0; k < NPL; k++) {
+    if (!g_dirty[k]) continue;
+    const int q = gj[k] * nx + gi[k];
+    d.T[q] = L.T[wg[k]]; d.GX[q] = L.GX[wg[k]]; d.GZ[q] = L.GZ[wg[k]]; d.TIE[q] = L.tie[wg[k]];
+    mask |= edge(gi[k] - tx0, gj[k] - tz0);
+    kmin = fmin(kmin, L.T[wg[k]]);
+  }
+  if (i_dirty) {
+    const int q = (nz + is) * nx + ic;
+    d.T[q] = L.iT[wi]; d.GX[q] = L.iGX[wi]; d.GZ[q] = L.iGZ[wi];
+    // border position of a dirty interface node: its home row
+    mask |= edge(ic - tx0, i_home - tz0);
+    kmin = fmin(kmin, L.iT[wi]);
+  }
+  if (more) mask |= 256;  // sweep cap reached: run this tile again
   for (int off = 32; off > 0; off >>= 1) {
     mask |= (unsigned)__shfl_xor((int)mask, off, 64);
     kmin = fmin(kmin, __shfl_xor(kmin, off, 64));
@@ -511,6 +641,7 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
 
 // Persistent-style grid: blocks take the due tiles of this launch in turn
 // (the due count is known only on the device).
+template <int TX, int TZ>
 __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
                                                        const RegionDev* __restrict__ regs, const int2* __restrict__ list,
                                                        const int* __restrict__ count_due, int ntx, int ntz,
@@ -518,14 +649,14 @@ __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restri
                                                        int2* list_next, int* count_next, int max_sweeps,
                                                        int max_sweeps_heavy, int colors, unsigned long long* acc,
                                                        int* launch_max, int due_cap) {
-  __shared__ TileLds L;
+  __shared__ TileLds<TX, TZ> L;
   const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
     // a tile cut by an interface gets its own sweep cap: its visits are the
     // slow ones, and a capped visit continues in a later launch
-    fim_tile(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, k < nh ? max_sweeps_heavy : max_sweeps,
-             colors, acc, launch_max, L);
+    fim_tile<TX, TZ>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next,
+                     k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, L);
     __syncthreads();
   }
 }
@@ -1021,11 +1152,11 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
 }
 
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
-                             unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
-                             hipStream_t st) {
+                             int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
+                             int* count_next, hipStream_t st) {
   const int total = nfield * nx;
   hipLaunchKernelGGL(seed_tiles_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, nfield, ntx, ntz,
-                     in_set, tkey, list_next, count_next);
+                     tx, tz, in_set, tkey, list_next, count_next);
   return hipGetLastError();
 }
 
@@ -1043,11 +1174,23 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
 }
 
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
-                      int ntx, int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                      int* count_next, int max_sweeps, int max_sweeps_heavy, int colors, unsigned long long* acc,
-                      int* launch_max, int due_cap, hipStream_t st) {
-  hipLaunchKernelGGL(fim_tiles_kernel, dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz, in_set,
-                     tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max, due_cap);
+                      int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
+                      int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
+                      unsigned long long* acc, int* launch_max, int due_cap, hipStream_t st) {
+  if (tx == 8 && tz == 8)
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 8>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
+                       due_cap);
+  else if (tx == 16 && tz == 8)
+    hipLaunchKernelGGL((fim_tiles_kernel<16, 8>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
+                       due_cap);
+  else if (tx == 8 && tz == 16)
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 16>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
+                       due_cap);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -50,6 +50,12 @@ class AdjointStats(C.Structure):
                 ("ms_total", C.c_double), ("peak_bytes", C.c_longlong)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_x", C.c_int), ("tile_z", C.c_int), ("max_sweeps", C.c_int), ("max_sweeps_heavy", C.c_int),
+                ("colors", C.c_int), ("window_tiles", C.c_double), ("ms_best", C.c_double), ("ms_start", C.c_double),
+                ("trials", C.c_int)]
+
+
 class SystemParams(C.Structure):
     _fields_ = [("crit_chi", C.c_double), ("phase_wt_psed", C.c_double), ("phase_wt_pmp", C.c_double),
                 ("phase_wt_pgss", C.c_double), ("refl_weight", C.c_double), ("alt_mode", C.c_int)]
@@ -125,6 +131,8 @@ def load() -> C.CDLL:
     lib.fmt2d_set_frechet_chunk.argtypes = [C.c_void_p, C.c_longlong]
     lib.fmt2d_set_heavy_sweeps.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.fmt2d_autotune.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(TuneResult)]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
     lib.fmt2d_ray_info.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip]
@@ -153,7 +161,8 @@ def load() -> C.CDLL:
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
-           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap",
+           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_tile",
+           "fmt2d_autotune",
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
@@ -219,6 +228,18 @@ class Context:
 
     def set_heavy_sweeps(self, max_sweeps_heavy: int = 8):
         _check(self._lib.fmt2d_set_heavy_sweeps(self._h, int(max_sweeps_heavy)))
+
+    def set_tile(self, tile_x: int = 8, tile_z: int = 8):
+        _check(self._lib.fmt2d_set_tile(self._h, int(tile_x), int(tile_z)))
+
+    def autotune(self, src_x, flags: int = 0, level: int = 0) -> TuneResult:
+        """Pick the fastest tile schedule for these sources (fmt2d_autotune);
+        the context keeps it and ends solved for the sources."""
+        x = np.ascontiguousarray(src_x, dtype=np.float64)
+        r = TuneResult()
+        self._nray = 0   # a forward drops the rays of the last trace
+        _check(self._lib.fmt2d_autotune(self._h, len(x), _p(x), int(flags), int(level), C.byref(r)))
+        return r
 
     def set_launch_cap(self, max_launches: int = 0):
         _check(self._lib.fmt2d_set_launch_cap(self._h, int(max_launches)))

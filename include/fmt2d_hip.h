@@ -118,6 +118,27 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
  * identical for any cap).  Default 8.  (Solver tuning of this build, no
  * counterpart in the reference's heap-ordered march.) */
 int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy);
+/* Tile shape of the solver: 8x8 (default; one node per lane of a
+ * wavefront), 16x8 or 8x16 (two nodes per lane, one of each checkerboard
+ * colour).  Results are identical for every shape.  (Solver tuning of this
+ * build, no counterpart in the reference.) */
+int fmt2d_set_tile(fmt2d_ctx* ctx, int tile_x, int tile_z);
+typedef struct {
+  int tile_x, tile_z;       /* chosen tile shape */
+  int max_sweeps, max_sweeps_heavy, colors;
+  double window_tiles;
+  double ms_best;           /* forward wall time with the chosen schedule */
+  double ms_start;          /* forward wall time with the schedule set on entry */
+  int trials;               /* forwards run */
+} fmt2d_tune_result;
+/* Autotuner of the tile schedule on the model and sources at hand (no
+ * counterpart in the reference): times one forward per candidate by
+ * coordinate descent over the tile shape (8x8, 16x8, 8x16), the sweep caps
+ * of light and heavy tiles and, with level 1, the arrival window and the
+ * in-tile order; keeps the fastest in the context.  Every schedule gives
+ * the same fields bit for bit, so the context ends solved for these
+ * sources.  About 11 forwards at level 0, 16 at level 1. */
+int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int level, fmt2d_tune_result* out);
 /* Solver launches allowed per stage level before fmt2d_forward gives up with
  * FMT2D_ENOCONV (0, the default: 64 x (tile columns + tile rows) + 4000).  A
  * forward that gave up leaves the context usable: the next forward starts

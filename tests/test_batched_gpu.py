@@ -16,9 +16,10 @@ one fmt2d_forward.
   along the Moho refit the gradients of their neighbours: the bar is a mean
   below 0.02 and fewer than 1 % of nodes above 0.05 per field; measured:
   mean at most 0.012, share at most 0.6 %).
-* Schedule invariance: another arrival window, sweep caps and the four-colour
-  in-tile order give the same bits in every one of the 2400 fields (hash of
-  T and both gradient components over all slots)."""
+* Schedule invariance: another tile shape (16 x 8, two nodes per lane),
+  arrival window, sweep caps and the four-colour in-tile order give the same
+  bits in every one of the 2400 fields (hash of T and both gradient
+  components over all slots)."""
 import os
 
 import numpy as np
@@ -84,11 +85,13 @@ def test_schedule_invariance_all_fields(batched):
     ctx, summ = batched["ctx"], batched["summary"]
     obs = list(batched["obs"])
     try:
-        ctx.set_tuning(1.5, 12, 4)
-        ctx.set_heavy_sweeps(6)
+        ctx.set_tile(16, 8)
+        ctx.set_tuning(1.5, 16, 4)
+        ctx.set_heavy_sweeps(12)
         ctx.forward(obs)
         other = ctx.summary()
     finally:
+        ctx.set_tile()
         ctx.set_tuning()
         ctx.set_heavy_sweeps()
     same = other["hash"] == summ["hash"]

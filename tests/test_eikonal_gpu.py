@@ -93,24 +93,48 @@ def test_multistage_fields(ms_fields, isrc):
             print(f"tf{itf}: {nb} of {fin.sum()} gradients differ in the last bits (max rel {g.max():.1e})")
 
 
-@pytest.mark.parametrize("tuning", [(2.0, 6, 4, 12), (3.0, 12, 2, 3)])
+@pytest.mark.parametrize("tuning", [(2.0, 6, 4, 12, (8, 8)), (3.0, 12, 2, 3, (8, 8)), (3.0, 16, 2, 16, (16, 8)),
+                                    (2.0, 24, 1, 24, (8, 16)), (4.0, 8, 4, 8, (16, 8))])
 def test_schedule_does_not_change_the_fields(fields, tuning):
-    """The tile schedule (window, sweep caps, in-tile sweep order) decides
-    only how the fixed point is reached: every field of the plain case is
-    the same, bit for bit, as with the default schedule."""
+    """The tile schedule (tile shape, window, sweep caps, in-tile sweep
+    order) decides only how the fixed point is reached: every field of the
+    plain case is the same, bit for bit, as with the default schedule."""
     ctx, _ = fields
-    window, light, colors, heavy = tuning
+    window, light, colors, heavy, tile = tuning
     before = [np.concatenate(ctx.field(i, f)).view(np.uint64).copy() for i in range(len(SOURCES)) for f in range(1, 13)]
     try:
+        ctx.set_tile(*tile)
         ctx.set_tuning(window, light, colors)
         ctx.set_heavy_sweeps(heavy)
         ctx.forward(SOURCES)
         after = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
     finally:
+        ctx.set_tile()
         ctx.set_tuning()
         ctx.set_heavy_sweeps()
         ctx.forward(SOURCES)
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_autotune_keeps_the_fields(fields):
+    """fmt2d_autotune times the candidate schedules on this model, keeps the
+    fastest and leaves the context solved: the fields are the default
+    schedule's, bit for bit."""
+    ctx, _ = fields
+    before = [np.concatenate(ctx.field(i, f)).view(np.uint64).copy() for i in range(len(SOURCES)) for f in range(1, 13)]
+    try:
+        r = ctx.autotune(SOURCES, level=1)
+        print(f"autotune: tile {r.tile_x}x{r.tile_z}, caps {r.max_sweeps}/{r.max_sweeps_heavy}, colours {r.colors}, "
+              f"window {r.window_tiles}, {r.trials} forwards, {r.ms_start:.1f} -> {r.ms_best:.1f} ms")
+        assert (r.tile_x, r.tile_z) in ((8, 8), (16, 8), (8, 16))
+        assert r.trials >= 11 and r.ms_best <= r.ms_start
+        after = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
+        assert all(np.array_equal(a, b) for a, b in zip(before, after))
+    finally:
+        ctx.set_tile()
+        ctx.set_tuning()
+        ctx.set_heavy_sweeps()
+        ctx.forward(SOURCES)
 
 
 def test_forward_after_launch_cap_matches_fresh(fields):
