@@ -456,6 +456,7 @@ def joint_figure(iters: int, cpu: bool = False):
     dm_sum = np.zeros(n_par)
     first = 1
     log, phases = [], {"forward": [], "rays": [], "system": [], "lsqr": [], "line_search": []}
+    modes = {}
     for it in range(1, iters + 1):
         mode = 2 if it % 4 == 0 else 1        # three velocity iterations, then one interface iteration
         t0 = time.perf_counter()
@@ -469,18 +470,29 @@ def joint_figure(iters: int, cpu: bool = False):
         dm, dres = ctx.damped_update(dm_sum, lam_damp=0.1, lam_damp_iface=0.1, smooth_type=3, w_smooth=0.3,
                                      w_smooth_iface=0.3, max_iter=400, corr_x=10.0, corr_z=3.0, corr_x_iface=8.0)
         t4 = time.perf_counter()
+        if it == iters:
+            # the last iteration's system solved once more in each LSQR
+            # reduction mode (fmt2d_set_lsqr_mode): ordered (the reference's
+            # rounding) and parallel (tree / wave reductions), same damping
+            for lm in (0, 1):
+                ctx.set_lsqr_mode(lm)
+                modes[lm] = ctx.lsqr(dm_sum, wdv=dres.wd, wdv_iface=dres.wd_iface, w_smooth=0.3,
+                                     w_smooth_iface=0.3, smooth_type=3, max_iter=400, corr_x=10.0, corr_z=3.0,
+                                     corr_x_iface=8.0)
+            ctx.set_lsqr_mode(0)
+        t4b = time.perf_counter()
         alive = np.ascontiguousarray(valid, dtype=np.int32)
         dm, r = ctx.line_search(t_obs, dm, crit_chi=1.0, first_step=first, valid=alive)
         t5 = time.perf_counter()
         first = r.next_first_step
         dm_sum += dm
-        for k, v in zip(phases, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
+        for k, v in zip(phases, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4b)):
             phases[k].append(v)
         log.append({"iteration": it, "mode": "velocity" if mode == 1 else "interfaces", "ndata": nd, "nnz": nnz,
                     "rms": round(r.rms_orig, 6), "rms_new": round(r.rms_new, 6), "step": r.step,
                     "lsqr_iterations": dres.iterations, "line_search_forwards": r.forwards,
                     "forward_reused": bool(reused),
-                    "seconds": round(t5 - t0, 3)})
+                    "seconds": round(t5 - t0 - (t4b - t4), 3)})
         print(f"joint iteration {it}: {log[-1]}", file=sys.stderr, flush=True)
     ctx.close()
     tot = [sum(v) for v in zip(*phases.values())]
@@ -498,6 +510,15 @@ def joint_figure(iters: int, cpu: bool = False):
                 "algorithmic_bytes_per_iteration": int(lsqr_bytes),
                 "achieved_GBps": round(lsqr_bytes / (lsqr_ms_it * 1e-3) / 1e9, 2),
                 "frac_of_hbm_peak": round(lsqr_bytes / (lsqr_ms_it * 1e-3) / 1e9 / PEAK_HBM_GBS, 5)}
+    last_bytes = 2 * log[-1]["nnz"] * 12 + (3 * (log[-1]["ndata"] + 2 * n_par) + 8 * n_par) * 8
+    (dm0, s0), (dm1, s1) = modes[0], modes[1]
+    for key, (dmx, sx) in (("last_system_ordered", modes[0]), ("last_system_parallel", modes[1])):
+        ms_it = sx.ms / max(sx.iterations, 1)
+        lsqr_fig[key] = {"iterations": sx.iterations, "ms_per_iteration": round(ms_it, 3),
+                         "achieved_GBps": round(last_bytes / (ms_it * 1e-3) / 1e9, 2),
+                         "frac_of_hbm_peak": round(last_bytes / (ms_it * 1e-3) / 1e9 / PEAK_HBM_GBS, 5)}
+    lsqr_fig["last_system_parallel"]["max_rel_dm_vs_ordered"] = float(
+        np.max(np.abs(dm1 - dm0)) / max(np.max(np.abs(dm0)), 1e-300))
     cpu_b = joint_cpu_sample(m0, g, t_obs) if cpu else None
     return {"metric": "joint inversion seconds per iteration", "value": round(float(np.mean(tot)), 3), "unit": "s",
             "higher_is_better": False,

@@ -238,6 +238,12 @@ int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual) {
   return FMT2D_OK;
 }
 
+int fmt2d_set_lsqr_mode(fmt2d_ctx* ctx, int mode) {
+  if (!ctx || (mode != 0 && mode != 1)) return fail(FMT2D_EINVAL, "LSQR mode must be 0 (ordered) or 1 (parallel)");
+  ctx->lsqr_fast = mode;
+  return FMT2D_OK;
+}
+
 int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats) {
   return lsqr_solve(ctx, p, dm_sum, dm_out, stats);
@@ -358,6 +364,7 @@ static int lsqr_prepare(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double
   }
   if (want_i) { S.rif_rp = ctx->gi_rp.p; S.rif_col = ctx->gi_col.p; S.rif_val = ctx->gi_val.p; }
   S.gsi = (p->smooth_type == 2 && want_i) ? 1 : 0;
+  S.fast = ctx->lsqr_fast;
   return FMT2D_OK;
 }
 
@@ -374,7 +381,9 @@ static int lsqr_work(fmt2d_ctx* ctx, int q, size_t na, LsqrWork& W) {
   for (int k = 0; k < 8; k++)
     if (B[k]->n < need[k]) HIPCHK(B[k]->alloc(need[k]));
   if (!sc.p) HIPCHK(sc.alloc(1));
-  W = LsqrWork{B[0]->p, B[1]->p, B[2]->p, B[3]->p, B[4]->p, B[5]->p, B[6]->p, B[7]->p, sc.p};
+  DBuf<double>& part = q == 0 ? ctx->l_part : ctx->l2_part;
+  if (!part.p) HIPCHK(part.alloc(2 * LSQR_NPART));
+  W = LsqrWork{B[0]->p, B[1]->p, B[2]->p, B[3]->p, B[4]->p, B[5]->p, B[6]->p, B[7]->p, sc.p, part.p};
   return FMT2D_OK;
 }
 

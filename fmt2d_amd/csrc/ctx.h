@@ -124,6 +124,7 @@ struct fmt2d_ctx {
   int heavy_tx = 0, heavy_tz = 0;     // shape the slow-tile flags (theavy) were built for
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
+  int lsqr_fast = 0;                  // LSQR reductions: 0 in the reference's order, 1 tree / wave reductions
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
   DBuf<int> stop_flag;                // set when a time is NaN (the reference's stop case)
   DBuf<int> sel_which, sel_up;
@@ -193,6 +194,7 @@ struct fmt2d_ctx {
   // second solve running beside the first (damping searches)
   DBuf<double> l2_u, l2_v, l2_w, l2_dm, l2_gv, l2_gtu, l2_sm, l2_sm2, l2_wd;
   DBuf<LsqrScalars> l2_sc;
+  DBuf<double> l_part, l2_part;       // partial sums of the tree-reduced norms
   hipStream_t stream2 = nullptr;
   // inversion parameterisation (frechet.in): velocity grid and interfaces;
   // n_vel_par / n_par count the Frechet columns (grid nodes, interface

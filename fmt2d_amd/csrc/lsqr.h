@@ -38,12 +38,17 @@ struct LsqrSys {
   const int *rif_rp, *rif_col;                   // R_if rows (nullptr: not built)
   const double* rif_val;
   int gsi;                                       // smooth_type 2: R_if in use
+  // 0: every reduction in the reference's order (bit-exact model update);
+  // 1: tree-reduced norms and wave-reduced sparse products (other rounding)
+  int fast;
 };
 
 struct LsqrWork {
   double *u, *v, *w, *dm, *gv, *gtu, *sm, *sm2;
   LsqrScalars* sc;
+  double* part;   // partial sums of the tree-reduced norms (2 LSQR_NPART: high and low parts)
 };
+constexpr int LSQR_NPART = 1024;
 
 hipError_t lsqr_run(const LsqrSys& S, LsqrWork& W, int max_iter, double atol, int poll, hipStream_t st,
                     LsqrScalars* host_out);

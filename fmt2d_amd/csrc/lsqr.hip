@@ -10,10 +10,13 @@
 // does not round differently), and the scalar recurrences run in one
 // thread.  The host launches a fixed sequence per iteration and polls the
 // stop flag every few iterations; kernels of iterations after the stop do
-// nothing, so the iteration count is the reference's.
+// nothing, so the iteration count is the reference's.  A context setting
+// (fmt2d_set_lsqr_mode) switches the norms and sparse products to tree and
+// wave reductions instead: faster, with rounding of their own.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "lsqr.h"
@@ -164,6 +167,90 @@ __global__ void __launch_bounds__(256) seqnorm_kernel(const double* __restrict__
     __syncthreads();
   }
   if (t == 0) *dst = acc;
+}
+
+// Parallel-reduction mode: the norms as trees (blocks of 256, then the
+// block sums) and the sparse products with a wave reduction per row.  Every
+// partial carries its rounding error (two-sum and the fma residue of each
+// product: double-double accumulation), so the sums are close to exact and
+// differ from the ordered ones by about the ordered sums' own rounding.
+struct DD {
+  double hi, lo;
+};
+__device__ inline DD dd_add(DD a, double b) {
+  const double s = a.hi + b, bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b - bb);
+  return DD{s, a.lo + e};
+}
+__device__ inline DD dd_add(DD a, DD b) {
+  DD r = dd_add(a, b.hi);
+  r.lo += b.lo;
+  return r;
+}
+__device__ inline DD dd_fma(DD acc, double x, double y) {
+  const double p = x * y;
+  const double e = fma(x, y, -p);
+  DD r = dd_add(acc, p);
+  r.lo += e;
+  return r;
+}
+__device__ inline DD dd_shfl_xor(DD a, int o) { return DD{__shfl_xor(a.hi, o, 64), __shfl_xor(a.lo, o, 64)}; }
+
+__global__ void __launch_bounds__(256) tree_norm_part_kernel(const double* __restrict__ a, long long n,
+                                                            double* part, const LsqrScalars* s) {
+  if (stopped(s)) return;
+  __shared__ double rh[256], rl[256];
+  DD acc{0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double x = a[i];
+    acc = dd_fma(acc, x, x);
+  }
+  rh[threadIdx.x] = acc.hi;
+  rl[threadIdx.x] = acc.lo;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const DD r = dd_add(DD{rh[threadIdx.x], rl[threadIdx.x]}, DD{rh[threadIdx.x + o], rl[threadIdx.x + o]});
+      rh[threadIdx.x] = r.hi;
+      rl[threadIdx.x] = r.lo;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rh[0]; part[2 * blockIdx.x + 1] = rl[0]; }
+}
+
+__global__ void __launch_bounds__(256) tree_norm_final_kernel(const double* __restrict__ part, int np, double* dst,
+                                                             const LsqrScalars* s) {
+  if (stopped(s)) return;
+  __shared__ double rh[256], rl[256];
+  DD acc{0.0, 0.0};
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc = dd_add(acc, DD{part[2 * i], part[2 * i + 1]});
+  rh[threadIdx.x] = acc.hi;
+  rl[threadIdx.x] = acc.lo;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const DD r = dd_add(DD{rh[threadIdx.x], rl[threadIdx.x]}, DD{rh[threadIdx.x + o], rl[threadIdx.x + o]});
+      rh[threadIdx.x] = r.hi;
+      rl[threadIdx.x] = r.lo;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dst = rh[0] + rl[0];
+}
+
+// y[i] = sum_k M(i,k) x(idx k): one wave per row, lanes stride the row,
+// then a butterfly over the wave
+__global__ void __launch_bounds__(64) wave_spmv_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                      const double* __restrict__ val, int nrow,
+                                                      const double* __restrict__ x, double* y, const LsqrScalars* s) {
+  const int i = blockIdx.x;
+  if (i >= nrow || stopped(s)) return;
+  const int a = ptr[i], b = ptr[i + 1];
+  DD acc{0.0, 0.0};
+  for (int k = a + (int)threadIdx.x; k < b; k += 64) acc = dd_fma(acc, val[k], x[idx[k]]);
+  for (int o = 32; o > 0; o >>= 1) acc = dd_add(acc, dd_shfl_xor(acc, o));
+  if (threadIdx.x == 0) y[i] = acc.hi + acc.lo;
 }
 
 // ---- element-wise steps
@@ -320,6 +407,25 @@ __global__ void test_kernel(LsqrScalars* s, int iter, int max_iter, double atol)
 
 static inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// sum of squares of a[0..n) into *dst, in the mode of the system
+static void launch_norm(const LsqrSys& S, LsqrWork& W, const double* a, long long n, double* dst, hipStream_t st) {
+  if (S.fast) {
+    const int g = (int)std::min<long long>(LSQR_NPART, std::max<long long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(tree_norm_part_kernel, dim3(g), dim3(256), 0, st, a, n, W.part, W.sc);
+    hipLaunchKernelGGL(tree_norm_final_kernel, dim3(1), dim3(256), 0, st, W.part, g, dst, W.sc);
+  } else {
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, a, n, dst, W.sc);
+  }
+}
+
+static void launch_spmv(const LsqrSys& S, const int* ptr, const int* idx, const double* val, int nrow,
+                        const double* x, double* y, const LsqrScalars* sc, hipStream_t st) {
+  if (S.fast)
+    hipLaunchKernelGGL(wave_spmv_kernel, dim3((unsigned)nrow), dim3(64), 0, st, ptr, idx, val, nrow, x, y, sc);
+  else
+    hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nrow), dim3(64), 0, st, ptr, idx, val, nrow, x, y, sc);
+}
+
 // The kernels of the start of a solve: u = [r; -w L dm_sum; -wd dm_sum]
 // (Gaussian: [r; -w R_h dm_sum; -w R_v dm_sum; -wd dm_sum]), beta, u /= beta,
 // v = A^T u masked, alpha, v /= alpha, w = v.
@@ -334,15 +440,14 @@ static hipError_t launch_start(const LsqrSys& S, LsqrWork& W, int max_iter, doub
   else if (S.smooth_on)
     hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, S.dm_sum, W.sm, sc);
   hipLaunchKernelGGL(rhs_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.sm, W.u);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
+  launch_norm(S, W, W.u, na, &sc->norm, st);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 0, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 0, sc);
-  hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
-                     W.gtu, sc);
+  launch_spmv(S, S.cp, S.crow, S.cval, S.nparm, W.u, W.gtu, sc, st);
   if (!gauss && S.smooth_on)
     hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
   hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 1, sc);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
+  launch_norm(S, W, W.v, np, &sc->norm, st);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 1, 0, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 0, sc);
   hipLaunchKernelGGL(w_init_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm);
@@ -356,23 +461,20 @@ static hipError_t launch_iteration(const LsqrSys& S, LsqrWork& W, int it, int ma
   const long long nd = S.ndata, np = S.nparm, na = nd + (gauss ? 3 : 2) * np;
   LsqrScalars* sc = W.sc;
   hipLaunchKernelGGL(mask_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, sc);
-  if (nd > 0)
-    hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)nd), dim3(64), 0, st, S.rp, S.col, S.val, S.ndata, W.v,
-                       W.gv, sc);
+  if (nd > 0) launch_spmv(S, S.rp, S.col, S.val, S.ndata, W.v, W.gv, sc, st);
   if (gauss)
     hipLaunchKernelGGL(gauss_fwd_kernel, dim3(nb(np, 128)), dim3(128), 0, st, S, W.v, W.sm, W.sm2, 0, sc);
   else if (S.smooth_on)
     hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.sm, sc);
   hipLaunchKernelGGL(u_update_kernel, dim3(nb(na, 256)), dim3(256), 0, st, S, W.gv, W.sm, W.sm2, W.v, W.u, sc);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.u, na, &sc->norm, sc);
+  launch_norm(S, W, W.u, na, &sc->norm, st);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 2, it, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(na, 256)), dim3(256), 0, st, W.u, na, &sc->beta, 1, sc);
-  hipLaunchKernelGGL(ordered_spmv_kernel, dim3((unsigned)np), dim3(64), 0, st, S.cp, S.crow, S.cval, S.nparm, W.u,
-                     W.gtu, sc);
+  launch_spmv(S, S.cp, S.crow, S.cval, S.nparm, W.u, W.gtu, sc, st);
   if (!gauss && S.smooth_on)
     hipLaunchKernelGGL(smooth_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.u + nd, W.sm, sc);
   hipLaunchKernelGGL(v_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.gtu, W.sm, W.u, W.v, 0, sc);
-  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, W.v, np, &sc->norm, sc);
+  launch_norm(S, W, W.v, np, &sc->norm, st);
   hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, sc, 3, it, max_iter, atol);
   hipLaunchKernelGGL(scale_kernel, dim3(nb(np, 256)), dim3(256), 0, st, W.v, np, &sc->alpha, 1, sc);
   hipLaunchKernelGGL(w_update_kernel, dim3(nb(np, 256)), dim3(256), 0, st, S, W.v, W.w, W.dm, sc);

@@ -132,6 +132,7 @@ def load() -> C.CDLL:
     lib.fmt2d_set_heavy_sweeps.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.fmt2d_set_lsqr_mode.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_autotune.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(TuneResult)]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
@@ -162,7 +163,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
            "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_tile",
-           "fmt2d_autotune",
+           "fmt2d_autotune", "fmt2d_set_lsqr_mode",
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
@@ -240,6 +241,11 @@ class Context:
         self._nray = 0   # a forward drops the rays of the last trace
         _check(self._lib.fmt2d_autotune(self._h, len(x), _p(x), int(flags), int(level), C.byref(r)))
         return r
+
+    def set_lsqr_mode(self, mode: int = 0):
+        """0: LSQR reductions in the reference's order (bit-exact); 1: tree /
+        wave reductions."""
+        _check(self._lib.fmt2d_set_lsqr_mode(self._h, int(mode)))
 
     def set_launch_cap(self, max_launches: int = 0):
         _check(self._lib.fmt2d_set_launch_cap(self._h, int(max_launches)))
