@@ -40,6 +40,34 @@ int build_ray_geom(fmt2d_ctx* ctx) {
     RayRegionDev& d = G.reg[r];
     d.id = r; d.nnode = R.nnode; d.itop = R.itop; d.ibot = R.ibot;
     d.order = ctx->rorder[r].p; d.flags = ctx->rflags[r].p; d.vel = ctx->rvel[r].p;
+    // column index of the nearest-node search: the run of region rows per
+    // column and bounds on the x of the nodes of the columns from / up to c
+    const int nx = m.nx, nz = m.nz;
+    std::vector<int> jlo(nx, 1), jhi(nx, 0);
+    std::vector<double> xlo(nx), xhi(nx);
+    bool runs = true;
+    const HostInterface& T = m.isec[R.itop];
+    const HostInterface& B = m.isec[R.ibot];
+    for (int c = 0; c < nx; c++) {
+      int lo = -1, hi = -1, cnt = 0;
+      for (int j = 0; j < nz; j++)
+        if (R.flags[(size_t)j * nx + c] & RF_IN) { if (lo < 0) lo = j; hi = j; cnt++; }
+      if (lo >= 0) {
+        jlo[c] = lo; jhi[c] = hi;
+        if (cnt != hi - lo + 1) runs = false;
+      }
+      double a = std::min(T.x[c], B.x[c]), b = std::max(T.x[c], B.x[c]);
+      if (lo >= 0) { a = std::min(a, m.xg[c]); b = std::max(b, m.xg[c]); }
+      xlo[c] = a; xhi[c] = b;
+    }
+    for (int c = nx - 2; c >= 0; c--) xlo[c] = std::min(xlo[c], xlo[c + 1]);
+    for (int c = 1; c < nx; c++) xhi[c] = std::max(xhi[c], xhi[c - 1]);
+    HIPCHK(upload(ctx->rjlo[r], jlo, st));
+    HIPCHK(upload(ctx->rjhi[r], jhi, st));
+    HIPCHK(upload(ctx->rxlo[r], xlo, st));
+    HIPCHK(upload(ctx->rxhi[r], xhi, st));
+    d.indexed = (runs && (int)T.x.size() == nx && (int)B.x.size() == nx && ctx->ray_scan == 0) ? 1 : 0;
+    d.jlo = ctx->rjlo[r].p; d.jhi = ctx->rjhi[r].p; d.xlo_from = ctx->rxlo[r].p; d.xhi_upto = ctx->rxhi[r].p;
   }
   G.reg[0] = G.reg[1];
   HIPCHK(upload(ctx->rsfx, m.if_x[2], st));
@@ -501,12 +529,18 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   if (ctx->rout.n < (size_t)nray) HIPCHK(ctx->rout.alloc((size_t)nray));
   const int nsf = ctx->hgeom.nsf;
   if (ctx->rtsf.n < (size_t)ctx->nsrc * 3 * nsf) HIPCHK(ctx->rtsf.alloc((size_t)ctx->nsrc * 3 * nsf));
+  HIPCHK(nearest_calls_reset(st));
   HIPCHK(launch_seafloor_times(ctx->rgeom.p, ctx->rfields.p, ctx->nsrc, nsf, ctx->rtsf.p, st));
   HIPCHK(launch_trace(ctx->rgeom.p, ctx->rfields.p, ctx->d_srcx.p, ctx->d_srcz.p, ctx->rtsf.p, ctx->rjobs.p, nray,
                       ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, ctx->fwd_program ? 1 : 0, st));
   std::vector<RayOut> out(nray);
   HIPCHK(hipMemcpyAsync(out.data(), ctx->rout.p, sizeof(RayOut) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  {
+    unsigned long long nc = 0;
+    HIPCHK(nearest_calls_read(&nc, st));
+    ctx->ray_nearest_calls = (long long)nc;
+  }
   ctx->h_tcalc.resize(nray);
   ctx->h_valid.resize(nray);
   for (int r = 0; r < nray; r++) {
@@ -516,6 +550,43 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   ctx->g_nnz = -1;
   ctx->gp_nnz = -1;
   ctx->sys_ndata = -1;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_ray_search(fmt2d_ctx* ctx, int mode) {
+  if (!ctx || (mode != 0 && mode != 1)) return fail(FMT2D_EINVAL, "search mode must be 0 (indexed) or 1 (scan)");
+  if (ctx->ray_scan != mode) {
+    ctx->ray_scan = mode;
+    ctx->rgeom_ok = false;
+  }
+  return FMT2D_OK;
+}
+
+int fmt2d_ray_nearest_check(fmt2d_ctx* ctx, int region, int n, const double* px, const double* pz, int* idx,
+                            int* scan) {
+  if (!ctx || n < 0 || (n > 0 && (!px || !pz || !idx || !scan))) return fail(FMT2D_EINVAL, "NULL argument");
+  if (region < 1 || region > NREG) return fail(FMT2D_EINVAL, "region must be 1..6");
+  if (!ctx->have_model) return fail(FMT2D_ESTATE, "no model set");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->rgeom_ok) { const int rc = build_ray_geom(ctx); if (rc) return rc; }
+  if (n == 0) return FMT2D_OK;
+  hipStream_t st = ctx->stream;
+  DBuf<double> dx, dz;
+  DBuf<int> di, ds;
+  HIPCHK(upload(dx, std::vector<double>(px, px + n), st));
+  HIPCHK(upload(dz, std::vector<double>(pz, pz + n), st));
+  HIPCHK(di.alloc(n));
+  HIPCHK(ds.alloc(n));
+  HIPCHK(launch_nearest_check(ctx->rgeom.p, region, n, dx.p, dz.p, di.p, ds.p, st));
+  HIPCHK(hipMemcpyAsync(idx, di.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(scan, ds.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return FMT2D_OK;
+}
+
+int fmt2d_ray_nearest_calls(fmt2d_ctx* ctx, long long* n) {
+  if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
+  *n = ctx->ray_nearest_calls;
   return FMT2D_OK;
 }
 

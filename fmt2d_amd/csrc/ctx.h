@@ -143,6 +143,10 @@ struct fmt2d_ctx {
   RayGeomDev hgeom{};
   DBuf<RayGeomDev> rgeom;
   DBuf<int> rorder[NREG + 1];
+  DBuf<int> rjlo[NREG + 1], rjhi[NREG + 1];        // nearest-node search: rows per column
+  DBuf<double> rxlo[NREG + 1], rxhi[NREG + 1];     // and x bounds of the columns from / up to c
+  int ray_scan = 0;                                // 1: nearest-node search by the full scan
+  long long ray_nearest_calls = 0;                 // nearest-node searches of the last trace
   DBuf<double> risx[NIF + 1], risz[NIF + 1], rifx_in[NIF + 1];
   DBuf<double> rsfx, rsfz, rsfz0, rpvel, rvx, rvz;
   DBuf<int> rinv;

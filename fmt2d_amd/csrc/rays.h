@@ -21,6 +21,15 @@ struct RayRegionDev {
   const int* order;        // reference node k (0-based) -> field slot
   const uint16_t* flags;   // (nz, nx)
   const double* vel;       // (nz+2, nx) region node velocities
+  // column index of the nearest-node search (nearest_node): rows of the
+  // region's grid nodes per column (jlo > jhi: none), and bounds on the x of
+  // every node (grid and interface) of the columns from c on (xlo_from) and
+  // up to c (xhi_upto); indexed = 0 when some column's rows are not one run
+  int indexed;
+  const int* jlo;
+  const int* jhi;
+  const double* xlo_from;
+  const double* xhi_upto;
 };
 
 struct RayGeomDev {
@@ -98,5 +107,13 @@ hipError_t launch_frechet_contrib(const RayGeomDev* G, const RayField* fields, c
                                   int ray0, const long long* pt_off, const double2* pts, const RayOut* out,
                                   const long long* coff, long long* count, unsigned long long* keys, double* vals,
                                   hipStream_t st);
+
+// nearest node of region `region` for n points, by the column index (idx;
+// -2 when the region has none) and by the scan (scan)
+hipError_t launch_nearest_check(const RayGeomDev* G, int region, int n, const double* px, const double* pz, int* idx,
+                                int* scan, hipStream_t st);
+// count of nearest-node searches (fallbacks of the interpolators)
+hipError_t nearest_calls_reset(hipStream_t st);
+hipError_t nearest_calls_read(unsigned long long* n, hipStream_t st);
 
 }  // namespace fmt2d

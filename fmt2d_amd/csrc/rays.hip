@@ -55,7 +55,8 @@ __device__ inline void slot_xz(const RayGeomDev& G, const RayRegionDev& R, int q
 }
 
 // nearest region node over the whole node list, first minimum in node order
-__device__ int nearest_any(const RayGeomDev& G, const RayRegionDev& R, double px, double pz, double dmin0) {
+// (the reference's scan, interpolate_arrivaltime_reg / find_position_in_region2d)
+__device__ int nearest_scan(const RayGeomDev& G, const RayRegionDev& R, double px, double pz, double dmin0) {
   double dmin = dmin0;
   int best = -1;
   for (int k = 0; k < R.nnode; k++) {
@@ -66,6 +67,63 @@ __device__ int nearest_any(const RayGeomDev& G, const RayRegionDev& R, double px
     if (d < dmin) { dmin = d; best = q; }
   }
   return best;
+}
+
+// The same node from the column index: every node of the region lies on a
+// grid column (grid nodes at xg, the column's top and bottom interface nodes
+// at their own x), so columns are visited outward from the point's and
+// skipped once the x bound of the remaining columns alone exceeds the best
+// distance.  In a column the candidates are the region's grid rows next to
+// pz (clamped to the column's run of rows) and the two interface nodes.
+// Distances are formed as in the scan and ties go to the node that comes
+// first in node order (grid nodes ix-outer / iz-inner, then the top
+// interface's nodes, then the bottom's), so the result is the scan's.
+struct NearBest {
+  double d;
+  long long rank;
+  int q;
+};
+
+__device__ inline void near_offer(NearBest& b, double px, double pz, double x, double z, long long rank, int q) {
+  const double d = (px - x) * (px - x) + (pz - z) * (pz - z);
+  if (d < b.d || (b.q >= 0 && d == b.d && rank < b.rank)) { b.d = d; b.rank = rank; b.q = q; }
+}
+
+__device__ inline void near_column(const RayGeomDev& G, const RayRegionDev& R, int c, double px, double pz,
+                                   NearBest& b) {
+  const long long ng = (long long)G.nx * G.nz;
+  const int lo = R.jlo[c], hi = R.jhi[c];
+  if (lo <= hi) {
+    // rows around pz, clamped to the run (a row offered twice changes nothing)
+    const int jr = ifloor((pz - G.z0) / G.dz);
+    for (int j = jr - 1; j <= jr + 2; j++) {
+      const int jj = j < lo ? lo : (j > hi ? hi : j);
+      near_offer(b, px, pz, G.xg[c], G.zg[jj], (long long)c * G.nz + jj, jj * G.nx + c);
+    }
+  }
+  near_offer(b, px, pz, G.isx[R.itop][c], G.isz[R.itop][c], ng + c, G.nz * G.nx + c);
+  near_offer(b, px, pz, G.isx[R.ibot][c], G.isz[R.ibot][c], ng + G.nx + c, (G.nz + 1) * G.nx + c);
+}
+
+__device__ unsigned long long g_nearest_calls;  // nearest-node searches (diagnostics)
+
+__device__ int nearest_any(const RayGeomDev& G, const RayRegionDev& R, double px, double pz, double dmin0) {
+  atomicAdd(&g_nearest_calls, 1ull);
+  if (!R.indexed) return nearest_scan(G, R, px, pz, dmin0);
+  NearBest b{dmin0, 0, -1};
+  int c0 = ifloor((px - G.x0) / G.dx + 0.5);
+  c0 = c0 < 0 ? 0 : (c0 >= G.nx ? G.nx - 1 : c0);
+  for (int c = c0; c < G.nx; c++) {
+    const double xl = R.xlo_from[c];
+    if (xl > px && (xl - px) * (xl - px) > b.d) break;
+    near_column(G, R, c, px, pz, b);
+  }
+  for (int c = c0 - 1; c >= 0; c--) {
+    const double xh = R.xhi_upto[c];
+    if (xh < px && (px - xh) * (px - xh) > b.d) break;
+    near_column(G, R, c, px, pz, b);
+  }
+  return b.q;
 }
 
 // find_position_in_region2d (rays2d.f90:1076-1131)
@@ -857,6 +915,36 @@ __global__ void adjoint_sources_kernel(const RayGeomDev* __restrict__ Gp, const 
   }
   const int m = nearest_any(G, R, px, pz, RAY_HUGE);
   if (m >= 0) atomicAdd(L + m, gr);
+}
+
+// Nearest region node of each point by the column index and by the scan
+// (diagnostics: the two must agree).
+__global__ void nearest_check_kernel(const RayGeomDev* __restrict__ Gp, int region, int n, const double* __restrict__ px,
+                                     const double* __restrict__ pz, int* idx, int* scan) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const RayGeomDev& G = *Gp;
+  RayRegionDev R = G.reg[region];
+  scan[k] = nearest_scan(G, R, px[k], pz[k], RAY_HUGE);
+  idx[k] = R.indexed ? nearest_any(G, R, px[k], pz[k], RAY_HUGE) : -2;
+}
+
+hipError_t launch_nearest_check(const RayGeomDev* G, int region, int n, const double* px, const double* pz, int* idx,
+                                int* scan, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(nearest_check_kernel, dim3((n + 63) / 64), dim3(64), 0, st, G, region, n, px, pz, idx, scan);
+  return hipGetLastError();
+}
+
+hipError_t nearest_calls_reset(hipStream_t st) {
+  const unsigned long long z = 0;
+  return hipMemcpyToSymbolAsync(HIP_SYMBOL(g_nearest_calls), &z, sizeof(z), 0, hipMemcpyHostToDevice, st);
+}
+
+hipError_t nearest_calls_read(unsigned long long* n, hipStream_t st) {
+  hipError_t e = hipMemcpyFromSymbolAsync(n, HIP_SYMBOL(g_nearest_calls), sizeof(*n), 0, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
 }
 
 hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, const RayOut* out,

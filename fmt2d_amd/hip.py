@@ -133,6 +133,9 @@ def load() -> C.CDLL:
     lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_lsqr_mode.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_set_ray_search.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_ray_nearest_calls.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    lib.fmt2d_ray_nearest_check.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _ip]
     lib.fmt2d_autotune.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.POINTER(TuneResult)]
     lib.fmt2d_mesh_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.fmt2d_trace.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _ip]
@@ -163,7 +166,8 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
            "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_tile",
-           "fmt2d_autotune", "fmt2d_set_lsqr_mode",
+           "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_set_ray_search", "fmt2d_ray_nearest_calls",
+           "fmt2d_ray_nearest_check",
            "fmt2d_tf_region",
            "fmt2d_get_field",
            "fmt2d_set_inversion", "fmt2d_set_inversion_mesh", "fmt2d_set_frechet_chunk", "fmt2d_inversion_info", "fmt2d_mesh_info",
@@ -323,7 +327,29 @@ class Context:
         _check(self._lib.fmt2d_trace(self._h, n, src.ctypes.data_as(_ip), path.ctypes.data_as(_ip), _p(rx), _p(rz),
                                      _p(t), v.ctypes.data_as(_ip)))
         self._nray = n
+        self._last_trace_args = (src, path, rx, rz)
         return t, v.astype(bool)
+
+    def set_ray_search(self, mode: int = 0):
+        """0: indexed nearest-node search (default); 1: the full scan."""
+        _check(self._lib.fmt2d_set_ray_search(self._h, int(mode)))
+
+    def ray_nearest_calls(self) -> int:
+        n = C.c_longlong()
+        _check(self._lib.fmt2d_ray_nearest_calls(self._h, C.byref(n)))
+        return n.value
+
+    def ray_nearest_check(self, region: int, px, pz):
+        """(column-index node, scan node) per point of region `region`."""
+        x = np.ascontiguousarray(px, dtype=np.float64)
+        z = np.ascontiguousarray(pz, dtype=np.float64)
+        if len(x) != len(z):
+            raise ValueError("px and pz must have the same length")
+        a = np.zeros(len(x), dtype=np.int32)
+        b = np.zeros(len(x), dtype=np.int32)
+        _check(self._lib.fmt2d_ray_nearest_check(self._h, int(region), len(x), _p(x), _p(z), a.ctypes.data_as(_ip),
+                                                 b.ctypes.data_as(_ip)))
+        return a, b
 
     def ray_info(self, iray: int):
         e = np.zeros(2)

@@ -214,6 +214,20 @@ int fmt2d_mesh_info(fmt2d_ctx* ctx, int* vnx, int* vnz, int* n_frechet);
  * 5 Pg_ss2).  t_rcv: receiver time including the water leg; valid: 1/0. */
 int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const double* rx, const double* rz,
                 double* t_rcv, int* valid);
+/* Nearest-node search of the ray interpolators' fallbacks
+ * (interpolate_arrivaltime_reg, find_position_in_region2d,
+ * rays2d.f90:960-976, 1076-1131): 0 (default) a column index of each
+ * region, visited outward from the point; 1 the reference's scan over the
+ * region's whole node list.  Both return the same node (first minimum in
+ * node order).  fmt2d_ray_nearest_calls: searches made by the last trace.
+ * (Diagnostics of this build.) */
+int fmt2d_set_ray_search(fmt2d_ctx* ctx, int mode);
+int fmt2d_ray_nearest_calls(fmt2d_ctx* ctx, long long* n);
+/* The nearest node (field slot) of region `region` (1..6) for n points by
+ * both searches: idx (column index; -2 when the region has none) and scan
+ * (diagnostics: they must agree). */
+int fmt2d_ray_nearest_check(fmt2d_ctx* ctx, int region, int n, const double* px, const double* pz, int* idx,
+                            int* scan);
 /* Ray r of the last trace: entry point (x, z) the ray starts from, water-leg
  * time, points per section (4), flags[3] (headwave, diffracted, direct water
  * wave: a Pg of FMT2D_FWD_PROGRAM taken as the straight water path, no

@@ -178,3 +178,49 @@ def test_fine_grid_frechet_rows(traced_fine):
         nrow += 1
     print(f"{nrow} rows, {nent} entries, bit-identical {nbit}, max relative difference {rmax:.1e}")
     assert nrow > 10 and nbit > 0.95 * nent
+
+
+def test_fine_grid_indexed_search_is_the_scan(traced_fine):
+    """The interpolators' nearest-node fallback (interpolate_arrivaltime_reg,
+    find_position_in_region2d) from the column index gives the reference
+    scan's node: the same validity, times, entry points, point lists and
+    Frechet rows for every ray of the 0.1 km fixture, bit for bit."""
+    ctx, Z, t, v, rp, col, val = traced_fine
+    n_idx = ctx.ray_nearest_calls()
+    src = np.asarray(Z["src"]) if "src" in Z.files else None
+    try:
+        ctx.set_ray_search(1)
+        geom_rays = ctx._last_trace_args
+        t2, v2 = ctx.trace(*geom_rays)
+        n_scan = ctx.ray_nearest_calls()
+        rp2, col2, val2 = ctx.frechet_csr(index_base=1)
+        assert np.array_equal(v, v2)
+        assert np.array_equal(t.view(np.int64), t2.view(np.int64))
+        assert np.array_equal(rp, rp2) and np.array_equal(col, col2)
+        assert np.array_equal(val.view(np.int64), val2.view(np.int64))
+    finally:
+        ctx.set_ray_search(0)
+    print(f"nearest-node searches per trace: {n_idx} (indexed), {n_scan} (scan)")
+    assert n_idx == n_scan
+
+
+def test_indexed_nearest_node_on_random_points(traced_fine):
+    """fmt2d_ray_nearest_check: for 20 000 seeded points over and around the
+    0.1 km model, in every region, the column index returns the scan's node
+    (the first minimum in node order), and points on grid nodes and
+    half-way between them (distance ties) are in the set."""
+    ctx = traced_fine[0]
+    info = ctx.info
+    rng = np.random.default_rng(11)
+    x0, x1 = info.x0 - 1.0, info.x0 + info.dx * (info.nx - 1) + 1.0
+    z0, z1 = info.z0 - 1.0, info.z0 + info.dz * (info.nz - 1) + 1.0
+    px = np.concatenate([rng.uniform(x0, x1, 16000), info.x0 + info.dx * rng.integers(0, info.nx, 4000) / 2.0])
+    pz = np.concatenate([rng.uniform(z0, z1, 16000), info.z0 + info.dz * rng.integers(0, info.nz, 4000) / 2.0])
+    checked = 0
+    for region in range(1, 7):
+        a, b = ctx.ray_nearest_check(region, px, pz)
+        if np.all(a == -2):
+            continue
+        assert np.array_equal(a, b), f"region {region}: {int(np.sum(a != b))} points differ"
+        checked += 1
+    assert checked >= 5
