@@ -35,6 +35,11 @@ OBS, Pg and PmP (picks from a +-5 % checkerboard of 10 km x 3 km cells plus
 30 ms seeded noise, starting model without it), with the ray-based
 backward (Frechet rows + G^T r) timed beside it.
 
+Sixth figure (``fp32_storage``): the multistage forward with the tile
+solver's grid-node fields stored in FP32 beside FP64 storage, with the
+largest |dT| per stage field against the FP64 fields and the fields that
+have nodes beyond 1e-4 s named.
+
 Fifth figure (``joint``, ``--joint N``, default 4, 0 skips): seconds per iteration of a
 joint velocity + interface inversion on an inversion mesh coarser than the
 propagation grid: 2001x301, 200 OBS every 1 km, shots every 0.2 km, five
@@ -532,6 +537,55 @@ def joint_figure(iters: int, cpu: bool = False):
             "cpu_baseline": cpu_b, "log": log}
 
 
+def fp32_figure(steps: int = 2):
+    """The multistage forward (2001x301, 60 OBS) with the tile solver's
+    grid-node fields stored in FP32 (fmt2d_set_field_precision 32: T as an
+    FP32 offset from an FP64 base per tile, gradients FP32, compute FP64)
+    beside FP64 storage: time, bytes per node of the solver's field storage
+    and, per stage field, the largest |dT| against the FP64 fields; the
+    fields with nodes beyond 1e-4 s are named."""
+    from fmt2d_amd import hip, formats
+    m, g, desc = workload("multistage")
+    with tempfile.TemporaryDirectory() as d:
+        write_case(d, m, g)
+        sm = formats.read_smesh(os.path.join(d, "model.smesh"))
+        sx, sz = formats.read_refl(os.path.join(d, "sed.dat"))
+        mx, mz = formats.read_refl(os.path.join(d, "moho.dat"))
+    ctx = hip.Context(0)
+    ctx.set_model(sm, sx, sz, mx, mz)
+    obs = list(g.obs_x)
+    out = {}
+    for bits in (64, 32):
+        ctx.set_field_precision(bits)
+        ctx.forward(obs)
+        walls, fims = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ctx.forward(obs)
+            walls.append(time.perf_counter() - t0)
+            fims.append(ctx.stats().ms_fim)
+        out[bits] = {"ms_per_step": round(1e3 * float(np.mean(walls)), 1),
+                     "fim_kernel_ms_per_step": round(float(np.mean(fims)), 1),
+                     "tile_visits": int(ctx.stats().tiles)}
+        if bits == 64:
+            ctx.field_snapshot()
+    mdt, nd, nover = ctx.field_compare(1e-4)
+    summ = ctx.summary()
+    ctx.close()
+    per = mdt.reshape(len(obs), 12)
+    named = [{"obs": int(k // 12), "tf": int(k % 12 + 1), "nodes_over_1e-4": int(nover[k]),
+              "max_abs_dt_s": float(mdt[k])} for k in np.flatnonzero(nover > 0)]
+    return {"workload": desc, "fp64": out[64], "fp32": out[32],
+            "solver_field_bytes_per_node": {"fp64": 25, "fp32": 13,
+                                            "note": "T, dT/dx, dT/dz and the tie byte the tile solver reads and "
+                                                    "writes per grid node and visit (model: velocity 8 B and "
+                                                    "flags 2 B read in both)"},
+            "max_abs_dt_per_stage_field_s": [float(x) for x in per.max(axis=0)],
+            "fields": int(len(mdt)), "fields_with_nodes_over_1e-4": len(named),
+            "nodes_over_1e-4": int(np.sum(nover)), "reached_slots": int(np.sum(summ["reached"])),
+            "reach_mismatches": int(np.sum(nd)), "max_abs_dt_s": float(mdt.max()), "named_fields": named[:40]}
+
+
 def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int, autotune: bool = False):
     """Source-stages per second of one workload's forward (the headline's
     measurement; see the module docstring), with the tile-kernel time from
@@ -589,6 +643,7 @@ def main():
     ap.add_argument("--no-iteration", action="store_true", help="skip the inversion-iteration figure")
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 field-storage figure")
     ap.add_argument("--joint", type=int, default=4,
                     help="iterations of the joint inversion on the 0.5 km / 1 km mesh (2001x301, 200 OBS, five "
                          "phases; about a minute for 4); 0 skips it")
@@ -633,6 +688,7 @@ def main():
               "solver_launches_per_step": int(np.mean(f["launches"])), "tiles_per_step": int(np.mean(f["tiles"])),
               "fim_kernel_ms_per_step": round(1e3 * float(np.mean(f["fims"])), 3)}
         print(f"multistage figure done: {ms['value']}", file=sys.stderr, flush=True)
+    f32 = None if args.no_fp32 else fp32_figure()
     it = None if args.no_iteration else inversion_iteration(max(1, min(args.steps, 3)), 1, args.cpu_sample > 0)
     progs = None if args.no_programs else programs_figure(args.cpu_sample > 0)
     grad = None if args.no_gradient else gradient_figure(max(1, min(args.steps, 2)))
@@ -677,6 +733,7 @@ def main():
                                           f"{alg_bytes / ndisp:.0f} B per dispatch"},
         "cpu_baseline": head["cpu"],
         "multistage": ms,
+        "fp32_storage": f32,
         "iteration": it,
         "programs": progs,
         "gradient": grad,

@@ -607,6 +607,7 @@ int fmt2d_adjoint_gradient(fmt2d_ctx* ctx, const double* dJdt, double* grad_vel,
   if (!ctx || !dJdt) return fail(FMT2D_EINVAL, "NULL argument");
   if (ctx->nray < 1) return fail(FMT2D_ESTATE, "no rays traced");
   if (ctx->fwd_program) return fail(FMT2D_ESTATE, "the adjoint needs a forward without FMT2D_FWD_PROGRAM");
+  if (ctx->fwd_f32) return fail(FMT2D_ESTATE, "the adjoint needs a forward with FP64 field storage");
   HIPCHK(hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = ctx->stream;

@@ -254,6 +254,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   const bool program = (flags & FMT2D_FWD_PROGRAM) != 0;
   if (program) flags |= FMT2D_FWD_NO_HEADWAVE;
   ctx->fwd_program = program;
+  ctx->fwd_f32 = ctx->field_f32;
   HostModel& m = ctx->m;
   hipStream_t st = ctx->stream;
   const int nx = m.nx, nz = m.nz;
@@ -356,9 +357,9 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   ctx->stats.nsrc = nsrc;
   ctx->stats.nfield = nfield;
   ctx->stats.region_nodes = region_nodes;
-  HIPCHK(upload(ctx->descs, fd, st));
-  HIPCHK(upload(ctx->links, lk, st));
-  // ---- tile bookkeeping sized for the largest level
+  // ---- tile bookkeeping sized for the largest level (32-bit storage: 8x8 tiles)
+  const bool f32 = ctx->field_f32 != 0;
+  if (f32 && (ctx->tile_x != 8 || ctx->tile_z != 8)) { ctx->tile_x = 8; ctx->tile_z = 8; }
   const int tx = ctx->tile_x, tz = ctx->tile_z;
   if (ctx->heavy_tx != tx || ctx->heavy_tz != tz) {
     const int rc = build_heavy_tiles(ctx);
@@ -367,6 +368,24 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   const int ntx = (nx + tx - 1) / tx, ntz = (nz + tz - 1) / tz, ntiles = ntx * ntz;
   int maxlev = 0;
   for (int L = 0; L < 4; L++) maxlev = std::max(maxlev, level_off[L + 1] - level_off[L]);
+  if (f32) {
+    // 32-bit working storage for one level's fields, reused level by level
+    const size_t ng = (size_t)nx * nz;
+    if (ctx->f32T.n < (size_t)maxlev * ng) {
+      HIPCHK(ctx->f32T.alloc((size_t)maxlev * ng));
+      HIPCHK(ctx->f32GX.alloc((size_t)maxlev * ng));
+      HIPCHK(ctx->f32GZ.alloc((size_t)maxlev * ng));
+    }
+    for (int L = 0; L < 4; L++)
+      for (int f = level_off[L]; f < level_off[L + 1]; f++) {
+        const size_t pos = (size_t)(f - level_off[L]);
+        fd[f].T32 = ctx->f32T.p + pos * ng;
+        fd[f].GX32 = ctx->f32GX.p + pos * ng;
+        fd[f].GZ32 = ctx->f32GZ.p + pos * ng;
+      }
+  }
+  HIPCHK(upload(ctx->descs, fd, st));
+  HIPCHK(upload(ctx->links, lk, st));
   const size_t ntt = (size_t)maxlev * ntiles;
   if (ctx->tin.n < ntt) {
     HIPCHK(ctx->tin.alloc(ntt));
@@ -398,6 +417,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
     // [1] / [2] sizes of active lists 0 / 1
     HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 4, st));
+    if (f32) HIPCHK(launch_f32_reset(dl, nf, nx * nz, st));
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, tx, tz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
                              ctx->tcount.p + 1, st));
     if (ctx->ttrace.n < (size_t)max_launch + 1) HIPCHK(ctx->ttrace.alloc((size_t)max_launch + 1));
@@ -427,7 +447,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
                           ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->max_sweeps_heavy,
                           ctx->colors,
-                          ctx->acc.p, ctx->tmax.p + launches, (int)ntt, st));
+                          ctx->acc.p, ctx->tmax.p + launches, (int)ntt, f32 ? 1 : 0, ctx->stop_flag.p, st));
         HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
         cur = nxt;
       }
@@ -468,6 +488,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         }
       ctx->stats.launches += used;
     }
+    // 32-bit storage: the solved level in FP64 for the post-passes and the rays
+    if (f32) HIPCHK(launch_f32_export(dl, nf, nx * nz, st));
     // heap-exhaustion stop of the reference (src/propagate2d.f90:74-80)
     {
       HIPCHK(launch_member_time(dl, ctx->regs.p, nf, nx, nz, st));
@@ -531,7 +553,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   ctx->stats.ms_fim = fim_ms;
   int stop = 0;
   HIPCHK(hipMemcpy(&stop, ctx->stop_flag.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (stop) return fail(FMT2D_EINVAL, "negative determinant too large in regular_update2d (the reference stops here)");
+  if (stop & 1) return fail(FMT2D_EINVAL, "negative determinant too large in regular_update2d (the reference stops here)");
+  if (stop & 2) return fail(FMT2D_EINVAL, "a traveltime beyond 255 s does not fit the 32-bit field storage");
   ctx->stats.dispatches = (int)nev;
   unsigned long long acc[3 + 64];
   HIPCHK(hipMemcpy(acc, ctx->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
@@ -548,6 +571,59 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
   ctx->colors = colors;
   ctx->window_tiles = window_tiles;
   ctx->max_sweeps = max_sweeps;
+  return FMT2D_OK;
+}
+
+int fmt2d_field_snapshot(fmt2d_ctx* ctx) {
+  if (!ctx) return fail(FMT2D_EINVAL, "NULL context");
+  if (ctx->nfield < 1) return fail(FMT2D_ESTATE, "no forward has run");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)(ctx->m.nz + 2) * ctx->m.nx * ctx->nfield;
+  if (ctx->snapT.n < n) HIPCHK(ctx->snapT.alloc(n));
+  HIPCHK(hipMemcpyAsync(ctx->snapT.p, ctx->fT.p, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->snap_nfield = ctx->nfield;
+  ctx->snap_src = ctx->src_x;
+  return FMT2D_OK;
+}
+
+int fmt2d_field_compare(fmt2d_ctx* ctx, double tol, double* max_dt, long long* n_reach_diff, long long* n_over,
+                        int cap, int* n) {
+  if (!ctx || !n) return fail(FMT2D_EINVAL, "NULL argument");
+  if (ctx->snap_nfield != ctx->nfield || ctx->snap_src != ctx->src_x)
+    return fail(FMT2D_ESTATE, "the snapshot is of other sources or no snapshot was taken");
+  *n = ctx->nfield;
+  if (!max_dt || !n_reach_diff || !n_over || cap <= 0) return FMT2D_OK;
+  if (cap < ctx->nfield) return fail(FMT2D_EINVAL, "output arrays too small");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nf = ctx->nfield;
+  DBuf<double> dm;
+  DBuf<long long> dn, dover;
+  HIPCHK(dm.alloc(nf));
+  HIPCHK(dn.alloc(nf));
+  HIPCHK(dover.alloc(nf));
+  const long long nslot = (long long)(ctx->m.nz + 2) * ctx->m.nx;
+  HIPCHK(launch_field_compare(ctx->descs.p, nf, ctx->snapT.p, nslot, tol, dm.p, dn.p, dover.p, ctx->stream));
+  std::vector<double> hm(nf);
+  std::vector<long long> hn(nf), ho(nf);
+  HIPCHK(hipMemcpyAsync(hm.data(), dm.p, sizeof(double) * nf, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(hn.data(), dn.p, sizeof(long long) * nf, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ho.data(), dover.p, sizeof(long long) * nf, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // source-major order: isrc * 12 + itf - 1
+  for (int k = 0; k < nf; k++) {
+    const int f = ctx->field_of[k];
+    max_dt[k] = hm[f];
+    n_reach_diff[k] = hn[f];
+    n_over[k] = ho[f];
+  }
+  return FMT2D_OK;
+}
+
+int fmt2d_set_field_precision(fmt2d_ctx* ctx, int bits) {
+  if (!ctx || (bits != 32 && bits != 64)) return fail(FMT2D_EINVAL, "field precision must be 32 or 64");
+  ctx->field_f32 = bits == 32 ? 1 : 0;
+  ctx->fields_current = false;
   return FMT2D_OK;
 }
 
@@ -605,6 +681,7 @@ int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int
   const int shapes[3][2] = {{8, 8}, {16, 8}, {8, 16}};
   for (auto& sh : shapes) {
     if (sh[0] == best.tx && sh[1] == best.tz) continue;
+    if (ctx->field_f32 && (sh[0] != 8 || sh[1] != 8)) continue;  // 32-bit storage solves 8x8 tiles
     Cfg c = best; c.tx = sh[0]; c.tz = sh[1];
     if (c.tx * c.tz > 64) { c.cap = std::max(c.cap, 16); c.hcap = std::max(c.hcap, 16); }
     cands.push_back(c);

@@ -121,6 +121,13 @@ struct fmt2d_ctx {
   int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   int tile_x = 8, tile_z = 8;         // tile shape of the solver (8x8, 16x8, 8x16)
+  int field_f32 = 0;                  // 1: the tile solver stores grid-node T and gradients in 32 bits
+  int fwd_f32 = 0;                    // the last forward did
+  DBuf<unsigned> f32T;                // its storage for one level's fields: T (fixed point)
+  DBuf<float> f32GX, f32GZ;           // and the gradients (FP32)
+  DBuf<double> snapT;                 // fmt2d_field_snapshot: T of every field of a forward
+  int snap_nfield = 0;
+  std::vector<double> snap_src;
   int heavy_tx = 0, heavy_tz = 0;     // shape the slow-tile flags (theavy) were built for
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once

@@ -27,7 +27,19 @@ struct FieldDesc {
   double* OGX;       // gradient after the refit / fix post-passes (output)
   double* OGZ;
   uint8_t* TIE;      // equal-time support mask of the grid nodes (TIE_* bits)
+  // 32-bit storage of the tile solver (T32 == nullptr: FP64 storage): the
+  // grid nodes' T in unsigned 32-bit fixed point (T32_UNIT s, one grid of values for
+  // every node, so the stored field does not depend on the order of the
+  // updates), the raw gradients as FP32; compute stays FP64 and the
+  // interface nodes stay FP64.  Copied to T / GX / GZ after the solve.
+  unsigned* T32;
+  float* GX32;
+  float* GZ32;
 };
+constexpr double T32_SCALE = 16777216.0;         // 2^24 units per second
+constexpr double T32_UNIT = 1.0 / T32_SCALE;     // 6.0e-8 s
+constexpr double T32_MAX = 255.0;                // times up to here are stored (the grid spans 256 s)
+constexpr unsigned T32_NONE = 0xFFFFFFFFu;       // unreached
 
 // Interface stage: where the start-interface values come from.
 struct StageLink {
@@ -72,7 +84,16 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
                       int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, int* launch_max, int due_cap, hipStream_t st);
+                      unsigned long long* acc, int* launch_max, int due_cap, int f32, int* flag,
+                      hipStream_t st);
+// per field: max |T - snap| over slots both reach, count of slots one
+// reaches, count of slots both reach with |T - snap| > tol
+hipError_t launch_field_compare(const FieldDesc* descs, int nfield, const double* snap, long long nslot, double tol,
+                                double* max_dt, long long* n_diff, long long* n_over, hipStream_t st);
+// 32-bit storage: reset the level's arrays (nslotg grid slots per field),
+// and copy them out to T / GX / GZ
+hipError_t launch_f32_reset(const FieldDesc* descs, int nfield, int nslotg, hipStream_t st);
+hipError_t launch_f32_export(const FieldDesc* descs, int nfield, int nslotg, hipStream_t st);
 hipError_t launch_member_time(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,
                               hipStream_t st);
 // device work arrays of the heap-exhaustion stop (heap_exit.h); bin arrays

@@ -291,7 +291,8 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
   const unsigned long long key = key_of(d.isT[c]);
   for (int i = max(0, c - 1); i <= min(nx - 1, c + 1); i++)
     for (int j = j0; j <= j1; j++)
-      if ((j % tz) == 0 || j == j0) add_tile(f, (j / tz) * ntx + i / tx, ntiles, key, in_set, tkey, list_next, count_next);
+      if ((j % tz) == 0 || j == j0)
+        add_tile(f, (j / tz) * ntx + i / tx, ntiles, key, in_set, tkey, list_next, count_next);
 }
 
 // Due tiles of a launch: count_due[0] tiles cut by an interface (placed
@@ -380,12 +381,27 @@ __device__ inline unsigned tile_iface_cols(int a0, int a1) {
   return m | (m << TX);
 }
 
+// 32-bit storage of T: the fixed-point grid.  A reached time beyond its
+// range sets bit 2 of *flag (the forward then fails) and is kept as none.
+__device__ inline unsigned t32_encode(double t) {
+  if (!(t <= T32_MAX)) return T32_NONE;
+  return (unsigned)llrint(t * T32_SCALE);
+}
+__device__ inline double t32_round(double t, int* flag) {
+  if (!(t <= T32_MAX)) {
+    if (t < FMT_HUGE_HALF && flag) atomicOr(flag, 2);
+    return t;
+  }
+  return (double)llrint(t * T32_SCALE) * T32_UNIT;
+}
+
 // One visit of tile ft: stage the window, sweep to the tile's local fixed
 // point, write back, activate the neighbours whose halo changed.
-template <int TX, int TZ>
+template <int TX, int TZ, bool F32>
 __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
                          int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
-                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, TileLds<TX, TZ>& L) {
+                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, int* flag,
+                         TileLds<TX, TZ>& L) {
   using S = Shape<TX, TZ>;
   constexpr int NPL = S::NPL, WX = S::WX, WZ = S::WZ, W2 = S::W2;
   const int f = ft.x, t = ft.y;
@@ -400,7 +416,16 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     const int gi = wx0 + k % WX, gj = wz0 + k / WX;
     if (gi >= 0 && gi < nx && gj >= 0 && gj < nz) {
       const int q = gj * nx + gi;
-      L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
+      if constexpr (F32) {
+        if (q == d.fixed_key) { L.T[k] = d.t0; L.GX[k] = 0.0; L.GZ[k] = 0.0; }
+        else {
+          const unsigned tv = d.T32[q];
+          L.T[k] = tv == T32_NONE ? FMT_HUGE_TIME : (double)tv * T32_UNIT;
+          L.GX[k] = (double)d.GX32[q]; L.GZ[k] = (double)d.GZ32[q];
+        }
+      } else {
+        L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
+      }
       L.vel[k] = R.vel[q]; L.flags[k] = R.flags[q]; L.tie[k] = d.TIE[q];
     } else {
       L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0; L.tie[k] = 0;
@@ -544,6 +569,12 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
             b.T = t0; b.gx = gx0; b.gz = gz0; b.susp = 0; b.reg_set = 0; b.tie = 0; b.indep = 1;
             update_grid_node<double>(v, ci, cj, b);
           }
+          if constexpr (F32) {
+            // the value as stored: T on the fixed-point grid, FP32 gradient
+            b.T = t32_round(b.T, flag);
+            b.gx = (double)(float)b.gx;
+            b.gz = (double)(float)b.gz;
+          }
           const bool u = __double_as_longlong(b.T) != __double_as_longlong(L.T[cw]) ||
                          __double_as_longlong(b.gx) != __double_as_longlong(L.GX[cw]) ||
                          __double_as_longlong(b.gz) != __double_as_longlong(L.GZ[cw]) || b.tie != (int)L.tie[cw];
@@ -612,7 +643,13 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
   for (int k = 0; k < NPL; k++) {
     if (!g_dirty[k]) continue;
     const int q = gj[k] * nx + gi[k];
-    d.T[q] = L.T[wg[k]]; d.GX[q] = L.GX[wg[k]]; d.GZ[q] = L.GZ[wg[k]]; d.TIE[q] = L.tie[wg[k]];
+    if constexpr (F32) {
+      d.T32[q] = t32_encode(L.T[wg[k]]);
+      d.GX32[q] = (float)L.GX[wg[k]]; d.GZ32[q] = (float)L.GZ[wg[k]];
+    } else {
+      d.T[q] = L.T[wg[k]]; d.GX[q] = L.GX[wg[k]]; d.GZ[q] = L.GZ[wg[k]];
+    }
+    d.TIE[q] = L.tie[wg[k]];
     mask |= edge(gi[k] - tx0, gj[k] - tz0);
     kmin = fmin(kmin, L.T[wg[k]]);
   }
@@ -641,22 +678,22 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
 
 // Persistent-style grid: blocks take the due tiles of this launch in turn
 // (the due count is known only on the device).
-template <int TX, int TZ>
+template <int TX, int TZ, bool F32>
 __global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
                                                        const RegionDev* __restrict__ regs, const int2* __restrict__ list,
                                                        const int* __restrict__ count_due, int ntx, int ntz,
                                                        unsigned int* in_set, unsigned long long* tkey,
                                                        int2* list_next, int* count_next, int max_sweeps,
                                                        int max_sweeps_heavy, int colors, unsigned long long* acc,
-                                                       int* launch_max, int due_cap) {
+                                                       int* launch_max, int due_cap, int* flag) {
   __shared__ TileLds<TX, TZ> L;
   const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
     // a tile cut by an interface gets its own sweep cap: its visits are the
     // slow ones, and a capped visit continues in a later launch
-    fim_tile<TX, TZ>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next,
-                     k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, L);
+    fim_tile<TX, TZ, F32>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next,
+                     k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, flag, L);
     __syncthreads();
   }
 }
@@ -1173,22 +1210,115 @@ hipError_t launch_schedule(const int2* list, const int* count, int nfield, int n
   return hipGetLastError();
 }
 
+// Per field: the largest |T - S| over slots both reach and the slots only
+// one of them reaches (S: an earlier forward's T of the same fields).  One
+// block per field.
+__global__ void __launch_bounds__(256) field_compare_kernel(const FieldDesc* __restrict__ descs,
+                                                            const double* __restrict__ snap, long long nslot,
+                                                            double tol, double* max_dt, long long* n_diff,
+                                                            long long* n_over) {
+  const int f = blockIdx.x;
+  const double* T = descs[f].T;
+  const double* S = snap + (long long)f * nslot;
+  double m = 0.0;
+  long long nd = 0, no = 0;
+  for (long long q = threadIdx.x; q < nslot; q += blockDim.x) {
+    const bool a = T[q] < 0.5e20, b = S[q] < 0.5e20;
+    if (a != b) nd++;
+    else if (a) {
+      const double e = fabs(T[q] - S[q]);
+      m = fmax(m, e);
+      if (e > tol) no++;
+    }
+  }
+  __shared__ double sm[256];
+  __shared__ long long sn[256], so[256];
+  sm[threadIdx.x] = m;
+  sn[threadIdx.x] = nd;
+  so[threadIdx.x] = no;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+      sn[threadIdx.x] += sn[threadIdx.x + o];
+      so[threadIdx.x] += so[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { max_dt[f] = sm[0]; n_diff[f] = sn[0]; n_over[f] = so[0]; }
+}
+
+hipError_t launch_field_compare(const FieldDesc* descs, int nfield, const double* snap, long long nslot, double tol,
+                                double* max_dt, long long* n_diff, long long* n_over, hipStream_t st) {
+  if (nfield <= 0) return hipSuccess;
+  hipLaunchKernelGGL(field_compare_kernel, dim3(nfield), dim3(256), 0, st, descs, snap, nslot, tol, max_dt, n_diff,
+                     n_over);
+  return hipGetLastError();
+}
+
+// 32-bit storage of a level: every grid node unreached, the gradients 0
+__global__ void f32_reset_kernel(const FieldDesc* __restrict__ descs, int nfield, int nslotg) {
+  const long long total = (long long)nfield * nslotg;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslotg), q = (int)(k % nslotg);
+    const FieldDesc& d = descs[f];
+    d.T32[q] = T32_NONE; d.GX32[q] = 0.0f; d.GZ32[q] = 0.0f;
+  }
+}
+
+// The solved level in FP64 for everything after the solve (the on-node
+// source keeps t0 and its zero gradient from the seeding).
+__global__ void f32_export_kernel(const FieldDesc* __restrict__ descs, int nfield, int nslotg) {
+  const long long total = (long long)nfield * nslotg;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(k / nslotg), q = (int)(k % nslotg);
+    const FieldDesc& d = descs[f];
+    if (q == d.fixed_key) continue;
+    const unsigned tv = d.T32[q];
+    d.T[q] = tv == T32_NONE ? FMT_HUGE_TIME : (double)tv * T32_UNIT;
+    d.GX[q] = (double)d.GX32[q];
+    d.GZ[q] = (double)d.GZ32[q];
+  }
+}
+
+hipError_t launch_f32_reset(const FieldDesc* descs, int nfield, int nslotg, hipStream_t st) {
+  const long long total = (long long)nfield * nslotg;
+  const int blocks = (int)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+  hipLaunchKernelGGL(f32_reset_kernel, dim3(blocks), dim3(256), 0, st, descs, nfield, nslotg);
+  return hipGetLastError();
+}
+
+hipError_t launch_f32_export(const FieldDesc* descs, int nfield, int nslotg, hipStream_t st) {
+  const long long total = (long long)nfield * nslotg;
+  const int blocks = (int)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+  hipLaunchKernelGGL(f32_export_kernel, dim3(blocks), dim3(256), 0, st, descs, nfield, nslotg);
+  return hipGetLastError();
+}
+
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
                       int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, int* launch_max, int due_cap, hipStream_t st) {
-  if (tx == 8 && tz == 8)
-    hipLaunchKernelGGL((fim_tiles_kernel<8, 8>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+                      unsigned long long* acc, int* launch_max, int due_cap, int f32, int* flag,
+                      hipStream_t st) {
+  if (f32) {
+    if (tx != 8 || tz != 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, true>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx,
+                       ntz, in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
+                       due_cap, flag);
+  } else if (tx == 8 && tz == 8)
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
                        in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap);
+                       due_cap, flag);
   else if (tx == 16 && tz == 8)
-    hipLaunchKernelGGL((fim_tiles_kernel<16, 8>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+    hipLaunchKernelGGL((fim_tiles_kernel<16, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
                        in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap);
+                       due_cap, flag);
   else if (tx == 8 && tz == 16)
-    hipLaunchKernelGGL((fim_tiles_kernel<8, 16>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 16, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
                        in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap);
+                       due_cap, flag);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -133,6 +133,10 @@ def load() -> C.CDLL:
     lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_lsqr_mode.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_set_field_precision.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_field_snapshot.argtypes = [C.c_void_p]
+    lib.fmt2d_field_compare.argtypes = [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_longlong), C.c_int, _ip]
     lib.fmt2d_set_ray_search.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_ray_nearest_calls.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     lib.fmt2d_ray_nearest_check.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _ip]
@@ -166,7 +170,8 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
            "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_tile",
-           "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_set_ray_search", "fmt2d_ray_nearest_calls",
+           "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_set_field_precision", "fmt2d_field_snapshot",
+           "fmt2d_field_compare", "fmt2d_set_ray_search", "fmt2d_ray_nearest_calls",
            "fmt2d_ray_nearest_check",
            "fmt2d_tf_region",
            "fmt2d_get_field",
@@ -245,6 +250,27 @@ class Context:
         self._nray = 0   # a forward drops the rays of the last trace
         _check(self._lib.fmt2d_autotune(self._h, len(x), _p(x), int(flags), int(level), C.byref(r)))
         return r
+
+    def set_field_precision(self, bits: int = 64):
+        """Storage precision of the tile solver's grid-node fields (64 or 32)."""
+        _check(self._lib.fmt2d_set_field_precision(self._h, int(bits)))
+
+    def field_snapshot(self):
+        _check(self._lib.fmt2d_field_snapshot(self._h))
+
+    def field_compare(self, tol: float = 1e-4):
+        """(max |dT| per field, slots reached by only one, slots with
+        |dT| > tol) against the snapshot, fields in the order
+        isrc * 12 + itf - 1."""
+        n = C.c_int()
+        _check(self._lib.fmt2d_field_compare(self._h, float(tol), None, None, None, 0, C.byref(n)))
+        m = np.zeros(n.value)
+        c = np.zeros(n.value, dtype=np.int64)
+        o = np.zeros(n.value, dtype=np.int64)
+        _llp = C.POINTER(C.c_longlong)
+        _check(self._lib.fmt2d_field_compare(self._h, float(tol), _p(m), c.ctypes.data_as(_llp),
+                                             o.ctypes.data_as(_llp), n.value, C.byref(n)))
+        return m, c, o
 
     def set_lsqr_mode(self, mode: int = 0):
         """0: LSQR reductions in the reference's order (bit-exact); 1: tree /

@@ -118,6 +118,25 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
  * identical for any cap).  Default 8.  (Solver tuning of this build, no
  * counterpart in the reference's heap-ordered march.) */
 int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy);
+/* Storage precision of the tile solver's grid-node fields: 64 (default;
+ * every result bit-exact against the reference) or 32: T stored in 32-bit
+ * fixed point (units of 2^-22 s = 2.4e-7 s, times below 511 s; one grid of
+ * values for every node, so the result does not depend on the order of
+ * the updates), raw gradients in FP32, every update still computed in FP64
+ * and rounded to the stored values; 8x8 tiles.  Interface nodes, seeds and
+ * everything after the solve (post-passes, rays, Frechet rows) stay FP64;
+ * the adjoint gradient needs an FP64 forward.  Halves the solver's field
+ * traffic; fmt2d_field_snapshot / fmt2d_field_compare measure its effect
+ * per field.  No counterpart in the reference. */
+int fmt2d_set_field_precision(fmt2d_ctx* ctx, int bits);
+/* Keep a copy of T of every field of the last forward (device memory), and
+ * compare a later forward of the same sources with it: per field (order
+ * isrc * 12 + itf - 1) the largest |dT| over slots both reach, the count of
+ * slots only one reaches and the count of slots with |dT| > tol.  With an
+ * output NULL or cap 0 only *n.  (Diagnostics of this build.) */
+int fmt2d_field_snapshot(fmt2d_ctx* ctx);
+int fmt2d_field_compare(fmt2d_ctx* ctx, double tol, double* max_dt, long long* n_reach_diff, long long* n_over,
+                        int cap, int* n);
 /* Tile shape of the solver: 8x8 (default; one node per lane of a
  * wavefront), 16x8 or 8x16 (two nodes per lane, one of each checkerboard
  * colour).  Results are identical for every shape.  (Solver tuning of this
