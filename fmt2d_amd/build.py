@@ -8,6 +8,7 @@ bind that ABI through iso_c_binding (fmt2d_amd/fortran/fmt2d_hip_mod.f90).
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -45,7 +46,7 @@ def build(verbose: bool = False) -> str:
     cc = hipcc()
     objdir = os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES + HOST_SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         cmd = [cc] + COMMON + ["-I", os.path.join(ROOT, "include")]
@@ -54,10 +55,19 @@ def build(verbose: bool = False) -> str:
         cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
+        jobs.append((src, obj, cmd))
+
+    def compile_one(job):
+        src, obj, cmd = job
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"compile of {src} failed:\n{r.stderr}")
-        objs.append(obj)
+        return obj
+
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(len(jobs), 16, os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        objs = list(pool.map(compile_one, jobs))
     tmp = LIB + ".tmp"
     cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -28,6 +28,13 @@
 
 namespace fmt2d {
 
+// Load through a global-address-space pointer (a pointer read from memory is
+// generic to the compiler, which then orders it against LDS stores).
+template <typename T>
+__device__ inline T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // Tile shapes.  A wavefront solves one tile of TX x TZ grid nodes with a
 // 2-node halo staged in LDS.  8 x 8: one node per lane.  16 x 8 and 8 x 16:
 // two nodes per lane, one of each checkerboard colour (a horizontal pair),
@@ -411,39 +418,71 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
   const int ntiles = ntx * ntz;
   const int tx0 = (t % ntx) * TX, tz0 = (t / ntx) * TZ;
   const int wx0 = tx0 - HALO, wz0 = tz0 - HALO;
-  // ---- stage the window
-  for (int k = lane; k < W2; k += 64) {
+  // ---- stage the window.  Every load of the window and of the interface
+  // columns is issued into registers before the first LDS write: the field
+  // pointers are loaded from memory, so the compiler cannot tell them from
+  // LDS addresses and would otherwise wait for each load before the next
+  // LDS store (one memory round trip per value).
+  constexpr int NIT = (W2 + 63) / 64;
+  double sT[NIT], sGX[NIT], sGZ[NIT], sV[NIT];
+  uint16_t sF[NIT];
+  uint8_t sTie[NIT];
+  bool sOk[NIT];
+  // addresses are clamped into the grid so that every lane loads without a
+  // branch; values outside the grid are replaced afterwards
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int k = lane + 64 * it;
     const int gi = wx0 + k % WX, gj = wz0 + k / WX;
-    if (gi >= 0 && gi < nx && gj >= 0 && gj < nz) {
-      const int q = gj * nx + gi;
-      if constexpr (F32) {
-        if (q == d.fixed_key) { L.T[k] = d.t0; L.GX[k] = 0.0; L.GZ[k] = 0.0; }
-        else {
-          const unsigned tv = d.T32[q];
-          L.T[k] = tv == T32_NONE ? FMT_HUGE_TIME : (double)tv * T32_UNIT;
-          L.GX[k] = (double)d.GX32[q]; L.GZ[k] = (double)d.GZ32[q];
-        }
-      } else {
-        L.T[k] = d.T[q]; L.GX[k] = d.GX[q]; L.GZ[k] = d.GZ[q];
-      }
-      L.vel[k] = R.vel[q]; L.flags[k] = R.flags[q]; L.tie[k] = d.TIE[q];
+    const bool ok = k < W2 && gi >= 0 && gi < nx && gj >= 0 && gj < nz;
+    const int q = min(max(gj, 0), nz - 1) * nx + min(max(gi, 0), nx - 1);
+    if constexpr (F32) {
+      const unsigned tv = gload(d.T32 + q);
+      const double gx = (double)gload(d.GX32 + q), gz = (double)gload(d.GZ32 + q);
+      const bool fx = q == d.fixed_key;
+      sT[it] = fx ? d.t0 : (tv == T32_NONE ? FMT_HUGE_TIME : (double)tv * T32_UNIT);
+      sGX[it] = fx ? 0.0 : gx; sGZ[it] = fx ? 0.0 : gz;
     } else {
-      L.T[k] = FMT_HUGE_TIME; L.GX[k] = 0.0; L.GZ[k] = 0.0; L.vel[k] = 0.0; L.flags[k] = 0; L.tie[k] = 0;
+      sT[it] = gload(d.T + q); sGX[it] = gload(d.GX + q); sGZ[it] = gload(d.GZ + q);
+    }
+    sV[it] = gload(R.vel + q); sF[it] = gload(R.flags + q); sTie[it] = gload(d.TIE + q);
+    sOk[it] = ok;
+  }
+  double cx;
+  {
+    const int gi = min(max(wx0 + lane, 0), nx - 1), gj = min(max(wz0 + lane - WX, 0), nz - 1);
+    const double xv = gload(R.xg + gi), zv = gload(R.zg + gj);
+    cx = lane < WX ? ((wx0 + lane >= 0 && wx0 + lane < nx) ? xv : 0.0)
+                   : ((wz0 + lane - WX >= 0 && wz0 + lane - WX < nz) ? zv : 0.0);
+  }
+  double iT, iGX, iGZ, iV, ifx, ifz;
+  int ifcc;
+  uint8_t ifok;
+  {
+    const int s = lane / WX > 1 ? 1 : lane / WX, gr = wx0 + lane % WX;
+    const int gi = min(max(gr, 0), nx - 1);
+    const int q = (nz + s) * nx + gi;
+    iT = gload(d.T + q); iGX = gload(d.GX + q); iGZ = gload(d.GZ + q);
+    iV = gload(R.vel + q); ifx = gload(R.ifx[s] + gi); ifz = gload(R.ifz[s] + gi);
+    ifcc = gload(R.ifcc[s] + gi); ifok = gload(R.ifok[s] + gi);
+    if (!(lane < 2 * WX && gr >= 0 && gr < nx)) {
+      iT = FMT_HUGE_TIME; iGX = 0.0; iGZ = 0.0; iV = 0.0; ifx = 0.0; ifz = 0.0; ifcc = -1; ifok = 0;
     }
   }
-  if (lane < WX) { const int gi = wx0 + lane; L.xg[lane] = (gi >= 0 && gi < nx) ? R.xg[gi] : 0.0; }
-  else if (lane < WX + WZ) { const int gj = wz0 + lane - WX; L.zg[lane - WX] = (gj >= 0 && gj < nz) ? R.zg[gj] : 0.0; }
-  if (lane < 2 * WX) {
-    const int s = lane / WX, gi = wx0 + lane % WX;
-    if (gi >= 0 && gi < nx) {
-      const int q = (nz + s) * nx + gi;
-      L.iT[lane] = d.T[q]; L.iGX[lane] = d.GX[q]; L.iGZ[lane] = d.GZ[q];
-      L.ivel[lane] = R.vel[q]; L.ifx[lane] = R.ifx[s][gi]; L.ifz[lane] = R.ifz[s][gi];
-      L.ifcc[lane] = R.ifcc[s][gi]; L.ifok[lane] = R.ifok[s][gi];
-    } else {
-      L.iT[lane] = FMT_HUGE_TIME; L.iGX[lane] = 0.0; L.iGZ[lane] = 0.0; L.ivel[lane] = 0.0;
-      L.ifx[lane] = 0.0; L.ifz[lane] = 0.0; L.ifcc[lane] = -1; L.ifok[lane] = 0;
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int k = lane + 64 * it;
+    if (k < W2) {
+      const bool ok = sOk[it];
+      L.T[k] = ok ? sT[it] : FMT_HUGE_TIME; L.GX[k] = ok ? sGX[it] : 0.0; L.GZ[k] = ok ? sGZ[it] : 0.0;
+      L.vel[k] = ok ? sV[it] : 0.0; L.flags[k] = ok ? sF[it] : (uint16_t)0; L.tie[k] = ok ? sTie[it] : (uint8_t)0;
     }
+  }
+  if (lane < WX) L.xg[lane] = cx;
+  else if (lane < WX + WZ) L.zg[lane - WX] = cx;
+  if (lane < 2 * WX) {
+    L.iT[lane] = iT; L.iGX[lane] = iGX; L.iGZ[lane] = iGZ; L.ivel[lane] = iV;
+    L.ifx[lane] = ifx; L.ifz[lane] = ifz; L.ifcc[lane] = ifcc; L.ifok[lane] = ifok;
   }
   __syncthreads();
   LView<TX, TZ> v{&R, &d, &L, wx0, wz0};
@@ -679,7 +718,7 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
 // Persistent-style grid: blocks take the due tiles of this launch in turn
 // (the due count is known only on the device).
 template <int TX, int TZ, bool F32>
-__global__ void __launch_bounds__(64) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
+__global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
                                                        const RegionDev* __restrict__ regs, const int2* __restrict__ list,
                                                        const int* __restrict__ count_due, int ntx, int ntz,
                                                        unsigned int* in_set, unsigned long long* tkey,
@@ -719,7 +758,15 @@ __global__ void refit_gradients_kernel(const FieldDesc* __restrict__ descs, cons
         GView v{&R, &d};
         double t0, a, b;
         seed_grid(d, R, i, j, t0, a, b);
-        if (grid_suspect<double>(v, i, j, t0)) {
+        bool susp;
+        if (reg_fast_node(R.flags[q])) {
+          // four regular cells, no partner: the fast path's walk gives the flag
+          const RegStatic rs = reg_static(v, i, j);
+          susp = reg_suspect(rs, reg_dyn(v, rs, i, j), t0);
+        } else {
+          susp = grid_suspect<double>(v, i, j, t0) != 0;
+        }
+        if (susp) {
           double gx, gz;
           fit_gradient_grid<double>(v, i, j, gx, gz);
           ox = gx; oz = gz;

@@ -222,10 +222,9 @@ FMT_HD void reg_quadrant(RegBest& b, const RegStatic& s, const RegDyn& y, const 
   }
 }
 
-// The update of a fast-path node: the same (T, dT/dx, dT/dz, equal-time
-// mask) as update_grid_node for a node seeded with (T0, gx0, gz0).
-FMT_HD void reg_update(const RegStatic& s, const RegDyn& y, double T0, double gx0, double gz0, double& T,
-                       double& gx, double& gz, int& tie_out) {
+// The quadrant walk of a fast-path node seeded with time T0: the running
+// minimum and the candidate it keeps.
+FMT_HD RegBest reg_walk(const RegStatic& s, const RegDyn& y, double T0) {
   const RegTerm t0 = reg_term(s, y, 0), t1 = reg_term(s, y, 1), t2 = reg_term(s, y, 2), t3 = reg_term(s, y, 3);
   double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
   int e0 = 0, e1 = 0, e2 = 0, e3 = 0;
@@ -238,6 +237,22 @@ FMT_HD void reg_update(const RegStatic& s, const RegDyn& y, double T0, double gx
   if (s.cells & 2u) reg_quadrant(b, s, y, t0, 0, v0, c0, e0, t3, 3, v3, c3, e3);
   if (s.cells & 4u) reg_quadrant(b, s, y, t1, 1, v1, c1, e1, t2, 2, v2, c2, e2);
   if (s.cells & 8u) reg_quadrant(b, s, y, t1, 1, v1, c1, e1, t3, 3, v3, c3, e3);
+  return b;
+}
+
+// Suspect flag of a fast-path node (grid_suspect): its value comes from a
+// one-member regular update (x or z member alone, :784), which for a node
+// with regular cells only is the candidate the walk keeps.
+FMT_HD bool reg_suspect(const RegStatic& s, const RegDyn& y, double T0) {
+  const int g = reg_walk(s, y, T0).gsel;
+  return g == 1 || g == 2;
+}
+
+// The update of a fast-path node: the same (T, dT/dx, dT/dz, equal-time
+// mask) as update_grid_node for a node seeded with (T0, gx0, gz0).
+FMT_HD void reg_update(const RegStatic& s, const RegDyn& y, double T0, double gx0, double gz0, double& T,
+                       double& gx, double& gz, int& tie_out) {
+  const RegBest b = reg_walk(s, y, T0);
   T = b.T;
   tie_out = b.indep ? 0 : b.tie;
   if (b.gsel == 0) { gx = gx0; gz = gz0; return; }

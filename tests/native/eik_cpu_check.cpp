@@ -346,7 +346,17 @@ int cpu_point_source(void* hp, int r, double sx, double* outT, double* outGX, do
     for (int i = 0; i < nx; i++) {
       const size_t g = (size_t)j * nx + i;
       if (!(R.flags[g] & RF_IN) || (int)g == F.fixed_key) continue;
-      if (grid_suspect<RT>(v, i, j, F.T0[g])) { RT dx, dz; fit_gradient_grid<RT>(v, i, j, dx, dz); OX[g] = dx; OZ[g] = dz; }
+      const bool susp = grid_suspect<RT>(v, i, j, F.T0[g]) != 0;
+      if (g_fast_check && reg_fast_node(v.flags(i, j))) {
+        // the refit pass takes the flag of fast-path nodes from the fast path's walk
+        const RegStatic rs = reg_static(v, i, j);
+        g_fast_checked++;
+        if (reg_suspect(rs, reg_dyn(v, rs, i, j), F.T0[g]) != susp) {
+          if (g_fast_bad < 5) fprintf(stderr, "fast-path suspect flag differs at (%d,%d)\n", i, j);
+          g_fast_bad++;
+        }
+      }
+      if (susp) { RT dx, dz; fit_gradient_grid<RT>(v, i, j, dx, dz); OX[g] = dx; OZ[g] = dz; }
     }
   const HostRegion& HR = m.reg[r];
   for (int k = 0; k < HR.nnode; k++) {
