@@ -48,10 +48,13 @@ int build_ray_geom(fmt2d_ctx* ctx) {
     bool runs = true;
     const HostInterface& T = m.isec[R.itop];
     const HostInterface& B = m.isec[R.ibot];
+    // rows in storage order; per column: first and last region row, count
+    std::vector<int> clo(nx, -1), chi(nx, -1), ccnt(nx, 0);
+    for (int j = 0; j < nz; j++)
+      for (int c = 0; c < nx; c++)
+        if (R.flags[(size_t)j * nx + c] & RF_IN) { if (clo[c] < 0) clo[c] = j; chi[c] = j; ccnt[c]++; }
     for (int c = 0; c < nx; c++) {
-      int lo = -1, hi = -1, cnt = 0;
-      for (int j = 0; j < nz; j++)
-        if (R.flags[(size_t)j * nx + c] & RF_IN) { if (lo < 0) lo = j; hi = j; cnt++; }
+      const int lo = clo[c], hi = chi[c], cnt = ccnt[c];
       if (lo >= 0) {
         jlo[c] = lo; jhi[c] = hi;
         if (cnt != hi - lo + 1) runs = false;

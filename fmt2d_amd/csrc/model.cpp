@@ -7,7 +7,9 @@
 #include <math.h>
 
 #include <algorithm>
+#include <functional>
 #include <sstream>
+#include <thread>
 
 #include "eik_common.h"
 
@@ -210,30 +212,41 @@ std::string rebuild_geometry(HostModel& m) {
   // seafloor->Moho
   static const int RTOP[NREG + 1] = {0, 1, 2, 3, 4, 2, 2};
   static const int RBOT[NREG + 1] = {0, 2, 3, 4, 5, 5, 4};
-  m.node_region.assign((size_t)nx * nz, 0);
-  for (int r = 1; r <= NREG; r++) {
+  // Every per-node loop below runs row by row (i fastest), in the arrays'
+  // storage order: the nodes are independent of each other, so the order of
+  // the loops does not change any value.
+  // Regions are independent of each other: each one's membership, and later
+  // its flags, velocities and node order, is built on a host thread of its
+  // own.
+  auto per_region = [](const std::function<void(int)>& fn) {
+    std::vector<std::thread> th;
+    for (int r = 1; r <= NREG; r++) th.emplace_back(fn, r);
+    for (auto& t : th) t.join();
+  };
+  per_region([&](int r) {
     HostRegion& R = m.reg[r];
     R.id = r; R.itop = RTOP[r]; R.ibot = RBOT[r];
     R.flags.assign((size_t)nx * nz, 0);
     const HostInterface& t = m.isec[R.itop];
     const HostInterface& b = m.isec[R.ibot];
-    for (int i = 0; i < nx; i++) {
-      const double zt = iface_z_grid(m, t, m.xg[i]), zb = iface_z_grid(m, b, m.xg[i]);
-      for (int j = 0; j < nz; j++) {
-        const size_t g = (size_t)j * nx + i;
-        if (zt <= m.zg[j] && m.zg[j] <= zb) {
-          R.flags[g] |= RF_IN;
-          m.node_region[g] = r;
-        } else if (m.node_region[g] == 0) {
-          m.node_region[g] = -1;
-        }
-      }
-    }
+    std::vector<double> zt(nx), zb(nx);
+    for (int i = 0; i < nx; i++) { zt[i] = iface_z_grid(m, t, m.xg[i]); zb[i] = iface_z_grid(m, b, m.xg[i]); }
+    for (int j = 0; j < nz; j++)
+      for (int i = 0; i < nx; i++)
+        if (zt[i] <= m.zg[j] && m.zg[j] <= zb[i]) R.flags[(size_t)j * nx + i] |= RF_IN;
+  });
+  // the last region that holds a node names it (regions are written in
+  // order, :1576-1577); -1 for a node of none
+  m.node_region.assign((size_t)nx * nz, -1);
+  for (int r = 1; r <= NREG; r++) {
+    const std::vector<uint16_t>& F = m.reg[r].flags;
+    for (size_t g = 0; g < F.size(); g++)
+      if (F[g] & RF_IN) m.node_region[g] = r;
   }
   // fully regular nodes (:1932-1979)
   m.fully_regular.assign((size_t)nx * nz, 0);
-  for (int i = 0; i < nx; i++)
-    for (int j = 0; j < nz; j++) {
+  for (int j = 0; j < nz; j++)
+    for (int i = 0; i < nx; i++) {
       const size_t g = (size_t)j * nx + i;
       if (m.node_region[g] <= 0) continue;
       bool reg = true;
@@ -245,13 +258,13 @@ std::string rebuild_geometry(HostModel& m) {
       if (i == 0 || i == nx - 1 || j == 0 || j == nz - 1) reg = false;
       m.fully_regular[g] = reg;
     }
-  for (int r = 1; r <= NREG; r++) {
+  per_region([&](int r) {
     HostRegion& R = m.reg[r];
     const HostInterface& t = m.isec[R.itop];
     const HostInterface& b = m.isec[R.ibot];
     auto in = [&](int i, int j) { return (R.flags[(size_t)j * nx + i] & RF_IN) != 0; };
-    for (int i = 0; i < nx; i++)
-      for (int j = 0; j < nz; j++) {
+    for (int j = 0; j < nz; j++)
+      for (int i = 0; i < nx; i++) {
         const size_t g = (size_t)j * nx + i;
         if (!in(i, j)) continue;
         uint16_t f = R.flags[g];
@@ -281,8 +294,8 @@ std::string rebuild_geometry(HostModel& m) {
       R.vel[(size_t)nz * nx + i] = t.vel_bot[i];
       R.vel[(size_t)(nz + 1) * nx + i] = b.vel_top[i];
     }
-    for (int i = 0; i < nx; i++)
-      for (int j = 0; j < nz; j++) {
+    for (int j = 0; j < nz; j++)
+      for (int i = 0; i < nx; i++) {
         if (!in(i, j)) continue;
         double v = m.vgrid[(size_t)j * nx + i];
         if (j < nz - 1 && !in(i, j + 1)) {
@@ -297,16 +310,20 @@ std::string rebuild_geometry(HostModel& m) {
         }
         R.vel[(size_t)j * nx + i] = v;
       }
-    // reference node order: grid nodes ix-outer / iz-inner, then top, then bottom
+    // reference node order: grid nodes ix-outer / iz-inner, then top, then
+    // bottom; membership is gathered column by column from a transposed copy
+    std::vector<uint8_t> cm((size_t)nx * nz);
+    for (int j = 0; j < nz; j++)
+      for (int i = 0; i < nx; i++) cm[(size_t)i * nz + j] = in(i, j) ? 1 : 0;
     R.ref_order.clear();
     for (int i = 0; i < nx; i++)
       for (int j = 0; j < nz; j++)
-        if (in(i, j)) R.ref_order.push_back(j * nx + i);
+        if (cm[(size_t)i * nz + j]) R.ref_order.push_back(j * nx + i);
     R.ngnode = (int)R.ref_order.size();
     for (int c = 0; c < t.n; c++) R.ref_order.push_back(nz * nx + c);
     for (int c = 0; c < b.n; c++) R.ref_order.push_back((nz + 1) * nx + c);
     R.nnode = (int)R.ref_order.size();
-  }
+  });
   return "";
 }
 
