@@ -724,6 +724,8 @@ def main():
                    "tiles_per_step": int(np.mean(head["tiles"])), "fim_kernel_ms_per_step": round(1e3 * fim_s, 3),
                    "fim_kernel_dispatches_per_step": int(ndisp),
                    "fim_kernel_us_per_dispatch": round(1e6 * fim_s / ndisp, 2),
+                   "solver_launch": "8-launch batches captured once per dependency level in a HIP graph and "
+                                    "replayed; dispatch spans from the device clock",
                    "schedule": head["schedule"]},
         "roofline": {"bound": "hbm", "kernel": "fim_tiles_kernel", "achieved": round(achieved, 3),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 6),

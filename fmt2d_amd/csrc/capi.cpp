@@ -397,7 +397,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
   HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
   if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
-  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(4));
+  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(5));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
   if (!ctx->stop_flag.p) HIPCHK(ctx->stop_flag.alloc(1));
   HIPCHK(hipMemsetAsync(ctx->stop_flag.p, 0, sizeof(int), st));
@@ -407,7 +407,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   const bool headwave = !(flags & FMT2D_FWD_NO_HEADWAVE);
   // arrival-time window: `window_tiles` tile crossings at 4 km/s
   const double window = ctx->window_tiles * std::max(tx * m.dx, tz * m.dz) / 4.0;
-  size_t nev = 0;  // event pairs recorded around fim launches
+  size_t nev = 0;       // tile-solver dispatches
+  double fim_ms = 0.0;  // their time (first wave start to last wave end)
   for (int L = 0; L < 4; L++) {
     const int f0 = level_off[L], nf = level_off[L + 1] - level_off[L];
     const FieldDesc* dl = ctx->descs.p + f0;
@@ -415,42 +416,75 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz,
                                      program ? ctx->isecs_prog.p : ctx->isecs.p, st));
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
-    // [1] / [2] sizes of active lists 0 / 1
-    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 4, st));
+    // [1] / [2] sizes of active lists 0 / 1, [4] launches of the level
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 5, st));
     if (f32) HIPCHK(launch_f32_reset(dl, nf, nx * nz, st));
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, tx, tz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
                              ctx->tcount.p + 1, st));
-    if (ctx->ttrace.n < (size_t)max_launch + 1) HIPCHK(ctx->ttrace.alloc((size_t)max_launch + 1));
-    if (ctx->tmax.n < (size_t)max_launch + 1) HIPCHK(ctx->tmax.alloc((size_t)max_launch + 1));
-    HIPCHK(hipMemsetAsync(ctx->tmax.p, 0, sizeof(int) * ((size_t)max_launch + 1), st));
-    // launches are queued in batches; the host reads the active count once per batch
+    // Launches run in batches of BATCH, captured once per level in a HIP
+    // graph and replayed until the active set is empty: every argument of a
+    // batch is the same at each replay (the active lists alternate, and
+    // with them the checkerboard colour, with the launch's parity; the
+    // launch number the kernels record under is counted on the device).
+    // The host reads the active count once per replay.
     constexpr int BATCH = 8;
     const int fim_grid = 8192, sched_grid = 512;
-    int cur = 0, launches = 0;
+    const int trace_cap = max_launch + BATCH;
+    if (ctx->ttrace.n < (size_t)trace_cap) HIPCHK(ctx->ttrace.alloc((size_t)trace_cap));
+    if (ctx->tmax.n < (size_t)trace_cap) HIPCHK(ctx->tmax.alloc((size_t)trace_cap));
+    const size_t nspan = (size_t)trace_cap * (SPAN_SLOTS + 1);
+    if (ctx->tspan.n < nspan) HIPCHK(ctx->tspan.alloc(nspan));
+    HIPCHK(hipMemsetAsync(ctx->tmax.p, 0, sizeof(int) * (size_t)trace_cap, st));
+    HIPCHK(hipMemsetAsync(ctx->tspan.p, 0, sizeof(unsigned long long) * nspan, st));
+    if (ctx->clock_hz <= 0.0) {
+      int khz = 0;
+      HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+      if (khz <= 0) return fail(FMT2D_EHIP, "the device reports no constant clock rate");
+      ctx->clock_hz = 1000.0 * khz;
+    }
+    auto record_batch = [&]() -> hipError_t {
+      for (int b = 0; b < BATCH; b++) {
+        const int cur = b & 1, nxt = cur ^ 1;
+        hipError_t e = launch_schedule(ctx->tlist[cur].p, ctx->tcount.p + 1 + cur, nf, ntx, ntiles, ctx->tkey.p,
+                                       ctx->tfront.p, window, b & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p,
+                                       ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->tcount.p + 4, ctx->ttrace.p,
+                                       trace_cap, dl, ctx->theavy.p, (int)ntt, sched_grid, st);
+        if (e == hipSuccess)
+          e = launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
+                         ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps,
+                         ctx->max_sweeps_heavy, ctx->colors, ctx->acc.p, ctx->tcount.p + 4, ctx->tmax.p, trace_cap,
+                         ctx->tspan.p, (int)ntt, f32 ? 1 : 0, ctx->stop_flag.p, st);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    {
+      const hipError_t e = record_batch();
+      const hipError_t e2 = hipStreamEndCapture(st, &graph);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        HIPCHK(e != hipSuccess ? e : e2);
+      }
+    }
+    {
+      const hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(e);
+    }
+    struct ExecGuard {
+      hipGraphExec_t g;
+      ~ExecGuard() { if (g) (void)hipGraphExecDestroy(g); }
+    } guard{gexec};
+    const int cur = 0;  // a batch has an even number of launches: the active list is list 0 again
+    int launches = 0;
     bool done = false;
     while (!done) {
-      for (int b = 0; b < BATCH; b++, launches++) {
-        if (launches >= max_launch) break;
-        const int nxt = cur ^ 1;
-        HIPCHK(launch_schedule(ctx->tlist[cur].p, ctx->tcount.p + 1 + cur, nf, ntx, ntiles, ctx->tkey.p,
-                               ctx->tfront.p, window, launches & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p,
-                               ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, launches > 0 ? ctx->ttrace.p + launches - 1
-                                                                                          : nullptr,
-                               dl, ctx->theavy.p, (int)ntt, sched_grid, st));
-        const size_t ei = 2 * nev++;
-        while (ctx->ev.size() < ei + 2) {
-          hipEvent_t e;
-          HIPCHK(hipEventCreate(&e));
-          ctx->ev.push_back(e);
-        }
-        HIPCHK(hipEventRecord(ctx->ev[ei], st));
-        HIPCHK(launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
-                          ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps, ctx->max_sweeps_heavy,
-                          ctx->colors,
-                          ctx->acc.p, ctx->tmax.p + launches, (int)ntt, f32 ? 1 : 0, ctx->stop_flag.p, st));
-        HIPCHK(hipEventRecord(ctx->ev[ei + 1], st));
-        cur = nxt;
-      }
+      HIPCHK(hipGraphLaunch(gexec, st));
+      launches += BATCH;
+      nev += BATCH;
       HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       if (ctx->h_count[1 + cur] == 0) { done = true; break; }
@@ -469,6 +503,14 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       }
     }
     {
+      // tile-solver time: the launches' spans on the device clock
+      std::vector<unsigned long long> sp((size_t)launches * (SPAN_SLOTS + 1));
+      HIPCHK(hipMemcpy(sp.data(), ctx->tspan.p, sizeof(unsigned long long) * sp.size(), hipMemcpyDeviceToHost));
+      for (int k = 0; k < launches; k++) {
+        const unsigned long long* q = sp.data() + (size_t)k * (SPAN_SLOTS + 1);
+        const unsigned long long end = *std::max_element(q + 1, q + 1 + SPAN_SLOTS);
+        if (end > q[0]) fim_ms += 1e3 * (double)(end - q[0]) / ctx->clock_hz;
+      }
       // due tiles per launch (the last one is still in the due counter)
       std::vector<int> tr(launches), mx(launches);
       if (launches > 1)
@@ -544,12 +586,6 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   HIPCHK(hipStreamSynchronize(st));
   ctx->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-  double fim_ms = 0.0;
-  for (size_t k = 0; k < nev; k++) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
-    fim_ms += ms;
-  }
   ctx->stats.ms_fim = fim_ms;
   int stop = 0;
   HIPCHK(hipMemcpy(&stop, ctx->stop_flag.p, sizeof(int), hipMemcpyDeviceToHost));

@@ -115,6 +115,8 @@ struct fmt2d_ctx {
   DBuf<int> tcount;                   // [due, size of list 0, size of list 1]
   DBuf<int> ttrace;                   // due tiles per launch of a level
   DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
+  DBuf<unsigned long long> tspan;     // device clock at the start / ends of each launch of a level
+  double clock_hz = 0.0;              // rate of the device's constant clock (wall_clock64)
   DBuf<uint8_t> theavy;               // per (region, tile): cut by an interface (slow visits)
   double window_tiles = 3.0;          // window in tile crossing times
   int max_sweeps = 8;                 // a visit over the cap stays active and continues later
@@ -221,13 +223,11 @@ struct fmt2d_ctx {
   DBuf<double> gp_val;
   int gp_nnz = -1;                     // -1: not formed for the current trace
   int* h_count = nullptr;              // pinned
-  std::vector<hipEvent_t> ev;          // start/stop pairs of the tile-solver launches
   fmt2d_forward_stats stats{};
   std::vector<int> trace;              // active tiles per launch (all levels)
   std::vector<int> trace_sweeps;       // most sweeps of a tile visit per launch
   long long sweep_hist[64] = {};       // tile visits by sweep count (last bin: 64 or more)
   ~fmt2d_ctx() {
-    for (auto e : ev) (void)hipEventDestroy(e);
     if (h_count) (void)hipHostFree(h_count);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);

@@ -78,13 +78,18 @@ hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int 
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
                            unsigned long long* tkey, unsigned long long* front, double window, int color,
                            unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* trace_prev, const FieldDesc* descs, const uint8_t* heavy, int due_cap, int grid,
+                           int* launch_no, int* trace, int trace_cap, const FieldDesc* descs, const uint8_t* heavy,
+                           int due_cap, int grid,
                            hipStream_t st);
+// Per launch, span holds SPAN_SLOTS + 1 stamps of the device's constant
+// clock: the start, then end stamps (the launch ends at the largest).
+constexpr int SPAN_SLOTS = 64;
 // tile shapes tx x tz: 8x8, 16x8, 8x16 (hipErrorInvalidValue otherwise)
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
                       int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, int* launch_max, int due_cap, int f32, int* flag,
+                      unsigned long long* acc, const int* launch_no, int* sweep_max, int trace_cap,
+                      unsigned long long* span, int due_cap, int f32, int* flag,
                       hipStream_t st);
 // per field: max |T - snap| over slots both reach, count of slots one
 // reaches, count of slots both reach with |T - snap| > tol

@@ -309,11 +309,15 @@ __device__ inline int due_count(const int* count_due) { return count_due[0] + co
 
 // Clears the counters of a launch: due tiles and the next active list.
 // The previous launch's due count goes to its trace slot (when given).
+// launch_no counts the launches of a level (the kernels of a launch read it,
+// so a batch of launches has the same arguments every time it is replayed).
 __global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned long long* front, int nfield,
-                                    int* trace_prev) {
+                                    int* launch_no, int* trace, int trace_cap) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f == 0) {
-    if (trace_prev) *trace_prev = due_count(count_due);
+    const int prev = *launch_no - 1;
+    if (prev >= 0 && prev < trace_cap) trace[prev] = due_count(count_due);
+    *launch_no = prev + 2;
     count_due[0] = 0;
     count_due[3] = 0;
     *count_next = 0;
@@ -724,8 +728,19 @@ __global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(c
                                                        unsigned int* in_set, unsigned long long* tkey,
                                                        int2* list_next, int* count_next, int max_sweeps,
                                                        int max_sweeps_heavy, int colors, unsigned long long* acc,
-                                                       int* launch_max, int due_cap, int* flag) {
+                                                       const int* launch_no, int* sweep_max, int trace_cap,
+                                                       unsigned long long* span, int due_cap, int* flag) {
   __shared__ TileLds<TX, TZ> L;
+  // most sweeps of a visit in this launch, recorded under the launch number
+  const int lno = *launch_no - 1;
+  const bool slot = lno >= 0 && lno < trace_cap;
+  int* launch_max = slot ? sweep_max + lno : nullptr;
+  // the launch's span on the device's constant-rate clock: block 0 (the
+  // first dispatched) stamps the start in slot 0; every block that visited
+  // a tile, and block 0, stamps its end into one of SPAN_SLOTS slots
+  // (atomicMax, spread over slots to keep the atomics uncontended)
+  unsigned long long* sp = span + (size_t)lno * (SPAN_SLOTS + 1);
+  if (slot && blockIdx.x == 0 && threadIdx.x == 0) sp[0] = wall_clock64();
   const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
@@ -735,6 +750,8 @@ __global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(c
                      k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, flag, L);
     __syncthreads();
   }
+  if (slot && threadIdx.x == 0 && ((int)blockIdx.x < n || blockIdx.x == 0))
+    atomicMax(sp + 1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
 }
 
 // Post-pass: refit one-sided gradients of grid nodes; everything else keeps
@@ -1247,10 +1264,10 @@ hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int 
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
                            unsigned long long* tkey, unsigned long long* front, double window, int color,
                            unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* trace_prev, const FieldDesc* descs, const uint8_t* heavy, int due_cap, int grid,
-                           hipStream_t st) {
+                           int* launch_no, int* trace, int trace_cap, const FieldDesc* descs, const uint8_t* heavy,
+                           int due_cap, int grid, hipStream_t st) {
   hipLaunchKernelGGL(launch_reset_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, count_due, count_carry, front,
-                     nfield, trace_prev);
+                     nfield, launch_no, trace, trace_cap);
   hipLaunchKernelGGL(front_kernel, dim3(grid), dim3(256), 0, st, list, count, ntiles, tkey, front);
   hipLaunchKernelGGL(split_kernel, dim3(grid), dim3(256), 0, st, list, count, ntx, ntiles, tkey, front, window, color,
                      in_set, due, count_due, carry, count_carry, descs, heavy, due_cap);
@@ -1347,25 +1364,25 @@ hipError_t launch_f32_export(const FieldDesc* descs, int nfield, int nslotg, hip
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
                       int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, int* launch_max, int due_cap, int f32, int* flag,
-                      hipStream_t st) {
+                      unsigned long long* acc, const int* launch_no, int* sweep_max, int trace_cap,
+                      unsigned long long* span, int due_cap, int f32, int* flag, hipStream_t st) {
   if (f32) {
     if (tx != 8 || tz != 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL((fim_tiles_kernel<8, 8, true>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx,
-                       ntz, in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap, flag);
+                       ntz, in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
+                       sweep_max, trace_cap, span, due_cap, flag);
   } else if (tx == 8 && tz == 8)
     hipLaunchKernelGGL((fim_tiles_kernel<8, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap, flag);
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
+                       sweep_max, trace_cap, span, due_cap, flag);
   else if (tx == 16 && tz == 8)
     hipLaunchKernelGGL((fim_tiles_kernel<16, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap, flag);
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
+                       sweep_max, trace_cap, span, due_cap, flag);
   else if (tx == 8 && tz == 16)
     hipLaunchKernelGGL((fim_tiles_kernel<8, 16, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_max,
-                       due_cap, flag);
+                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
+                       sweep_max, trace_cap, span, due_cap, flag);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -82,7 +82,9 @@ typedef struct {
   long long tiles;          /* tiles processed over all launches */
   long long region_nodes;   /* sum of region node counts over the fields */
   double ms_total;          /* wall time of the forward */
-  double ms_fim;            /* summed device time of the tile-solver launches (HIP events) */
+  double ms_fim;            /* summed device time of the tile-solver launches: first wave start to last
+                               wave end of each, on the device's constant clock (the launches run in
+                               replayed HIP graphs) */
   long long exit_nodes;     /* nodes checked exactly by the heap-exhaustion stop */
   long long sweeps;         /* in-tile sweeps over all tile visits */
   long long capped;         /* tile visits that reached the sweep cap */
