@@ -122,7 +122,7 @@ int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int co
 int fmt2d_set_heavy_sweeps(fmt2d_ctx* ctx, int max_sweeps_heavy);
 /* Storage precision of the tile solver's grid-node fields: 64 (default;
  * every result bit-exact against the reference) or 32: T stored in 32-bit
- * fixed point (units of 2^-22 s = 2.4e-7 s, times below 511 s; one grid of
+ * fixed point (units of 2^-24 s = 6.0e-8 s, times up to 255 s; one grid of
  * values for every node, so the result does not depend on the order of
  * the updates), raw gradients in FP32, every update still computed in FP64
  * and rounded to the stored values; 8x8 tiles.  Interface nodes, seeds and

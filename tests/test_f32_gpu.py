@@ -1,6 +1,6 @@
 """32-bit field storage of the tile solver (fmt2d_set_field_precision 32)
 on the 0.1 km grid (2001x301), three OBS of the 60-OBS multistage case, all
-twelve stage fields: grid-node times stored in 32-bit fixed point (2^-22 s),
+twelve stage fields: grid-node times stored in 32-bit fixed point (2^-24 s),
 raw gradients in FP32, every update computed in FP64.
 
 Against the FP64 path (fmt2d_field_snapshot / fmt2d_field_compare):
