@@ -149,7 +149,7 @@ __device__ int position_in_region(const RayGeomDev& G, const RayRegionDev& R, do
 }
 
 // interpolate_time_gradient2d (rays2d.f90:825-894)
-__device__ void interp_grad(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz,
+__device__ __attribute__((always_inline)) inline void interp_grad(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz,
                             double& dtdx, double& dtdz) {
   int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
   if (ix < 1) ix = 1;
@@ -157,16 +157,21 @@ __device__ void interp_grad(const RayGeomDev& G, const RayRegionDev& R, const Ra
   if (iz < 1) iz = 1;
   if (iz >= G.nz) iz = G.nz - 1;
   const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
-  int n1 = rnode(G, R, ix, iz), n2 = rnode(G, R, ix + 1, iz), n3 = rnode(G, R, ix, iz + 1),
-      n4 = rnode(G, R, ix + 1, iz + 1);
-  if (n1 < 0) n1 = rnode_nearby(G, R, ix, iz);
-  if (n2 < 0) n2 = rnode_nearby(G, R, ix + 1, iz);
-  if (n3 < 0) n3 = rnode_nearby(G, R, ix, iz + 1);
-  if (n4 < 0) n4 = rnode_nearby(G, R, ix + 1, iz + 1);
-  const double g1x = n1 >= 0 ? F.GX[n1] : 0.0, g1z = n1 >= 0 ? F.GZ[n1] : 0.0;
-  const double g2x = n2 >= 0 ? F.GX[n2] : 0.0, g2z = n2 >= 0 ? F.GZ[n2] : 0.0;
-  const double g3x = n3 >= 0 ? F.GX[n3] : 0.0, g3z = n3 >= 0 ? F.GZ[n3] : 0.0;
-  const double g4x = n4 >= 0 ? F.GX[n4] : 0.0, g4z = n4 >= 0 ? F.GZ[n4] : 0.0;
+  // the cell's corners lie in the grid (ix < nx, iz < nz): their gradients
+  // are loaded together with their region flags, and a corner outside the
+  // region takes the nearby node's instead
+  const int q1 = (iz - 1) * G.nx + (ix - 1), q2 = q1 + 1, q3 = q1 + G.nx, q4 = q3 + 1;
+  double g1x = F.GX[q1], g1z = F.GZ[q1], g2x = F.GX[q2], g2z = F.GZ[q2];
+  double g3x = F.GX[q3], g3z = F.GZ[q3], g4x = F.GX[q4], g4z = F.GZ[q4];
+  const bool in1 = R.flags[q1] & RF_IN, in2 = R.flags[q2] & RF_IN, in3 = R.flags[q3] & RF_IN,
+             in4 = R.flags[q4] & RF_IN;
+  if (!in1) { const int m = rnode_nearby(G, R, ix, iz); g1x = m >= 0 ? F.GX[m] : 0.0; g1z = m >= 0 ? F.GZ[m] : 0.0; }
+  if (!in2) { const int m = rnode_nearby(G, R, ix + 1, iz); g2x = m >= 0 ? F.GX[m] : 0.0; g2z = m >= 0 ? F.GZ[m] : 0.0; }
+  if (!in3) { const int m = rnode_nearby(G, R, ix, iz + 1); g3x = m >= 0 ? F.GX[m] : 0.0; g3z = m >= 0 ? F.GZ[m] : 0.0; }
+  if (!in4) {
+    const int m = rnode_nearby(G, R, ix + 1, iz + 1);
+    g4x = m >= 0 ? F.GX[m] : 0.0; g4z = m >= 0 ? F.GZ[m] : 0.0;
+  }
   double gx = (1.0 - w1) * (1.0 - w2) * g1x + w1 * (1.0 - w2) * g2x + (1.0 - w1) * w2 * g3x + w1 * w2 * g4x;
   double gz = (1.0 - w1) * (1.0 - w2) * g1z + w1 * (1.0 - w2) * g2z + (1.0 - w1) * w2 * g3z + w1 * w2 * g4z;
   if (sqrt(gx * gx + gz * gz) < 0.01) {
@@ -180,24 +185,22 @@ __device__ void interp_grad(const RayGeomDev& G, const RayRegionDev& R, const Ra
 }
 
 // interpolate_arrivaltime2d (rays2d.f90:898-958)
-__device__ double interp_time(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz) {
+__device__ __attribute__((always_inline)) inline double interp_time(const RayGeomDev& G, const RayRegionDev& R, const RayField& F, double px, double pz) {
   int ix = ifloor((px - G.x0) / G.dx) + 1, iz = ifloor((pz - G.z0) / G.dz) + 1;
   if (ix < 1) ix = 1;
   if (ix >= G.nx) ix = G.nx - 1;
   if (iz < 1) iz = 1;
   if (iz >= G.nz) iz = G.nz - 1;
   const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
-  const int n1 = rnode(G, R, ix, iz), n2 = rnode(G, R, ix + 1, iz), n3 = rnode(G, R, ix, iz + 1),
-            n4 = rnode(G, R, ix + 1, iz + 1);
-  double t1 = RAY_HUGE, t2 = RAY_HUGE, t3 = RAY_HUGE, t4 = RAY_HUGE;
-  if (n1 >= 0) t1 = F.T[n1];
-  if (n2 >= 0) t2 = F.T[n2];
-  if (n3 >= 0) t3 = F.T[n3];
-  if (n4 >= 0) t4 = F.T[n4];
-  if (n1 < 0) { const int m = rnode_nearby(G, R, ix, iz); if (m >= 0) t1 = F.T[m]; }
-  if (n2 < 0) { const int m = rnode_nearby(G, R, ix + 1, iz); if (m >= 0) t2 = F.T[m]; }
-  if (n3 < 0) { const int m = rnode_nearby(G, R, ix, iz + 1); if (m >= 0) t3 = F.T[m]; }
-  if (n4 < 0) { const int m = rnode_nearby(G, R, ix + 1, iz + 1); if (m >= 0) t4 = F.T[m]; }
+  // corners in the grid: times loaded together with the region flags
+  const int q1 = (iz - 1) * G.nx + (ix - 1), q2 = q1 + 1, q3 = q1 + G.nx, q4 = q3 + 1;
+  double t1 = F.T[q1], t2 = F.T[q2], t3 = F.T[q3], t4 = F.T[q4];
+  const bool in1 = R.flags[q1] & RF_IN, in2 = R.flags[q2] & RF_IN, in3 = R.flags[q3] & RF_IN,
+             in4 = R.flags[q4] & RF_IN;
+  if (!in1) { const int m = rnode_nearby(G, R, ix, iz); t1 = m >= 0 ? F.T[m] : RAY_HUGE; }
+  if (!in2) { const int m = rnode_nearby(G, R, ix + 1, iz); t2 = m >= 0 ? F.T[m] : RAY_HUGE; }
+  if (!in3) { const int m = rnode_nearby(G, R, ix, iz + 1); t3 = m >= 0 ? F.T[m] : RAY_HUGE; }
+  if (!in4) { const int m = rnode_nearby(G, R, ix + 1, iz + 1); t4 = m >= 0 ? F.T[m] : RAY_HUGE; }
   const double half = RAY_HUGE * 0.5;
   if (t1 < half && t2 < half && t3 < half && t4 < half)
     return (1.0 - w1) * (1.0 - w2) * t1 + w1 * (1.0 - w2) * t2 + (1.0 - w1) * w2 * t3 + w1 * w2 * t4;
@@ -341,7 +344,7 @@ __device__ bool interface_crossing(const RayGeomDev& G, double2* pt, int n, cons
 
 // Headwave / refraction decision at the section's ending interface
 // (rays2d.f90:451-455 and repeats): true = the ray refracts (section ends).
-__device__ bool refracts(const RayGeomDev& G, const SecInfo& S, int ifend, double px, double zi, double grad0[2],
+__device__ __attribute__((always_inline)) inline bool refracts(const RayGeomDev& G, const SecInfo& S, int ifend, double px, double zi, double grad0[2],
                          RayState& st) {
   double nx_, nz_, h, g1[2];
   iface_normal(G, ifend, px, nx_, nz_, h);
@@ -354,7 +357,7 @@ __device__ bool refracts(const RayGeomDev& G, const SecInfo& S, int ifend, doubl
 
 // trace_ray_section2d (rays2d.f90:157-720).  pt holds RAY_MAXP points; on
 // return *npts is the section's point count (0 when invalid before any).
-__device__ void trace_section(const RayGeomDev& G, const SecInfo& S, double sx, double sz, double x_in, double z_in,
+__device__ __attribute__((always_inline)) inline void trace_section(const RayGeomDev& G, const SecInfo& S, double sx, double sz, double x_in, double z_in,
                               double2* pt, int* npts, double& x_out, double& z_out, RayState& st) {
   const RayRegionDev& R = *S.R;
   const int iftop = R.itop, ifbot = R.ibot;
@@ -610,7 +613,7 @@ __global__ void seafloor_times_kernel(const RayGeomDev* __restrict__ Gp, const R
   tsf[k] = interp_time(G, G.reg[F.region], F, G.sfx[i], G.sfz[i]);
 }
 
-__global__ void trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
+__global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
                              const double* __restrict__ src_x, const double* __restrict__ src_z,
                              const double* __restrict__ tsf, const RayJob* __restrict__ jobs, int nray,
                              const long long* __restrict__ pt_off, double2* pts, RayOut* out, int program) {
