@@ -10,7 +10,10 @@ completes 5 x nobs source-stages.  Default workload: the batched forward on
 the 0.1 km grid (2001x301 nodes), 200 OBS every 1 km with the sediment base
 and Moho perturbed by seeded Fourier modes: 1000 source-stages (2400 fields)
 in one fmt2d_forward.  ``multistage`` (60 OBS every 3.3 km) is reported
-beside it.
+beside it, and ``per_source``: the same 200-OBS forward solved one OBS at a
+time (the per-source stage loop of the reference, on the GPU), timed on four
+OBS across the model and scaled to all 200, with the batched step's speedup
+over it.
 
 Second figure (``iteration``): seconds per inversion iteration = forward +
 rays + Frechet rows + system assembly + one LSQR solve (fixed damping, update
@@ -540,7 +543,7 @@ def joint_figure(iters: int, cpu: bool = False):
 def fp32_figure(steps: int = 2):
     """The multistage forward (2001x301, 60 OBS) with the tile solver's
     grid-node fields stored in FP32 (fmt2d_set_field_precision 32: T as an
-    FP32 offset from an FP64 base per tile, gradients FP32, compute FP64)
+    32-bit fixed point of 2^-24 s, gradients FP32, compute FP64)
     beside FP64 storage: time, bytes per node of the solver's field storage
     and, per stage field, the largest |dT| against the FP64 fields; the
     fields with nodes beyond 1e-4 s are named."""
@@ -586,14 +589,18 @@ def fp32_figure(steps: int = 2):
             "reach_mismatches": int(np.sum(nd)), "max_abs_dt_s": float(mdt.max()), "named_fields": named[:40]}
 
 
-def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int, autotune: bool = False):
+def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int, autotune: bool = False,
+                   per_source: int = 0):
     """Source-stages per second of one workload's forward (the headline's
-    measurement; see the module docstring), with the tile-kernel time from
-    HIP events on the solver stream and, when cpu_sample > 0, the reference
-    timed on that many of the workload's OBS.  autotune: the tile schedule
-    is first picked by fmt2d_autotune on this model and these OBS (before
-    the warm-up, outside the timed steps); every schedule gives the same
-    fields bit for bit."""
+    measurement; see the module docstring), with the tile-kernel time (the
+    dispatches' spans, stamped on the device clock inside the replayed
+    graphs) and, when cpu_sample > 0, the reference timed on that many of
+    the workload's OBS.  autotune: the tile schedule is first picked by
+    fmt2d_autotune on this model and these OBS (before the warm-up, outside
+    the timed steps); every schedule gives the same fields bit for bit.
+    per_source > 0: the same forward solved one OBS at a time (the per-stage
+    path of the reference's loop, each OBS's stage chain on its own) for
+    that many OBS spread over the model, after the timed steps."""
     from fmt2d_amd import hip, formats
     m, g, desc = workload(workload_name)
     with tempfile.TemporaryDirectory() as d:
@@ -624,10 +631,23 @@ def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int,
             launches.append(st.launches)
             tiles.append(st.tiles)
             disp.append(st.dispatches)
+        single = None
+        if per_source > 0:
+            pick = sorted({int(round(k * (len(obs) - 1) / max(per_source - 1, 1))) for k in range(per_source)})
+            ctx.forward([obs[pick[0]]])  # field storage for one OBS, outside the timing
+            sw, sl = [], []
+            for k in pick:
+                t0 = time.perf_counter()
+                ctx.forward([obs[k]])
+                sw.append(time.perf_counter() - t0)
+                sl.append(ctx.stats().launches)
+            per_obs = float(np.mean(sw))
+            single = {"obs_timed": [obs[k] for k in pick], "seconds_per_obs": round(per_obs, 4),
+                      "launches_per_obs": int(np.mean(sl)), "seconds_all_obs_estimate": round(per_obs * len(obs), 2)}
         cpu = cpu_baseline(d, obs, cpu_sample) if cpu_sample > 0 else None
         ctx.close()
     return dict(desc=desc, obs=obs, walls=walls, fims=fims, nodes=nodes, launches=launches, tiles=tiles, disp=disp,
-                cpu=cpu, schedule=sched)
+                cpu=cpu, schedule=sched, single=single)
 
 
 def main():
@@ -644,6 +664,7 @@ def main():
     ap.add_argument("--no-programs", action="store_true", help="skip the host-program figure")
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 field-storage figure")
+    ap.add_argument("--no-per-source", action="store_true", help="skip the one-OBS-at-a-time forward figure")
     ap.add_argument("--joint", type=int, default=4,
                     help="iterations of the joint inversion on the 0.5 km / 1 km mesh (2001x301, 200 OBS, five "
                          "phases; about a minute for 4); 0 skips it")
@@ -677,7 +698,8 @@ def main():
                                                            args.cpu_sample > 0)}))
         return
 
-    head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample, autotune=not args.no_autotune)
+    head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample, autotune=not args.no_autotune,
+                          per_source=0 if args.no_per_source else 4)
     print(f"headline forward done: {head['walls']}", file=sys.stderr, flush=True)
     ms = None
     if args.workload != "multistage" and not args.no_multistage:
@@ -734,6 +756,11 @@ def main():
                                           f"read, T and grad T written) = {alg_bytes} B per step, "
                                           f"{alg_bytes / ndisp:.0f} B per dispatch"},
         "cpu_baseline": head["cpu"],
+        "per_source": None if head["single"] is None else dict(
+            head["single"], workload="the same forward solved one OBS at a time (each OBS's twelve stage fields "
+                                     "on their own, as the reference's per-source loop runs them), timed on the "
+                                     "OBS listed and scaled to all of them",
+            batched_speedup=round(head["single"]["seconds_all_obs_estimate"] / (wall / args.steps), 1)),
         "multistage": ms,
         "fp32_storage": f32,
         "iteration": it,
