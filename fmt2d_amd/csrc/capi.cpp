@@ -67,7 +67,7 @@ int fmt2d_ctx_create(int device, fmt2d_ctx** out) {
   fmt2d_ctx* c = new fmt2d_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_count, sizeof(int) * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_count, sizeof(int) * 16, hipHostMallocDefault);
   if (e != hipSuccess) {
     delete c;
     return fail(FMT2D_EHIP, std::string("stream/pinned allocation: ") + hipGetErrorString(e));
@@ -396,8 +396,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   // returned early (launch cap, HIP error) leaves tiles marked active
   HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
   HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
-  if (ctx->tfront.n < (size_t)maxlev) HIPCHK(ctx->tfront.alloc(maxlev));
-  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(5));
+  if (ctx->tfront.n < 2 * (size_t)maxlev) HIPCHK(ctx->tfront.alloc(2 * (size_t)maxlev));
+  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(12));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
   if (!ctx->stop_flag.p) HIPCHK(ctx->stop_flag.alloc(1));
   HIPCHK(hipMemsetAsync(ctx->stop_flag.p, 0, sizeof(int), st));
@@ -416,17 +416,15 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       HIPCHK(launch_transfer_reflect(ctx->descs.p, f0, ctx->links.p, nf, nx, nz,
                                      program ? ctx->isecs_prog.p : ctx->isecs.p, st));
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
-    // [1] / [2] sizes of active lists 0 / 1, [4] launches of the level
-    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 5, st));
+    // [1] / [2] sizes of active lists 0 / 1, [4] launches of the level,
+    // [8] / [11] the due counters of odd launches in the level kernel
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 12, st));
+    // fronts of lists 0 and 1: no key (seeding lowers list 0's)
+    unsigned long long* front[2] = {ctx->tfront.p, ctx->tfront.p + nf};
+    HIPCHK(hipMemsetAsync(ctx->tfront.p, 0x7F, sizeof(unsigned long long) * 2 * nf, st));
     if (f32) HIPCHK(launch_f32_reset(dl, nf, nx * nz, st));
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, tx, tz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
-                             ctx->tcount.p + 1, st));
-    // Launches run in batches of BATCH, captured once per level in a HIP
-    // graph and replayed until the active set is empty: every argument of a
-    // batch is the same at each replay (the active lists alternate, and
-    // with them the checkerboard colour, with the launch's parity; the
-    // launch number the kernels record under is counted on the device).
-    // The host reads the active count once per replay.
+                             ctx->tcount.p + 1, front[0], st));
     constexpr int BATCH = 8;
     const int fim_grid = 8192, sched_grid = 512;
     const int trace_cap = max_launch + BATCH;
@@ -442,65 +440,104 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       if (khz <= 0) return fail(FMT2D_EHIP, "the device reports no constant clock rate");
       ctx->clock_hz = 1000.0 * khz;
     }
-    auto record_batch = [&]() -> hipError_t {
-      for (int b = 0; b < BATCH; b++) {
-        const int cur = b & 1, nxt = cur ^ 1;
-        hipError_t e = launch_schedule(ctx->tlist[cur].p, ctx->tcount.p + 1 + cur, nf, ntx, ntiles, ctx->tkey.p,
-                                       ctx->tfront.p, window, b & 1, ctx->tin.p, ctx->tlist[2].p, ctx->tcount.p,
-                                       ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->tcount.p + 4, ctx->ttrace.p,
-                                       trace_cap, dl, ctx->theavy.p, (int)ntt, sched_grid, st);
-        if (e == hipSuccess)
-          e = launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
-                         ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, ctx->max_sweeps,
-                         ctx->max_sweeps_heavy, ctx->colors, ctx->acc.p, ctx->tcount.p + 4, ctx->tmax.p, trace_cap,
-                         ctx->tspan.p, (int)ntt, f32 ? 1 : 0, ctx->stop_flag.p, st);
-        if (e != hipSuccess) return e;
+    // active tiles left after the cap: the error names a few of them
+    auto no_convergence = [&](int cur, int n) {
+      std::vector<int2> tail(std::min(n, 8));
+      (void)hipMemcpy(tail.data(), ctx->tlist[cur].p, tail.size() * sizeof(int2), hipMemcpyDeviceToHost);
+      std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
+                        ", " + std::to_string(n) + " tiles active; field/tile:";
+      for (auto& p : tail) {
+        const int itf = order[f0 + p.x] % FMT2D_NTF + 1;
+        msg += " tf" + std::to_string(itf) + "/src" + std::to_string(order[f0 + p.x] / FMT2D_NTF) + "/t" +
+               std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * tx) + "," + std::to_string((p.y / ntx) * tz) +
+               ")";
       }
-      return hipSuccess;
+      return fail(FMT2D_ENOCONV, msg + ")");
     };
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    {
-      const hipError_t e = record_batch();
-      const hipError_t e2 = hipStreamEndCapture(st, &graph);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        HIPCHK(e != hipSuccess ? e : e2);
-      }
-    }
-    {
-      const hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HIPCHK(e);
-    }
-    struct ExecGuard {
-      hipGraphExec_t g;
-      ~ExecGuard() { if (g) (void)hipGraphExecDestroy(g); }
-    } guard{gexec};
-    const int cur = 0;  // a batch has an even number of launches: the active list is list 0 again
     int launches = 0;
-    bool done = false;
-    while (!done) {
-      HIPCHK(hipGraphLaunch(gexec, st));
-      launches += BATCH;
-      nev += BATCH;
-      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      if (ctx->h_count[1 + cur] == 0) { done = true; break; }
-      if (launches >= max_launch) {
-        const int n = ctx->h_count[1 + cur];
-        std::vector<int2> tail(std::min(n, 8));
-        (void)hipMemcpy(tail.data(), ctx->tlist[cur].p, tail.size() * sizeof(int2), hipMemcpyDeviceToHost);
-        std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
-                          ", " + std::to_string(n) + " tiles active; field/tile:";
-        for (auto& p : tail) {
-          const int itf = order[f0 + p.x] % FMT2D_NTF + 1;
-          msg += " tf" + std::to_string(itf) + "/src" + std::to_string(order[f0 + p.x] / FMT2D_NTF) + "/t" +
-                 std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * tx) + "," + std::to_string((p.y / ntx) * tz) + ")";
-        }
-        return fail(FMT2D_ENOCONV, msg + ")");
+    if (ctx->solver_launch == FMT2D_LAUNCH_LEVEL) {
+      // one cooperative dispatch runs every launch of the level
+      const int gi = f32 ? 3 : (tx == 16 ? 1 : (tz == 16 ? 2 : 0));
+      if (ctx->level_grid[gi] <= 0) {
+        const hipError_t e = level_grid(ctx->device, tx, tz, f32 ? 1 : 0, &ctx->level_grid[gi]);
+        if (e == hipErrorNotSupported) return fail(FMT2D_EHIP, "the device does not run cooperative launches");
+        HIPCHK(e);
       }
+      LevelSched a{};
+      a.descs = dl; a.regs = ctx->regs.p; a.heavy = ctx->theavy.p;
+      a.list[0] = ctx->tlist[0].p; a.list[1] = ctx->tlist[1].p; a.due = ctx->tlist[2].p;
+      a.count = ctx->tcount.p; a.in_set = ctx->tin.p; a.tkey = ctx->tkey.p;
+      a.front[0] = front[0]; a.front[1] = front[1];
+      a.acc = ctx->acc.p; a.trace = ctx->ttrace.p; a.sweep_max = ctx->tmax.p; a.span = ctx->tspan.p;
+      a.flag = ctx->stop_flag.p; a.window = window;
+      a.nfield = nf; a.ntx = ntx; a.ntz = ntz; a.ntiles = ntiles; a.due_cap = (int)ntt;
+      a.max_sweeps = ctx->max_sweeps; a.max_sweeps_heavy = ctx->max_sweeps_heavy; a.colors = ctx->colors;
+      a.max_launch = max_launch; a.trace_cap = trace_cap;
+      HIPCHK(launch_fim_level(a, tx, tz, f32 ? 1 : 0, ctx->level_grid[gi], st));
+      nev += 1;
+      HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 5, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      launches = ctx->h_count[4];
+      const int cur = launches & 1;
+      if (ctx->h_count[1 + cur] != 0) return no_convergence(cur, ctx->h_count[1 + cur]);
+      if (launches > 0) HIPCHK(hipMemcpy(ctx->h_count, ctx->ttrace.p + launches - 1, sizeof(int), hipMemcpyDeviceToHost));
+    } else {
+      // Launches run in batches of BATCH, captured once per level in a HIP
+      // graph and replayed until the active set is empty: every argument of
+      // a batch is the same at each replay (the active lists alternate, and
+      // with them the checkerboard colour and the front arrays, with the
+      // launch's parity; the launch number the kernels record under is
+      // counted on the device).  The host reads the active count once per
+      // replay.
+      auto record_batch = [&]() -> hipError_t {
+        for (int b = 0; b < BATCH; b++) {
+          const int cur = b & 1, nxt = cur ^ 1;
+          hipError_t e = launch_schedule(ctx->tlist[cur].p, ctx->tcount.p + 1 + cur, nf, ntx, ntiles, ctx->tkey.p,
+                                         front[cur], front[nxt], window, b & 1, ctx->tin.p, ctx->tlist[2].p,
+                                         ctx->tcount.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt,
+                                         ctx->tcount.p + 4, ctx->ttrace.p, trace_cap, dl, ctx->theavy.p, (int)ntt,
+                                         sched_grid, st);
+          if (e == hipSuccess)
+            e = launch_fim(dl, ctx->regs.p, ctx->tlist[2].p, fim_grid, ctx->tcount.p, ntx, ntz, tx, tz, ctx->tin.p,
+                           ctx->tkey.p, ctx->tlist[nxt].p, ctx->tcount.p + 1 + nxt, front[nxt], ctx->max_sweeps,
+                           ctx->max_sweeps_heavy, ctx->colors, ctx->acc.p, ctx->tcount.p + 4, ctx->tmax.p,
+                           trace_cap, ctx->tspan.p, (int)ntt, f32 ? 1 : 0, ctx->stop_flag.p, st);
+          if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+      };
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t gexec = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      {
+        const hipError_t e = record_batch();
+        const hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          HIPCHK(e != hipSuccess ? e : e2);
+        }
+      }
+      {
+        const hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+      }
+      struct ExecGuard {
+        hipGraphExec_t g;
+        ~ExecGuard() { if (g) (void)hipGraphExecDestroy(g); }
+      } guard{gexec};
+      const int cur = 0;  // a batch has an even number of launches: the active list is list 0 again
+      for (;;) {
+        HIPCHK(hipGraphLaunch(gexec, st));
+        launches += BATCH;
+        nev += BATCH;
+        HIPCHK(hipMemcpyAsync(ctx->h_count, ctx->tcount.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (ctx->h_count[1 + cur] == 0) break;
+        if (launches >= max_launch) return no_convergence(cur, ctx->h_count[1 + cur]);
+      }
+      // the last launch's due count is still in the due counters
+      ctx->h_count[0] += ctx->h_count[3];
     }
     {
       // tile-solver time: the launches' spans on the device clock
@@ -516,7 +553,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       if (launches > 1)
         HIPCHK(hipMemcpy(tr.data(), ctx->ttrace.p, sizeof(int) * (launches - 1), hipMemcpyDeviceToHost));
       if (launches > 0) {
-        tr[launches - 1] = ctx->h_count[0] + ctx->h_count[3];
+        tr[launches - 1] = ctx->h_count[0];
         HIPCHK(hipMemcpy(mx.data(), ctx->tmax.p, sizeof(int) * launches, hipMemcpyDeviceToHost));
       }
       // launches queued after the set emptied (rest of the last batch) had nothing due
@@ -741,6 +778,13 @@ int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int
   out->colors = best.colors; out->window_tiles = best.window;
   out->ms_best = best_ms; out->ms_start = start_ms;
   out->trials = trials;
+  return FMT2D_OK;
+}
+
+int fmt2d_set_solver_launch(fmt2d_ctx* ctx, int mode) {
+  if (!ctx || (mode != FMT2D_LAUNCH_GRAPH && mode != FMT2D_LAUNCH_LEVEL))
+    return fail(FMT2D_EINVAL, "the solver launch mode must be FMT2D_LAUNCH_GRAPH or FMT2D_LAUNCH_LEVEL");
+  ctx->solver_launch = mode;
   return FMT2D_OK;
 }
 

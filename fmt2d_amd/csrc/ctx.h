@@ -110,9 +110,9 @@ struct fmt2d_ctx {
   DBuf<StageLink> links;
   DBuf<unsigned int> tin;             // tile in active set
   DBuf<unsigned long long> tkey;      // tile arrival key
-  DBuf<unsigned long long> tfront;    // per-field front key
+  DBuf<unsigned long long> tfront;    // per-field front key of active lists 0 and 1
   DBuf<int2> tlist[3];                // active set, its next state, due tiles
-  DBuf<int> tcount;                   // [due, size of list 0, size of list 1]
+  DBuf<int> tcount;                   // due counters, sizes of lists 0 / 1, launches (LevelSched)
   DBuf<int> ttrace;                   // due tiles per launch of a level
   DBuf<int> tmax;                     // most sweeps of a tile visit per launch of a level
   DBuf<unsigned long long> tspan;     // device clock at the start / ends of each launch of a level
@@ -132,6 +132,8 @@ struct fmt2d_ctx {
   std::vector<double> snap_src;
   int heavy_tx = 0, heavy_tz = 0;     // shape the slow-tile flags (theavy) were built for
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
+  int solver_launch = 0;              // FMT2D_LAUNCH_GRAPH or FMT2D_LAUNCH_LEVEL
+  int level_grid[4] = {0, 0, 0, 0};   // cooperative grid of the level kernel (8x8, 16x8, 8x16, 8x8 FP32)
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
   int lsqr_fast = 0;                  // LSQR reductions: 0 in the reference's order, 1 tree / wave reductions
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
@@ -222,7 +224,7 @@ struct fmt2d_ctx {
   DBuf<int> gp_rp, gp_col;             // G P of the last trace (0-based columns)
   DBuf<double> gp_val;
   int gp_nnz = -1;                     // -1: not formed for the current trace
-  int* h_count = nullptr;              // pinned
+  int* h_count = nullptr;              // pinned, 16 ints
   fmt2d_forward_stats stats{};
   std::vector<int> trace;              // active tiles per launch (all levels)
   std::vector<int> trace_sweeps;       // most sweeps of a tile visit per launch

@@ -72,25 +72,52 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
                                    int nz, const IsecDev* isecs, hipStream_t st);
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
                              int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                             int* count_next, hipStream_t st);
-// per launch: clear the due / next counters and field fronts, then split
-// the active list (its count on the device) into due tiles and carry-over
+                             int* count_next, unsigned long long* front_next, hipStream_t st);
+// per launch: clear the due / next counters and the next list's field
+// fronts, then split the active list (its count on the device) into due
+// tiles and carry-over (front: the active list's fronts)
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
-                           unsigned long long* tkey, unsigned long long* front, double window, int color,
-                           unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* launch_no, int* trace, int trace_cap, const FieldDesc* descs, const uint8_t* heavy,
-                           int due_cap, int grid,
-                           hipStream_t st);
+                           unsigned long long* tkey, const unsigned long long* front, unsigned long long* front_next,
+                           double window, int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
+                           int* count_carry, int* launch_no, int* trace, int trace_cap, const FieldDesc* descs,
+                           const uint8_t* heavy, int due_cap, int grid, hipStream_t st);
 // Per launch, span holds SPAN_SLOTS + 1 stamps of the device's constant
 // clock: the start, then end stamps (the launch ends at the largest).
 constexpr int SPAN_SLOTS = 64;
 // tile shapes tx x tz: 8x8, 16x8, 8x16 (hipErrorInvalidValue otherwise)
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
-                      int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, const int* launch_no, int* sweep_max, int trace_cap,
-                      unsigned long long* span, int due_cap, int f32, int* flag,
-                      hipStream_t st);
+                      int2* list_next, int* count_next, unsigned long long* front_next, int max_sweeps,
+                      int max_sweeps_heavy, int colors, unsigned long long* acc, const int* launch_no, int* sweep_max,
+                      int trace_cap, unsigned long long* span, int due_cap, int f32, int* flag, hipStream_t st);
+// All launches of a dependency level in one cooperative dispatch.  count
+// (12 ints): [1] / [2] sizes of active lists 0 / 1 (list 0 seeded, list 1
+// empty), [4] launches run (written at the end), [0] / [3] and [8] / [11]
+// due counters of even / odd launches (all zero at the start); front[0]
+// holds list 0's fronts, front[1] no key (0x7F bytes).  trace[b]: due tiles of
+// launch b.
+struct LevelSched {
+  const FieldDesc* descs;
+  const RegionDev* regs;
+  const uint8_t* heavy;
+  int2* list[2];
+  int2* due;
+  int* count;
+  unsigned int* in_set;
+  unsigned long long* tkey;
+  unsigned long long* front[2];
+  unsigned long long* acc;
+  int* trace;
+  int* sweep_max;
+  unsigned long long* span;
+  int* flag;
+  double window;
+  int nfield, ntx, ntz, ntiles, due_cap;
+  int max_sweeps, max_sweeps_heavy, colors, max_launch, trace_cap;
+};
+// blocks of the level kernel's cooperative grid (all resident on the device)
+hipError_t level_grid(int device, int tx, int tz, int f32, int* grid);
+hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st);
 // per field: max |T - snap| over slots both reach, count of slots one
 // reaches, count of slots both reach with |T - snap| > tol
 hipError_t launch_field_compare(const FieldDesc* descs, int nfield, const double* snap, long long nslot, double tol,

@@ -19,6 +19,7 @@
 // (this file is compiled with -ffp-contract=off): an FP32 traveltime field
 // accumulates ~1.5e-4 s of rounding over 400 nodes, above the 1e-4 s bar, and
 // exact FP64 reproduces the reference's tie decisions (gradient refits).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "eik_kernels.h"
@@ -257,10 +258,20 @@ __device__ inline int wave_append(int* counter, bool pred) {
   return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
+// The field's front over the next active set is kept as the tiles enter it
+// (front_next[f]: smallest key of field f's tiles in the set).  It only
+// falls while a launch runs, so a plain read that is not yet current reads
+// a value at or above the true one and never skips a needed atomic.
+__device__ inline void lower_front(unsigned long long* front_next, int f, unsigned long long key) {
+  if (key < gload(front_next + f)) atomicMin(front_next + f, key);
+}
+
 __device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key, unsigned int* in_set,
-                                unsigned long long* tkey, int2* list_next, int* count_next) {
+                                unsigned long long* tkey, int2* list_next, int* count_next,
+                                unsigned long long* front_next) {
   const long long k = (long long)f * ntiles + t;
   atomicMin(tkey + k, key);
+  lower_front(front_next, f, key);
   if (atomicExch(in_set + k, 1u) == 0u) {
     const int pos = atomicAdd(count_next, 1);
     list_next[pos] = make_int2(f, t);
@@ -271,7 +282,8 @@ __device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key
 // of a start-interface node that carries a time.
 __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
                                   int nfield, int ntx, int ntz, int tx, int tz, unsigned int* in_set,
-                                  unsigned long long* tkey, int2* list_next, int* count_next) {
+                                  unsigned long long* tkey, int2* list_next, int* count_next,
+                                  unsigned long long* front_next) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const RegionDev& R0 = regs[0];
   const int nx = R0.nx, nz = R0.nz;
@@ -288,7 +300,8 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
     for (int i = d.ix0; i <= d.ix0 + 1; i++)
       for (int j = d.iz0; j <= d.iz0 + 1; j++)
         if (i >= 0 && i < nx && j >= 0 && j < nz)
-          add_tile(f, (j / tz) * ntx + i / tx, ntiles, key_of(d.t0), in_set, tkey, list_next, count_next);
+          add_tile(f, (j / tz) * ntx + i / tx, ntiles, key_of(d.t0), in_set, tkey, list_next, count_next,
+                   front_next);
     return;
   }
   if (!(d.isT[c] < 0.5e20)) return;
@@ -299,7 +312,7 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
   for (int i = max(0, c - 1); i <= min(nx - 1, c + 1); i++)
     for (int j = j0; j <= j1; j++)
       if ((j % tz) == 0 || j == j0)
-        add_tile(f, (j / tz) * ntx + i / tx, ntiles, key, in_set, tkey, list_next, count_next);
+        add_tile(f, (j / tz) * ntx + i / tx, ntiles, key, in_set, tkey, list_next, count_next, front_next);
 }
 
 // Due tiles of a launch: count_due[0] tiles cut by an interface (placed
@@ -307,11 +320,12 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
 // end), so the slow tiles start first.
 __device__ inline int due_count(const int* count_due) { return count_due[0] + count_due[3]; }
 
-// Clears the counters of a launch: due tiles and the next active list.
+// Clears the counters of a launch: due tiles and the next active list, and
+// the fronts of the next list (its tiles lower them as they enter).
 // The previous launch's due count goes to its trace slot (when given).
 // launch_no counts the launches of a level (the kernels of a launch read it,
 // so a batch of launches has the same arguments every time it is replayed).
-__global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned long long* front, int nfield,
+__global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned long long* front_next, int nfield,
                                     int* launch_no, int* trace, int trace_cap) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f == 0) {
@@ -322,40 +336,32 @@ __global__ void launch_reset_kernel(int* count_due, int* count_next, unsigned lo
     count_due[3] = 0;
     *count_next = 0;
   }
-  if (f < nfield) front[f] = KEY_INF;
-}
-
-// earliest pending key per field over the active list (count on the device)
-__global__ void front_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntiles,
-                             const unsigned long long* __restrict__ tkey, unsigned long long* front) {
-  const int n = *count;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int2 e = list[k];
-    atomicMin(front + e.x, tkey[(long long)e.x * ntiles + e.y]);
-  }
+  if (f < nfield) front_next[f] = KEY_INF;
 }
 
 // Splits the active set into tiles due this launch and tiles carried over.
 // Tiles of one checkerboard colour run per launch, so two edge-adjacent
 // tiles never iterate against each other's stale halos (which can lock a
-// pair into a two-cycle of the non-monotone second-order switch).
-__global__ void split_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntx, int ntiles,
-                             unsigned long long* tkey, const unsigned long long* __restrict__ front, double window,
-                             int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
-                             int* count_carry, const FieldDesc* __restrict__ descs,
-                             const uint8_t* __restrict__ heavy, int due_cap) {
-  const int n = *count;
-  const int stride = gridDim.x * blockDim.x;
-  // every lane of a wave runs the same number of rounds (wave_append)
+// pair into a two-cycle of the non-monotone second-order switch).  Threads
+// tid, tid + stride, ... take the n entries of the list; every thread of
+// the grid calls it (wave_append).
+__device__ inline void split_pass(const int2* list, int n, int ntx, int ntiles, unsigned long long* tkey,
+                                  const unsigned long long* front, unsigned long long* front_next, double window,
+                                  int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
+                                  int* count_carry, const FieldDesc* __restrict__ descs,
+                                  const uint8_t* __restrict__ heavy, int due_cap, int tid, int stride) {
+  // every lane of a wave runs the same number of rounds
   const int rounds = (n + stride - 1) / stride;
   for (int r = 0; r < rounds; r++) {
-    const int k = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = r * stride + tid;
     bool is_due = false, is_carry = false;
     int2 e = make_int2(0, 0);
+    unsigned long long kb = KEY_INF;
     if (k < n) {
       e = list[k];
       const long long q = (long long)e.x * ntiles + e.y;
-      const double key = __longlong_as_double((long long)tkey[q]);
+      kb = tkey[q];
+      const double key = __longlong_as_double((long long)kb);
       const double fr = __longlong_as_double((long long)front[e.x]);
       const int tc = ((e.y % ntx) + (e.y / ntx)) & 1;
       is_due = key <= fr + window && tc == color;
@@ -368,8 +374,20 @@ __global__ void split_kernel(const int2* __restrict__ list, const int* __restric
     const int pc = wave_append(count_carry, is_carry);
     if (hv) due[ph] = e;
     else if (is_due) due[due_cap - 1 - pl] = e;
-    if (is_carry) carry[pc] = e;
+    if (is_carry) {
+      carry[pc] = e;
+      lower_front(front_next, e.x, kb);
+    }
   }
+}
+
+__global__ void split_kernel(const int2* __restrict__ list, const int* __restrict__ count, int ntx, int ntiles,
+                             unsigned long long* tkey, const unsigned long long* __restrict__ front,
+                             unsigned long long* front_next, double window, int color, unsigned int* in_set,
+                             int2* due, int* count_due, int2* carry, int* count_carry,
+                             const FieldDesc* __restrict__ descs, const uint8_t* __restrict__ heavy, int due_cap) {
+  split_pass(list, *count, ntx, ntiles, tkey, front, front_next, window, color, in_set, due, count_due, carry,
+             count_carry, descs, heavy, due_cap, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Tile nodes of the box [a0,a1] x [b0,b1], clipped to the tile.
@@ -409,10 +427,10 @@ __device__ inline double t32_round(double t, int* flag) {
 // One visit of tile ft: stage the window, sweep to the tile's local fixed
 // point, write back, activate the neighbours whose halo changed.
 template <int TX, int TZ, bool F32>
-__device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
+__device__ __attribute__((always_inline)) inline void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs, int2 ft, int ntx,
                          int ntz, unsigned int* in_set, unsigned long long* tkey, int2* list_next, int* count_next,
-                         int max_sweeps, int colors, unsigned long long* acc, int* launch_max, int* flag,
-                         TileLds<TX, TZ>& L) {
+                         unsigned long long* front_next, int max_sweeps, int colors, unsigned long long* acc,
+                         int* launch_max, int* flag, TileLds<TX, TZ>& L) {
   using S = Shape<TX, TZ>;
   constexpr int NPL = S::NPL, WX = S::WX, WZ = S::WZ, W2 = S::W2;
   const int f = ft.x, t = ft.y;
@@ -715,24 +733,20 @@ __device__ void fim_tile(const FieldDesc* __restrict__ descs, const RegionDev* _
     const int ddx = (int)((0x18858u >> (2 * lane)) & 3u) - 1, ddz = (int)((0x1a085u >> (2 * lane)) & 3u) - 1;
     const int nxi = tix + ddx, nzi = tiz + ddz;
     if (nxi >= 0 && nxi < ntx && nzi >= 0 && nzi < ntz)
-      add_tile(f, nzi * ntx + nxi, ntiles, key_of(kmin), in_set, tkey, list_next, count_next);
+      add_tile(f, nzi * ntx + nxi, ntiles, key_of(kmin), in_set, tkey, list_next, count_next, front_next);
   }
 }
 
-// Persistent-style grid: blocks take the due tiles of this launch in turn
-// (the due count is known only on the device).
+// The due tiles of launch lno (nh cut by an interface at the front of the
+// list, the rest from its end): blocks take them in turn.
 template <int TX, int TZ, bool F32>
-__global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
-                                                       const RegionDev* __restrict__ regs, const int2* __restrict__ list,
-                                                       const int* __restrict__ count_due, int ntx, int ntz,
-                                                       unsigned int* in_set, unsigned long long* tkey,
-                                                       int2* list_next, int* count_next, int max_sweeps,
-                                                       int max_sweeps_heavy, int colors, unsigned long long* acc,
-                                                       const int* launch_no, int* sweep_max, int trace_cap,
-                                                       unsigned long long* span, int due_cap, int* flag) {
-  __shared__ TileLds<TX, TZ> L;
+__device__ __attribute__((always_inline)) inline void tiles_pass(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
+                                  const int2* list, int nh, int n, int ntx, int ntz, unsigned int* in_set,
+                                  unsigned long long* tkey, int2* list_next, int* count_next,
+                                  unsigned long long* front_next, int max_sweeps, int max_sweeps_heavy, int colors,
+                                  unsigned long long* acc, int lno, int* sweep_max, int trace_cap,
+                                  unsigned long long* span, int due_cap, int* flag, TileLds<TX, TZ>& L) {
   // most sweeps of a visit in this launch, recorded under the launch number
-  const int lno = *launch_no - 1;
   const bool slot = lno >= 0 && lno < trace_cap;
   int* launch_max = slot ? sweep_max + lno : nullptr;
   // the launch's span on the device's constant-rate clock: block 0 (the
@@ -741,17 +755,84 @@ __global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(c
   // (atomicMax, spread over slots to keep the atomics uncontended)
   unsigned long long* sp = span + (size_t)lno * (SPAN_SLOTS + 1);
   if (slot && blockIdx.x == 0 && threadIdx.x == 0) sp[0] = wall_clock64();
-  const int nh = count_due[0], n = nh + count_due[3];
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
     // a tile cut by an interface gets its own sweep cap: its visits are the
     // slow ones, and a capped visit continues in a later launch
-    fim_tile<TX, TZ, F32>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next,
-                     k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, flag, L);
+    fim_tile<TX, TZ, F32>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, front_next,
+                          k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, flag, L);
     __syncthreads();
   }
   if (slot && threadIdx.x == 0 && ((int)blockIdx.x < n || blockIdx.x == 0))
     atomicMax(sp + 1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
+}
+
+// One launch's tile visits (the due count is known only on the device).
+template <int TX, int TZ, bool F32>
+__global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(const FieldDesc* __restrict__ descs,
+                                                       const RegionDev* __restrict__ regs, const int2* __restrict__ list,
+                                                       const int* __restrict__ count_due, int ntx, int ntz,
+                                                       unsigned int* in_set, unsigned long long* tkey,
+                                                       int2* list_next, int* count_next,
+                                                       unsigned long long* front_next, int max_sweeps,
+                                                       int max_sweeps_heavy, int colors, unsigned long long* acc,
+                                                       const int* launch_no, int* sweep_max, int trace_cap,
+                                                       unsigned long long* span, int due_cap, int* flag) {
+  __shared__ TileLds<TX, TZ> L;
+  const int nh = count_due[0];
+  tiles_pass<TX, TZ, F32>(descs, regs, list, nh, nh + count_due[3], ntx, ntz, in_set, tkey, list_next, count_next,
+                          front_next, max_sweeps, max_sweeps_heavy, colors, acc, *launch_no - 1, sweep_max,
+                          trace_cap, span, due_cap, flag, L);
+}
+
+// Counters the level kernel and the host share (ints): [1] / [2] sizes of
+// active lists 0 / 1, [4] launches run; due tiles of launch b in
+// [8 (b & 1)] (cut by an interface) and [8 (b & 1) + 3] (the others).
+constexpr int LVL_COUNT_DUE = 0, LVL_LAUNCHES = 4;
+
+// An agent-scope read of a counter other blocks of the grid write
+__device__ inline int load_count(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All launches of one dependency level in one dispatch.  Every block of the
+// cooperative grid is resident; the phases of a launch are separated by
+// grid-wide barriers (which order memory at agent scope).  Launch b: split
+// active list b & 1 into the due tiles of colour b & 1 and the tiles carried
+// into list (b & 1) ^ 1 | barrier | the tile visits, which append to list
+// (b & 1) ^ 1; block 0 meanwhile clears what launch b + 1 fills (list b & 1,
+// its front, the other due counters) | barrier | stop when the next list is
+// empty or max_launch launches have run.  Every block reads the same count
+// after the barrier and takes the same number of barriers.
+template <int TX, int TZ, bool F32>
+__global__ void __launch_bounds__(64, 2) fim_level_kernel(LevelSched a) {
+  __shared__ TileLds<TX, TZ> L;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (int b = 0;; b++) {
+    const int cur = b & 1, nxt = cur ^ 1;
+    int* cdue = a.count + 8 * cur;
+    split_pass(a.list[cur], load_count(a.count + 1 + cur), a.ntx, a.ntiles, a.tkey, a.front[cur], a.front[nxt],
+               a.window, cur, a.in_set, a.due, cdue, a.list[nxt], a.count + 1 + nxt, a.descs, a.heavy, a.due_cap,
+               tid, stride);
+    grid.sync();
+    const int nh = load_count(cdue), n = nh + load_count(cdue + 3);
+    if (tid == 0) {
+      if (b < a.trace_cap) a.trace[b] = n;
+      a.count[1 + cur] = 0;
+      a.count[8 * nxt] = 0;
+      a.count[8 * nxt + 3] = 0;
+    }
+    for (int f = tid; f < a.nfield; f += stride) a.front[cur][f] = KEY_INF;
+    tiles_pass<TX, TZ, F32>(a.descs, a.regs, a.due, nh, n, a.ntx, a.ntz, a.in_set, a.tkey, a.list[nxt],
+                            a.count + 1 + nxt, a.front[nxt], a.max_sweeps, a.max_sweeps_heavy, a.colors, a.acc, b,
+                            a.sweep_max, a.trace_cap, a.span, a.due_cap, a.flag, L);
+    grid.sync();
+    if (load_count(a.count + 1 + nxt) == 0 || b + 1 >= a.max_launch) {
+      if (tid == 0) a.count[LVL_LAUNCHES] = b + 1;
+      break;
+    }
+  }
 }
 
 // Post-pass: refit one-sided gradients of grid nodes; everything else keeps
@@ -1254,23 +1335,22 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
 
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
                              int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                             int* count_next, hipStream_t st) {
+                             int* count_next, unsigned long long* front_next, hipStream_t st) {
   const int total = nfield * nx;
   hipLaunchKernelGGL(seed_tiles_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, nfield, ntx, ntz,
-                     tx, tz, in_set, tkey, list_next, count_next);
+                     tx, tz, in_set, tkey, list_next, count_next, front_next);
   return hipGetLastError();
 }
 
 hipError_t launch_schedule(const int2* list, const int* count, int nfield, int ntx, int ntiles,
-                           unsigned long long* tkey, unsigned long long* front, double window, int color,
-                           unsigned int* in_set, int2* due, int* count_due, int2* carry, int* count_carry,
-                           int* launch_no, int* trace, int trace_cap, const FieldDesc* descs, const uint8_t* heavy,
-                           int due_cap, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(launch_reset_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, count_due, count_carry, front,
-                     nfield, launch_no, trace, trace_cap);
-  hipLaunchKernelGGL(front_kernel, dim3(grid), dim3(256), 0, st, list, count, ntiles, tkey, front);
-  hipLaunchKernelGGL(split_kernel, dim3(grid), dim3(256), 0, st, list, count, ntx, ntiles, tkey, front, window, color,
-                     in_set, due, count_due, carry, count_carry, descs, heavy, due_cap);
+                           unsigned long long* tkey, const unsigned long long* front, unsigned long long* front_next,
+                           double window, int color, unsigned int* in_set, int2* due, int* count_due, int2* carry,
+                           int* count_carry, int* launch_no, int* trace, int trace_cap, const FieldDesc* descs,
+                           const uint8_t* heavy, int due_cap, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(launch_reset_kernel, dim3((nfield + 255) / 256), dim3(256), 0, st, count_due, count_carry,
+                     front_next, nfield, launch_no, trace, trace_cap);
+  hipLaunchKernelGGL(split_kernel, dim3(grid), dim3(256), 0, st, list, count, ntx, ntiles, tkey, front, front_next,
+                     window, color, in_set, due, count_due, carry, count_carry, descs, heavy, due_cap);
   return hipGetLastError();
 }
 
@@ -1363,29 +1443,57 @@ hipError_t launch_f32_export(const FieldDesc* descs, int nfield, int nslotg, hip
 
 hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2* list, int grid, const int* count_due,
                       int ntx, int ntz, int tx, int tz, unsigned int* in_set, unsigned long long* tkey,
-                      int2* list_next, int* count_next, int max_sweeps, int max_sweeps_heavy, int colors,
-                      unsigned long long* acc, const int* launch_no, int* sweep_max, int trace_cap,
-                      unsigned long long* span, int due_cap, int f32, int* flag, hipStream_t st) {
+                      int2* list_next, int* count_next, unsigned long long* front_next, int max_sweeps,
+                      int max_sweeps_heavy, int colors, unsigned long long* acc, const int* launch_no, int* sweep_max,
+                      int trace_cap, unsigned long long* span, int due_cap, int f32, int* flag, hipStream_t st) {
+#define FIM_ARGS                                                                                                     \
+  descs, regs, list, count_due, ntx, ntz, in_set, tkey, list_next, count_next, front_next, max_sweeps,               \
+      max_sweeps_heavy, colors, acc, launch_no, sweep_max, trace_cap, span, due_cap, flag
   if (f32) {
     if (tx != 8 || tz != 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, true>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx,
-                       ntz, in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
-                       sweep_max, trace_cap, span, due_cap, flag);
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, true>), dim3(grid), dim3(64), 0, st, FIM_ARGS);
   } else if (tx == 8 && tz == 8)
-    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
-                       sweep_max, trace_cap, span, due_cap, flag);
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 8, false>), dim3(grid), dim3(64), 0, st, FIM_ARGS);
   else if (tx == 16 && tz == 8)
-    hipLaunchKernelGGL((fim_tiles_kernel<16, 8, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
-                       sweep_max, trace_cap, span, due_cap, flag);
+    hipLaunchKernelGGL((fim_tiles_kernel<16, 8, false>), dim3(grid), dim3(64), 0, st, FIM_ARGS);
   else if (tx == 8 && tz == 16)
-    hipLaunchKernelGGL((fim_tiles_kernel<8, 16, false>), dim3(grid), dim3(64), 0, st, descs, regs, list, count_due, ntx, ntz,
-                       in_set, tkey, list_next, count_next, max_sweeps, max_sweeps_heavy, colors, acc, launch_no,
-                       sweep_max, trace_cap, span, due_cap, flag);
+    hipLaunchKernelGGL((fim_tiles_kernel<8, 16, false>), dim3(grid), dim3(64), 0, st, FIM_ARGS);
   else
     return hipErrorInvalidValue;
+#undef FIM_ARGS
   return hipGetLastError();
+}
+
+// the level kernel's instantiation for a tile shape and storage width
+static const void* level_kernel(int tx, int tz, int f32) {
+  if (f32) return tx == 8 && tz == 8 ? (const void*)fim_level_kernel<8, 8, true> : nullptr;
+  if (tx == 8 && tz == 8) return (const void*)fim_level_kernel<8, 8, false>;
+  if (tx == 16 && tz == 8) return (const void*)fim_level_kernel<16, 8, false>;
+  if (tx == 8 && tz == 16) return (const void*)fim_level_kernel<8, 16, false>;
+  return nullptr;
+}
+
+hipError_t level_grid(int device, int tx, int tz, int f32, int* grid) {
+  const void* k = level_kernel(tx, tz, f32);
+  if (!k) return hipErrorInvalidValue;
+  int coop = 0, ncu = 0, per_cu = 0;
+  hipError_t e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+  if (e != hipSuccess) return e;
+  if (!coop) return hipErrorNotSupported;
+  e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0);
+  if (e != hipSuccess) return e;
+  *grid = per_cu * ncu;
+  return *grid > 0 ? hipSuccess : hipErrorInvalidConfiguration;
+}
+
+hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st) {
+  const void* k = level_kernel(tx, tz, f32);
+  if (!k) return hipErrorInvalidValue;
+  LevelSched arg = a;
+  void* args[] = {&arg};
+  return hipLaunchCooperativeKernel(k, dim3(grid), dim3(64), args, 0, st);
 }
 
 hipError_t launch_field_summary(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int nz,

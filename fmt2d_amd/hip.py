@@ -131,6 +131,7 @@ def load() -> C.CDLL:
     lib.fmt2d_set_frechet_chunk.argtypes = [C.c_void_p, C.c_longlong]
     lib.fmt2d_set_heavy_sweeps.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_launch_cap.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_set_solver_launch.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_lsqr_mode.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_field_precision.argtypes = [C.c_void_p, C.c_int]
@@ -169,7 +170,8 @@ def load() -> C.CDLL:
 EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_set_model",
            "fmt2d_grid_info_get", "fmt2d_forward", "fmt2d_forward_stats_get", "fmt2d_forward_trace",
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
-           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_tile",
+           "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_solver_launch",
+           "fmt2d_set_tile",
            "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_set_field_precision", "fmt2d_field_snapshot",
            "fmt2d_field_compare", "fmt2d_set_ray_search", "fmt2d_ray_nearest_calls",
            "fmt2d_ray_nearest_check",
@@ -279,6 +281,16 @@ class Context:
 
     def set_launch_cap(self, max_launches: int = 0):
         _check(self._lib.fmt2d_set_launch_cap(self._h, int(max_launches)))
+
+    def set_solver_launch(self, mode: str = "graph"):
+        """How the tile solver's launches reach the device: "graph" (batches
+        of launches replayed as HIP graphs) or "level" (one cooperative
+        dispatch per stage level, its launches separated by grid-wide
+        barriers).  Both reach the same fields bit for bit."""
+        modes = {"graph": 0, "level": 1}
+        if mode not in modes:
+            raise ValueError(f"solver launch mode {mode!r}: 'graph' or 'level'")
+        _check(self._lib.fmt2d_set_solver_launch(self._h, modes[mode]))
 
     def stats(self) -> ForwardStats:
         st = ForwardStats()

@@ -89,7 +89,8 @@ typedef struct {
   long long sweeps;         /* in-tile sweeps over all tile visits */
   long long capped;         /* tile visits that reached the sweep cap */
   int max_sweeps;           /* most sweeps of one tile visit */
-  int dispatches;           /* tile-kernel dispatches (launches plus empty ones that close a batch) */
+  int dispatches;           /* tile-kernel dispatches: launches plus empty ones that close a batch
+                               (FMT2D_LAUNCH_GRAPH), one per stage level (FMT2D_LAUNCH_LEVEL) */
 } fmt2d_forward_stats;
 
 /* Context on HIP device `device` (owns all device memory). */
@@ -166,6 +167,18 @@ int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int
  * from clean tile state.  (Solver control of this build, no counterpart in
  * the reference's heap-ordered march.) */
 int fmt2d_set_launch_cap(fmt2d_ctx* ctx, int max_launches);
+/* How the tile solver's launches reach the device.  FMT2D_LAUNCH_GRAPH (the
+ * default): each launch is a split and a tile dispatch; batches of 8
+ * launches are captured once per stage level in a HIP graph and replayed
+ * until the active set is empty.  FMT2D_LAUNCH_LEVEL: one cooperative
+ * dispatch per stage level runs all its launches, as phases separated by
+ * grid-wide barriers (FMT2D_EHIP where the device has no cooperative
+ * launch).  Both run the same schedule (the same split of the active set
+ * at each launch) and reach the same fields bit for bit.  (Solver control
+ * of this build, no counterpart in the reference.) */
+#define FMT2D_LAUNCH_GRAPH 0
+#define FMT2D_LAUNCH_LEVEL 1
+int fmt2d_set_solver_launch(fmt2d_ctx* ctx, int mode);
 /* Due-tile count and the most sweeps of one tile visit, per solver launch of
  * the last forward (diagnostics); either array may be NULL. */
 int fmt2d_forward_trace(fmt2d_ctx* ctx, int* counts, int* max_sweeps, int max, int* n);

@@ -116,6 +116,42 @@ def test_schedule_does_not_change_the_fields(fields, tuning):
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
 
 
+def test_level_launch_matches_the_oracle(fields):
+    """The level launch mode (one cooperative dispatch per stage level, the
+    solver's launches as phases between grid-wide barriers) on the plain
+    case: every field bit for bit the default mode's, which the tests above
+    hold to the oracle, and traveltimes of the first source bit for bit the
+    oracle's.  Its launch cap gives the same error as the default mode's."""
+    from fmt2d_amd import hip
+    ctx, om = fields
+    before = [np.concatenate(ctx.field(i, f)).view(np.uint64).copy() for i in range(len(SOURCES)) for f in range(1, 13)]
+    try:
+        ctx.set_solver_launch("level")
+        ctx.forward(SOURCES)
+        st = ctx.stats()
+        print(f"level launch: {st.launches} launches in {st.dispatches} dispatches, {st.ms_total:.1f} ms")
+        assert st.dispatches == 4
+        after = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
+        assert all(np.array_equal(a, b) for a, b in zip(before, after))
+        ref = om.forward_source(SOURCES[0], 12, headwave=True)
+        for itf in range(1, 13):
+            T = ctx.field(0, itf)[0]
+            fin = ref[itf - 1][0] < 1e19
+            assert np.array_equal(T[fin].view(np.int64), ref[itf - 1][0][fin].view(np.int64)), f"tf{itf}"
+        ctx.set_launch_cap(3)
+        with pytest.raises(hip.Fmt2dError, match="did not converge"):
+            ctx.forward(SOURCES)
+        ctx.set_launch_cap(0)
+        ctx.forward(SOURCES)
+        again = [np.concatenate(ctx.field(i, f)).view(np.uint64) for i in range(len(SOURCES)) for f in range(1, 13)]
+        assert all(np.array_equal(a, b) for a, b in zip(before, again))
+    finally:
+        ctx.set_launch_cap(0)
+        ctx.set_solver_launch("graph")
+    with pytest.raises(ValueError):
+        ctx.set_solver_launch("persistent")
+
+
 def test_autotune_keeps_the_fields(fields):
     """fmt2d_autotune times the candidate schedules on this model, keeps the
     fastest and leaves the context solved: the fields are the default
