@@ -13,7 +13,9 @@ in one fmt2d_forward.  ``multistage`` (60 OBS every 3.3 km) is reported
 beside it, and ``per_source``: the same 200-OBS forward solved one OBS at a
 time (the per-source stage loop of the reference, on the GPU), timed on four
 OBS across the model and scaled to all 200, with the batched step's speedup
-over it.
+over it.  ``level_launch``: the same forward with the tile solver's launches
+run by one cooperative dispatch per stage level instead of replayed graphs
+(its time, and a check that every field has the same bits).
 
 Second figure (``iteration``): seconds per inversion iteration = forward +
 rays + Frechet rows + system assembly + one LSQR solve (fixed damping, update
@@ -590,7 +592,7 @@ def fp32_figure(steps: int = 2):
 
 
 def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int, autotune: bool = False,
-                   per_source: int = 0):
+                   per_source: int = 0, level_steps: int = 0):
     """Source-stages per second of one workload's forward (the headline's
     measurement; see the module docstring), with the tile-kernel time (the
     dispatches' spans, stamped on the device clock inside the replayed
@@ -600,7 +602,11 @@ def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int,
     the timed steps); every schedule gives the same fields bit for bit.
     per_source > 0: the same forward solved one OBS at a time (the per-stage
     path of the reference's loop, each OBS's stage chain on its own) for
-    that many OBS spread over the model, after the timed steps."""
+    that many OBS spread over the model, after the timed steps.
+    level_steps > 0: that many more forwards in the level launch mode (one
+    cooperative dispatch per stage level runs all its solver launches),
+    timed the same way, with their fields checked against the graph
+    mode's (hash of every field)."""
     from fmt2d_amd import hip, formats
     m, g, desc = workload(workload_name)
     with tempfile.TemporaryDirectory() as d:
@@ -644,10 +650,28 @@ def forward_figure(workload_name: str, steps: int, warmup: int, cpu_sample: int,
             per_obs = float(np.mean(sw))
             single = {"obs_timed": [obs[k] for k in pick], "seconds_per_obs": round(per_obs, 4),
                       "launches_per_obs": int(np.mean(sl)), "seconds_all_obs_estimate": round(per_obs * len(obs), 2)}
+        level = None
+        if level_steps > 0:
+            ctx.forward(obs)
+            ref_hash = ctx.summary()["hash"].copy()
+            try:
+                ctx.set_solver_launch("level")
+                ctx.forward(obs)
+                lw, ld = [], []
+                for _ in range(level_steps):
+                    t0 = time.perf_counter()
+                    ctx.forward(obs)
+                    lw.append(time.perf_counter() - t0)
+                    ld.append(ctx.stats().dispatches)
+                same = bool(np.array_equal(ctx.summary()["hash"], ref_hash))
+            finally:
+                ctx.set_solver_launch("graph")
+            level = {"ms_per_step": round(1e3 * float(np.mean(lw)), 1), "solver_dispatches_per_step": int(np.mean(ld)),
+                     "fields_same_bits_as_graph_mode": same}
         cpu = cpu_baseline(d, obs, cpu_sample) if cpu_sample > 0 else None
         ctx.close()
     return dict(desc=desc, obs=obs, walls=walls, fims=fims, nodes=nodes, launches=launches, tiles=tiles, disp=disp,
-                cpu=cpu, schedule=sched, single=single)
+                cpu=cpu, schedule=sched, single=single, level=level)
 
 
 def main():
@@ -665,6 +689,7 @@ def main():
     ap.add_argument("--no-gradient", action="store_true", help="skip the misfit-gradient figure")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 field-storage figure")
     ap.add_argument("--no-per-source", action="store_true", help="skip the one-OBS-at-a-time forward figure")
+    ap.add_argument("--no-level", action="store_true", help="skip the level-launch-mode forward figure")
     ap.add_argument("--joint", type=int, default=4,
                     help="iterations of the joint inversion on the 0.5 km / 1 km mesh (2001x301, 200 OBS, five "
                          "phases; about a minute for 4); 0 skips it")
@@ -699,7 +724,7 @@ def main():
         return
 
     head = forward_figure(args.workload, args.steps, args.warmup, args.cpu_sample, autotune=not args.no_autotune,
-                          per_source=0 if args.no_per_source else 4)
+                          per_source=0 if args.no_per_source else 4, level_steps=0 if args.no_level else 2)
     print(f"headline forward done: {head['walls']}", file=sys.stderr, flush=True)
     ms = None
     if args.workload != "multistage" and not args.no_multistage:
@@ -756,6 +781,7 @@ def main():
                                           f"read, T and grad T written) = {alg_bytes} B per step, "
                                           f"{alg_bytes / ndisp:.0f} B per dispatch"},
         "cpu_baseline": head["cpu"],
+        "level_launch": head["level"],
         "per_source": None if head["single"] is None else dict(
             head["single"], workload="the same forward solved one OBS at a time (each OBS's twelve stage fields "
                                      "on their own, as the reference's per-source loop runs them), timed on the "

@@ -397,7 +397,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
   HIPCHK(hipMemsetAsync(ctx->tin.p, 0, ntt * sizeof(unsigned int), st));
   HIPCHK(hipMemsetAsync(ctx->tkey.p, 0x7F, ntt * sizeof(unsigned long long), st));
   if (ctx->tfront.n < 2 * (size_t)maxlev) HIPCHK(ctx->tfront.alloc(2 * (size_t)maxlev));
-  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(12));
+  if (!ctx->tcount.p) HIPCHK(ctx->tcount.alloc(16));
   HIPCHK(launch_fill(ctx->descs.p, nfield, (int)nslot, st));
   if (!ctx->stop_flag.p) HIPCHK(ctx->stop_flag.alloc(1));
   HIPCHK(hipMemsetAsync(ctx->stop_flag.p, 0, sizeof(int), st));
@@ -417,8 +417,9 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
                                      program ? ctx->isecs_prog.p : ctx->isecs.p, st));
     // counters: [0] / [3] due tiles of a launch cut by an interface / not,
     // [1] / [2] sizes of active lists 0 / 1, [4] launches of the level,
-    // [8] / [11] the due counters of odd launches in the level kernel
-    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 12, st));
+    // [8] / [11] the due counters of odd launches in the level kernel,
+    // [12] / [13] its tile hand-out counters
+    HIPCHK(hipMemsetAsync(ctx->tcount.p, 0, sizeof(int) * 16, st));
     // fronts of lists 0 and 1: no key (seeding lowers list 0's)
     unsigned long long* front[2] = {ctx->tfront.p, ctx->tfront.p + nf};
     HIPCHK(hipMemsetAsync(ctx->tfront.p, 0x7F, sizeof(unsigned long long) * 2 * nf, st));
@@ -463,13 +464,16 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
         if (e == hipErrorNotSupported) return fail(FMT2D_EHIP, "the device does not run cooperative launches");
         HIPCHK(e);
       }
+      const size_t nbar = level_bar_words(ctx->level_grid[gi]);
+      if (ctx->tbar.n < nbar) HIPCHK(ctx->tbar.alloc(nbar));
+      HIPCHK(hipMemsetAsync(ctx->tbar.p, 0, sizeof(unsigned) * nbar, st));
       LevelSched a{};
       a.descs = dl; a.regs = ctx->regs.p; a.heavy = ctx->theavy.p;
       a.list[0] = ctx->tlist[0].p; a.list[1] = ctx->tlist[1].p; a.due = ctx->tlist[2].p;
       a.count = ctx->tcount.p; a.in_set = ctx->tin.p; a.tkey = ctx->tkey.p;
       a.front[0] = front[0]; a.front[1] = front[1];
       a.acc = ctx->acc.p; a.trace = ctx->ttrace.p; a.sweep_max = ctx->tmax.p; a.span = ctx->tspan.p;
-      a.flag = ctx->stop_flag.p; a.window = window;
+      a.flag = ctx->stop_flag.p; a.bar = ctx->tbar.p; a.window = window;
       a.nfield = nf; a.ntx = ntx; a.ntz = ntz; a.ntiles = ntiles; a.due_cap = (int)ntt;
       a.max_sweeps = ctx->max_sweeps; a.max_sweeps_heavy = ctx->max_sweeps_heavy; a.colors = ctx->colors;
       a.max_launch = max_launch; a.trace_cap = trace_cap;

@@ -134,6 +134,7 @@ struct fmt2d_ctx {
   int launch_cap = 0;                 // solver launches per stage level before FMT2D_ENOCONV (0: by grid size)
   int solver_launch = 0;              // FMT2D_LAUNCH_GRAPH or FMT2D_LAUNCH_LEVEL
   int level_grid[4] = {0, 0, 0, 0};   // cooperative grid of the level kernel (8x8, 16x8, 8x16, 8x8 FP32)
+  DBuf<unsigned> tbar;                // its grid barrier's counters
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
   int lsqr_fast = 0;                  // LSQR reductions: 0 in the reference's order, 1 tree / wave reductions
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]

@@ -91,9 +91,11 @@ hipError_t launch_fim(const FieldDesc* descs, const RegionDev* regs, const int2*
                       int max_sweeps_heavy, int colors, unsigned long long* acc, const int* launch_no, int* sweep_max,
                       int trace_cap, unsigned long long* span, int due_cap, int f32, int* flag, hipStream_t st);
 // All launches of a dependency level in one cooperative dispatch.  count
-// (12 ints): [1] / [2] sizes of active lists 0 / 1 (list 0 seeded, list 1
+// (16 ints): [1] / [2] sizes of active lists 0 / 1 (list 0 seeded, list 1
 // empty), [4] launches run (written at the end), [0] / [3] and [8] / [11]
-// due counters of even / odd launches (all zero at the start); front[0]
+// due counters and [12] / [13] tile hand-out counters of even / odd
+// launches (all zero at the start); bar: level_bar_words(grid) zeros (the
+// grid barrier's counters); front[0]
 // holds list 0's fronts, front[1] no key (0x7F bytes).  trace[b]: due tiles of
 // launch b.
 struct LevelSched {
@@ -111,12 +113,14 @@ struct LevelSched {
   int* sweep_max;
   unsigned long long* span;
   int* flag;
+  unsigned* bar;
   double window;
   int nfield, ntx, ntz, ntiles, due_cap;
   int max_sweeps, max_sweeps_heavy, colors, max_launch, trace_cap;
 };
 // blocks of the level kernel's cooperative grid (all resident on the device)
 hipError_t level_grid(int device, int tx, int tz, int f32, int* grid);
+size_t level_bar_words(int grid);
 hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st);
 // per field: max |T - snap| over slots both reach, count of slots one
 // reaches, count of slots both reach with |T - snap| > tol

@@ -19,7 +19,6 @@
 // (this file is compiled with -ffp-contract=off): an FP32 traveltime field
 // accumulates ~1.5e-4 s of rounding over 400 nodes, above the 1e-4 s bar, and
 // exact FP64 reproduces the reference's tie decisions (gradient refits).
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "eik_kernels.h"
@@ -745,7 +744,7 @@ __device__ __attribute__((always_inline)) inline void tiles_pass(const FieldDesc
                                   unsigned long long* tkey, int2* list_next, int* count_next,
                                   unsigned long long* front_next, int max_sweeps, int max_sweeps_heavy, int colors,
                                   unsigned long long* acc, int lno, int* sweep_max, int trace_cap,
-                                  unsigned long long* span, int due_cap, int* flag, TileLds<TX, TZ>& L) {
+                                  unsigned long long* span, int due_cap, int* flag, int* grab, TileLds<TX, TZ>& L) {
   // most sweeps of a visit in this launch, recorded under the launch number
   const bool slot = lno >= 0 && lno < trace_cap;
   int* launch_max = slot ? sweep_max + lno : nullptr;
@@ -755,13 +754,24 @@ __device__ __attribute__((always_inline)) inline void tiles_pass(const FieldDesc
   // (atomicMax, spread over slots to keep the atomics uncontended)
   unsigned long long* sp = span + (size_t)lno * (SPAN_SLOTS + 1);
   if (slot && blockIdx.x == 0 && threadIdx.x == 0) sp[0] = wall_clock64();
-  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+  // tiles blockIdx.x, then (with grab, one-wave blocks) the next untaken
+  // one from a shared counter, asked for once the visit is done (asking
+  // before it would hold the next tile behind a slow visit), else every
+  // gridDim.x-th.  A block stops when handed a number past the due tiles,
+  // which comes only after every due tile was handed out.
+  for (int k = blockIdx.x; k < n;) {
     const int2 ft = k < nh ? list[k] : list[due_cap - 1 - (k - nh)];
     // a tile cut by an interface gets its own sweep cap: its visits are the
     // slow ones, and a capped visit continues in a later launch
     fim_tile<TX, TZ, F32>(descs, regs, ft, ntx, ntz, in_set, tkey, list_next, count_next, front_next,
                           k < nh ? max_sweeps_heavy : max_sweeps, colors, acc, launch_max, flag, L);
     __syncthreads();
+    if (grab) {
+      int nk = 0;
+      if (threadIdx.x == 0) nk = (int)gridDim.x + atomicAdd(grab, 1);
+      k = __shfl(nk, 0, 64);
+    } else
+      k += gridDim.x;
   }
   if (slot && threadIdx.x == 0 && ((int)blockIdx.x < n || blockIdx.x == 0))
     atomicMax(sp + 1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
@@ -782,52 +792,83 @@ __global__ void __launch_bounds__(64, (TX * TZ > 64) ? 2 : 1) fim_tiles_kernel(c
   const int nh = count_due[0];
   tiles_pass<TX, TZ, F32>(descs, regs, list, nh, nh + count_due[3], ntx, ntz, in_set, tkey, list_next, count_next,
                           front_next, max_sweeps, max_sweeps_heavy, colors, acc, *launch_no - 1, sweep_max,
-                          trace_cap, span, due_cap, flag, L);
+                          trace_cap, span, due_cap, flag, nullptr, L);
 }
 
 // Counters the level kernel and the host share (ints): [1] / [2] sizes of
 // active lists 0 / 1, [4] launches run; due tiles of launch b in
-// [8 (b & 1)] (cut by an interface) and [8 (b & 1) + 3] (the others).
-constexpr int LVL_COUNT_DUE = 0, LVL_LAUNCHES = 4;
+// [8 (b & 1)] (cut by an interface) and [8 (b & 1) + 3] (the others); tiles
+// handed out past the first gridDim.x in [12 + (b & 1)].
+constexpr int LVL_LAUNCHES = 4, LVL_GRAB = 12;
 
 // An agent-scope read of a counter other blocks of the grid write
 __device__ inline int load_count(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Grid-wide barrier of the level kernel (one-wave blocks, all resident).
+// Blocks arrive on a counter of their group of LVL_BAR_GROUP blocks (each
+// counter on its own 256-byte line); the last of a group arrives on the top
+// counter, which every block polls.  Counters only grow: barrier number
+// `epoch` (from 1) is complete when the top counter reaches epoch x groups.
+// A release fence before the arrival and an acquire fence after the wait
+// order the blocks' memory around the barrier at agent scope; the last
+// block of a group fences (acquire, release) between its two arrivals, so
+// that the top arrival carries every block of the group.
+constexpr int LVL_BAR_GROUP = 32, LVL_BAR_STRIDE = 64;
+__device__ inline void level_barrier(unsigned* bar, unsigned epoch) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (threadIdx.x == 0) {
+    const unsigned g = blockIdx.x / LVL_BAR_GROUP;
+    const unsigned gsize = min((unsigned)LVL_BAR_GROUP, gridDim.x - g * LVL_BAR_GROUP);
+    const unsigned ngroups = (gridDim.x + LVL_BAR_GROUP - 1) / LVL_BAR_GROUP;
+    const unsigned old = __hip_atomic_fetch_add(bar + LVL_BAR_STRIDE * (1 + g), 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    if (old == epoch * gsize - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch * ngroups)
+      __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // All launches of one dependency level in one dispatch.  Every block of the
 // cooperative grid is resident; the phases of a launch are separated by
-// grid-wide barriers (which order memory at agent scope).  Launch b: split
-// active list b & 1 into the due tiles of colour b & 1 and the tiles carried
-// into list (b & 1) ^ 1 | barrier | the tile visits, which append to list
-// (b & 1) ^ 1; block 0 meanwhile clears what launch b + 1 fills (list b & 1,
-// its front, the other due counters) | barrier | stop when the next list is
-// empty or max_launch launches have run.  Every block reads the same count
-// after the barrier and takes the same number of barriers.
+// grid-wide barriers.  Launch b: split active list b & 1 into the due tiles
+// of colour b & 1 and the tiles carried into list (b & 1) ^ 1 | barrier |
+// the tile visits, which append to list (b & 1) ^ 1; block 0 meanwhile
+// clears what launch b + 1 fills (list b & 1, its front, the other due and
+// hand-out counters) | barrier | stop when the next list is empty or
+// max_launch launches have run.  Every block reads the same count after the
+// barrier and takes the same number of barriers.
 template <int TX, int TZ, bool F32>
 __global__ void __launch_bounds__(64, 2) fim_level_kernel(LevelSched a) {
   __shared__ TileLds<TX, TZ> L;
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const int tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  unsigned epoch = 0;
   for (int b = 0;; b++) {
     const int cur = b & 1, nxt = cur ^ 1;
     int* cdue = a.count + 8 * cur;
     split_pass(a.list[cur], load_count(a.count + 1 + cur), a.ntx, a.ntiles, a.tkey, a.front[cur], a.front[nxt],
                a.window, cur, a.in_set, a.due, cdue, a.list[nxt], a.count + 1 + nxt, a.descs, a.heavy, a.due_cap,
                tid, stride);
-    grid.sync();
+    level_barrier(a.bar, ++epoch);
     const int nh = load_count(cdue), n = nh + load_count(cdue + 3);
     if (tid == 0) {
       if (b < a.trace_cap) a.trace[b] = n;
       a.count[1 + cur] = 0;
       a.count[8 * nxt] = 0;
       a.count[8 * nxt + 3] = 0;
+      a.count[LVL_GRAB + nxt] = 0;
     }
     for (int f = tid; f < a.nfield; f += stride) a.front[cur][f] = KEY_INF;
     tiles_pass<TX, TZ, F32>(a.descs, a.regs, a.due, nh, n, a.ntx, a.ntz, a.in_set, a.tkey, a.list[nxt],
                             a.count + 1 + nxt, a.front[nxt], a.max_sweeps, a.max_sweeps_heavy, a.colors, a.acc, b,
-                            a.sweep_max, a.trace_cap, a.span, a.due_cap, a.flag, L);
-    grid.sync();
+                            a.sweep_max, a.trace_cap, a.span, a.due_cap, a.flag, a.count + LVL_GRAB + cur, L);
+    level_barrier(a.bar, ++epoch);
     if (load_count(a.count + 1 + nxt) == 0 || b + 1 >= a.max_launch) {
       if (tid == 0) a.count[LVL_LAUNCHES] = b + 1;
       break;
@@ -1486,6 +1527,10 @@ hipError_t level_grid(int device, int tx, int tz, int f32, int* grid) {
   if (e != hipSuccess) return e;
   *grid = per_cu * ncu;
   return *grid > 0 ? hipSuccess : hipErrorInvalidConfiguration;
+}
+
+size_t level_bar_words(int grid) {
+  return (size_t)LVL_BAR_STRIDE * (1 + (grid + LVL_BAR_GROUP - 1) / LVL_BAR_GROUP);
 }
 
 hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st) {
