@@ -28,6 +28,14 @@ __device__ inline int ifloor(double v) {
 }
 __device__ inline double clamp01(double w) { return w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w); }
 
+// Two neighbouring entries of a node array in one load (the cell's corners
+// along x): every lane of a wave gathers from its own ray's cell, so fewer,
+// wider loads mean fewer cache requests per interpolation.
+struct __attribute__((aligned(8))) Pair64 { double lo, hi; };
+struct __attribute__((aligned(2))) Pair16 { uint16_t lo, hi; };
+__device__ inline Pair64 load_pair(const double* p) { return *(const Pair64*)p; }
+__device__ inline Pair16 load_pair(const uint16_t* p) { return *(const Pair16*)p; }
+
 // ---------------------------------------------------------------- region lookups
 
 // region_rnode_id(ix,iz) (1-based): field slot of the grid node, -1 outside
@@ -160,11 +168,13 @@ __device__ __attribute__((always_inline)) inline void interp_grad(const RayGeomD
   // the cell's corners lie in the grid (ix < nx, iz < nz): their gradients
   // are loaded together with their region flags, and a corner outside the
   // region takes the nearby node's instead
-  const int q1 = (iz - 1) * G.nx + (ix - 1), q2 = q1 + 1, q3 = q1 + G.nx, q4 = q3 + 1;
-  double g1x = F.GX[q1], g1z = F.GZ[q1], g2x = F.GX[q2], g2z = F.GZ[q2];
-  double g3x = F.GX[q3], g3z = F.GZ[q3], g4x = F.GX[q4], g4z = F.GZ[q4];
-  const bool in1 = R.flags[q1] & RF_IN, in2 = R.flags[q2] & RF_IN, in3 = R.flags[q3] & RF_IN,
-             in4 = R.flags[q4] & RF_IN;
+  const int q1 = (iz - 1) * G.nx + (ix - 1), q3 = q1 + G.nx;
+  const Pair64 x12 = load_pair(F.GX + q1), z12 = load_pair(F.GZ + q1);
+  const Pair64 x34 = load_pair(F.GX + q3), z34 = load_pair(F.GZ + q3);
+  const Pair16 f12 = load_pair(R.flags + q1), f34 = load_pair(R.flags + q3);
+  double g1x = x12.lo, g1z = z12.lo, g2x = x12.hi, g2z = z12.hi;
+  double g3x = x34.lo, g3z = z34.lo, g4x = x34.hi, g4z = z34.hi;
+  const bool in1 = f12.lo & RF_IN, in2 = f12.hi & RF_IN, in3 = f34.lo & RF_IN, in4 = f34.hi & RF_IN;
   if (!in1) { const int m = rnode_nearby(G, R, ix, iz); g1x = m >= 0 ? F.GX[m] : 0.0; g1z = m >= 0 ? F.GZ[m] : 0.0; }
   if (!in2) { const int m = rnode_nearby(G, R, ix + 1, iz); g2x = m >= 0 ? F.GX[m] : 0.0; g2z = m >= 0 ? F.GZ[m] : 0.0; }
   if (!in3) { const int m = rnode_nearby(G, R, ix, iz + 1); g3x = m >= 0 ? F.GX[m] : 0.0; g3z = m >= 0 ? F.GZ[m] : 0.0; }
@@ -193,10 +203,11 @@ __device__ __attribute__((always_inline)) inline double interp_time(const RayGeo
   if (iz >= G.nz) iz = G.nz - 1;
   const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
   // corners in the grid: times loaded together with the region flags
-  const int q1 = (iz - 1) * G.nx + (ix - 1), q2 = q1 + 1, q3 = q1 + G.nx, q4 = q3 + 1;
-  double t1 = F.T[q1], t2 = F.T[q2], t3 = F.T[q3], t4 = F.T[q4];
-  const bool in1 = R.flags[q1] & RF_IN, in2 = R.flags[q2] & RF_IN, in3 = R.flags[q3] & RF_IN,
-             in4 = R.flags[q4] & RF_IN;
+  const int q1 = (iz - 1) * G.nx + (ix - 1), q3 = q1 + G.nx;
+  const Pair64 t12 = load_pair(F.T + q1), t34 = load_pair(F.T + q3);
+  const Pair16 f12 = load_pair(R.flags + q1), f34 = load_pair(R.flags + q3);
+  double t1 = t12.lo, t2 = t12.hi, t3 = t34.lo, t4 = t34.hi;
+  const bool in1 = f12.lo & RF_IN, in2 = f12.hi & RF_IN, in3 = f34.lo & RF_IN, in4 = f34.hi & RF_IN;
   if (!in1) { const int m = rnode_nearby(G, R, ix, iz); t1 = m >= 0 ? F.T[m] : RAY_HUGE; }
   if (!in2) { const int m = rnode_nearby(G, R, ix + 1, iz); t2 = m >= 0 ? F.T[m] : RAY_HUGE; }
   if (!in3) { const int m = rnode_nearby(G, R, ix, iz + 1); t3 = m >= 0 ? F.T[m] : RAY_HUGE; }
@@ -210,7 +221,7 @@ __device__ __attribute__((always_inline)) inline double interp_time(const RayGeo
 }
 
 // interpolate_interface2d (rays2d.f90:1228-1262) on intersection k
-__device__ double iface_z(const RayGeomDev& G, int k, double px) {
+__device__ __attribute__((always_inline)) inline double iface_z(const RayGeomDev& G, int k, double px) {
   const int n = G.isn[k];
   const double* x = G.isx[k];
   const double* z = G.isz[k];
@@ -223,8 +234,33 @@ __device__ double iface_z(const RayGeomDev& G, int k, double px) {
   return (1.0 - w) * z[ix - 1] + w * z[ix];
 }
 
+// One intersection's node arrays, read once per ray section: iface_z on it
+// is the same function
+struct IfaceRef {
+  const double* x;
+  const double* z;
+  int n;
+  double x0;
+};
+__device__ __attribute__((always_inline)) inline IfaceRef iface_ref(const RayGeomDev& G, int k) {
+  IfaceRef f;
+  f.x = G.isx[k]; f.z = G.isz[k]; f.n = G.isn[k];
+  f.x0 = f.n >= 1 ? f.x[0] : 0.0;
+  return f;
+}
+__device__ __attribute__((always_inline)) inline double iface_z(const IfaceRef& f, double dx, double px) {
+  if (f.n < 1) return 0.0;
+  if (f.n < 2) return f.z[0];
+  const int ix = ifloor((px - f.x0) / dx) + 1;
+  if (ix < 1) return f.z[0];
+  if (ix >= f.n) return f.z[f.n - 1];
+  const double w = clamp01((px - f.x[ix - 1]) / dx);
+  const Pair64 zz = load_pair(f.z + ix - 1);
+  return (1.0 - w) * zz.lo + w * zz.hi;
+}
+
 // interface_normal2d (rays2d.f90:1266-1308)
-__device__ void iface_normal(const RayGeomDev& G, int k, double px, double& nx_, double& nz_, double& h) {
+__device__ __attribute__((always_inline)) inline void iface_normal(const RayGeomDev& G, int k, double px, double& nx_, double& nz_, double& h) {
   const int n = G.isn[k];
   const double* x = G.isx[k];
   const double* z = G.isz[k];
@@ -311,7 +347,7 @@ struct SecInfo {
 };
 
 // find_interface_crossing2d (rays2d.f90:764-821); false = no crossing
-__device__ bool interface_crossing(const RayGeomDev& G, double2* pt, int n, const double pp[2], int ifend,
+__device__ __attribute__((always_inline)) inline bool interface_crossing(const RayGeomDev& G, double2* pt, int n, const double pp[2], int ifend,
                                    const double grad0[2], double dl) {
   const double tol = G.dx < G.dz ? G.dx * 0.01 : G.dz * 0.01;  // grid%tolerance
   const double zi = iface_z(G, ifend, pp[0]);
@@ -355,246 +391,324 @@ __device__ __attribute__((always_inline)) inline bool refracts(const RayGeomDev&
   return sqrt(a * a + b * b) <= sqrt(grad0[0] * grad0[0] + grad0[1] * grad0[1]);
 }
 
-// trace_ray_section2d (rays2d.f90:157-720).  pt holds RAY_MAXP points; on
-// return *npts is the section's point count (0 when invalid before any).
-__device__ __attribute__((always_inline)) inline void trace_section(const RayGeomDev& G, const SecInfo& S, double sx, double sz, double x_in, double z_in,
-                              double2* pt, int* npts, double& x_out, double& z_out, RayState& st) {
-  const RayRegionDev& R = *S.R;
-  const int iftop = R.itop, ifbot = R.ibot;
-  const int seqnum = S.seqnum;
-  const double v_average = 5.0;
-  int ifend = 0, start_direction = 0;
-  bool section_is_refraction = false, same_internal_ifend = false, left_start_interface = true;
-  if (seqnum < S.nsec && seqnum > 1) {
-    ifend = S.iend;
-    start_direction = R.id - S.next_reg;
-    section_is_refraction = S.prev_reg != R.id;
+// trace_ray_from_receiver2d / trace_ray_section2d (rays2d.f90:7-720) as a
+// walk.  The reference traces a ray with nested loops: sections, the step
+// loop (ploop) and the corrector's step halvings.  Here a ray is a small
+// state machine that needs exactly one gradient interpolation per round
+// (the predictor's at the current point, or one corrector try), and the
+// kernel's single loop runs rounds until the ray is done: the lanes of a
+// wave then share one interpolation per round whatever step, halving or
+// section each is in, instead of waiting for each other at every nested
+// loop's end.  Every point is computed with the nested form's arithmetic,
+// in its order.  The step's points live in registers (a = pt[n-2],
+// b = pt[n-3], w = pt[n-1]); every change of pt[n-1] is also stored.
+struct RayWalk {
+  enum : int { PRED = 0, CORR = 1, SEC_END = 2, DONE = 3 };
+  const RayGeomDev& G;
+  const RayField* fields;
+  int path, nsec, src;
+  double sx, sz;
+  double2* P;
+  RayState st;
+  // section
+  int sec;
+  SecInfo S;
+  int iftop, ifbot, ifend, start_direction;
+  IfaceRef Itop, Ibot, Iend;
+  bool section_is_refraction, same_internal_ifend, left_start_interface, start_on_bot, start_on_top;
+  double2* pt;
+  int n;
+  double dist_headwave;
+  double x_in, z_in, x_out, z_out;
+  // step
+  double dlmax, p_tol, s_tol, dl;
+  int iter;
+  double grad0[2], pp[2];
+  double2 a, b, w;
+  // round
+  int phase, np_end;
+  double qx, qz;
+
+  __device__ __attribute__((always_inline)) void set_w(double x, double z) {
+    w = make_double2(x, z);
+    pt[n - 1] = w;
   }
-  if (seqnum == S.nsec && seqnum > 1) {
-    ifend = S.iend;
-    start_direction = 0;
-    section_is_refraction = S.prev_reg != R.id;
+  // snap_source on the step's point (the section toward the source)
+  __device__ __attribute__((always_inline)) bool snap_w() {
+    const double dd = sqrt((w.x - sx) * (w.x - sx) + (w.y - sz) * (w.y - sz));
+    if (dd < s_tol) { set_w(sx, sz); return true; }
+    return false;
   }
-  if (seqnum == 1) {
-    ifend = 0;
-    start_direction = (seqnum + 1 <= S.nsec) ? R.id - S.next_reg : 0;
+  __device__ __attribute__((always_inline)) void end_section(int np) {
+    np_end = np;
+    phase = SEC_END;
   }
-  if (ifend && S.istart) {
-    same_internal_ifend = ifend == S.istart && ifend != iftop && ifend != ifbot;
-    if (same_internal_ifend) left_start_interface = false;
+  __device__ __attribute__((always_inline)) void fail() {
+    st.valid = false;
+    end_section(0);
   }
-  const double dlmax = 0.1 * G.dx * v_average;
-  const double p_tol = 5.0e-4 * G.dx;
-  const double s_tol = 2.5 * G.dx;
-  int n = 1;
-  pt[0] = make_double2(x_in, z_in);
-  double dist_headwave = 0.0;
-  double zi = iface_z(G, ifbot, x_in);
-  const bool start_on_bot = fabs(z_in - zi) < 0.1 * G.dz;
-  zi = iface_z(G, iftop, x_in);
-  const bool start_on_top = fabs(z_in - zi) < 0.1 * G.dz;
-  // converted diving phases start the deep section from the interior node
-  // that seeded the return (rays2d.f90:234-264)
-  if (seqnum == 2 && start_on_top && S.istart && S.iend && S.istart == iftop && S.iend == iftop &&
-      (S.path == 5 || S.path == 6 || S.path == 8)) {
-    const int seed = diving_return_node(G, R, *S.F, S.rvel, x_in, z_in, 2.0 * G.dz);
-    if (seed >= 0) {
-      double qx, qz;
-      slot_xz(G, R, seed, qx, qz);
-      const double seed_dist = sqrt((qx - x_in) * (qx - x_in) + (qz - z_in) * (qz - z_in));
-      if (seed_dist > 1.0e-10) {
-        int nseed = (int)ceil(seed_dist / fmax(dlmax, 1.0e-10));
-        if (nseed < 1) nseed = 1;
-        if (nseed > RAY_MAXP - 2) nseed = RAY_MAXP - 2;
-        for (int k = 1; k <= nseed; k++) {
-          const double frac = (double)k / (double)nseed;
-          pt[k] = make_double2(x_in + frac * (qx - x_in), z_in + frac * (qz - z_in));
+
+  // section set-up (rays2d.f90:157-264): interfaces, start conditions,
+  // the first point, the diving phases' seed path
+  __device__ __attribute__((always_inline)) void begin_section() {
+    const int tf = c_path_tf[path][sec - 1];
+    S.seqnum = sec; S.nsec = nsec; S.path = path;
+    S.F = &fields[src * 12 + tf - 1];
+    S.R = &G.reg[S.F->region];
+    S.rvel = G.reg[S.F->region].vel;
+    S.istart = sec < nsec ? c_path_seq[path][2 * sec - 1] : 0;
+    S.iend = sec > 1 ? c_path_seq[path][2 * sec - 2] : 0;
+    S.next_reg = sec < nsec ? c_tf_region[c_path_tf[path][sec]] : 0;
+    if (sec > 1) {
+      const int tfn = c_path_tf[path][sec - 2];
+      S.Fn = &fields[src * 12 + tfn - 1];
+      S.Rn = &G.reg[S.Fn->region];
+      S.prev_reg = c_tf_region[tfn];
+    } else {
+      S.Fn = nullptr; S.Rn = nullptr; S.prev_reg = 0;
+    }
+    const RayRegionDev& R = *S.R;
+    iftop = R.itop; ifbot = R.ibot;
+    const int seqnum = sec;
+    ifend = 0; start_direction = 0;
+    section_is_refraction = false; same_internal_ifend = false; left_start_interface = true;
+    if (seqnum < nsec && seqnum > 1) {
+      ifend = S.iend;
+      start_direction = R.id - S.next_reg;
+      section_is_refraction = S.prev_reg != R.id;
+    }
+    if (seqnum == nsec && seqnum > 1) {
+      ifend = S.iend;
+      start_direction = 0;
+      section_is_refraction = S.prev_reg != R.id;
+    }
+    if (seqnum == 1) {
+      ifend = 0;
+      start_direction = (seqnum + 1 <= nsec) ? R.id - S.next_reg : 0;
+    }
+    if (ifend && S.istart) {
+      same_internal_ifend = ifend == S.istart && ifend != iftop && ifend != ifbot;
+      if (same_internal_ifend) left_start_interface = false;
+    }
+    Itop = iface_ref(G, iftop);
+    Ibot = iface_ref(G, ifbot);
+    Iend = iface_ref(G, ifend);
+    pt = P + (long long)(sec - 1) * RAY_MAXP;
+    n = 1;
+    set_w(x_in, z_in);
+    dist_headwave = 0.0;
+    double zi = iface_z(Ibot, G.dx, x_in);
+    start_on_bot = fabs(z_in - zi) < 0.1 * G.dz;
+    zi = iface_z(Itop, G.dx, x_in);
+    start_on_top = fabs(z_in - zi) < 0.1 * G.dz;
+    // converted diving phases start the deep section from the interior
+    // node that seeded the return (rays2d.f90:234-264)
+    if (seqnum == 2 && start_on_top && S.istart && S.iend && S.istart == iftop && S.iend == iftop &&
+        (path == 5 || path == 6 || path == 8)) {
+      const int seed = diving_return_node(G, R, *S.F, S.rvel, x_in, z_in, 2.0 * G.dz);
+      if (seed >= 0) {
+        double qx_, qz_;
+        slot_xz(G, R, seed, qx_, qz_);
+        const double seed_dist = sqrt((qx_ - x_in) * (qx_ - x_in) + (qz_ - z_in) * (qz_ - z_in));
+        if (seed_dist > 1.0e-10) {
+          int nseed = (int)ceil(seed_dist / fmax(dlmax, 1.0e-10));
+          if (nseed < 1) nseed = 1;
+          if (nseed > RAY_MAXP - 2) nseed = RAY_MAXP - 2;
+          for (int k = 1; k <= nseed; k++) {
+            const double frac = (double)k / (double)nseed;
+            a = w;
+            n = k + 1;
+            set_w(x_in + frac * (qx_ - x_in), z_in + frac * (qz_ - z_in));
+          }
         }
-        n = 1 + nseed;
       }
     }
   }
-  double grad0[2], grad1[2], pp[2];
-  double dl = dlmax;
-  auto snap_source = [&](int m) -> bool {  // point m (1-based) within s_tol of the source
-    const double dd = sqrt((pt[m - 1].x - sx) * (pt[m - 1].x - sx) + (pt[m - 1].y - sz) * (pt[m - 1].y - sz));
-    if (dd < s_tol) { pt[m - 1] = make_double2(sx, sz); return true; }
-    return false;
-  };
-  bool exited = false;  // left ploop through an ending interface
-  while (true) {  // ploop
+
+  // top of the step loop (rays2d.f90:267-311): next point, point cap,
+  // arrival at the source, head-wave length, step length
+  __device__ __attribute__((always_inline)) void enter_step() {
+    b = a;
+    a = w;
     n = n + 1;
-    if (n > RAY_MAXP) { st.valid = false; *npts = 0; return; }
-    if (seqnum == 1 && n > 2) {
-      const double dd = sqrt((pt[n - 2].x - sx) * (pt[n - 2].x - sx) + (pt[n - 2].y - sz) * (pt[n - 2].y - sz));
-      if (dd < s_tol) { pt[n - 1] = make_double2(sx, sz); *npts = n; return; }
+    if (n > RAY_MAXP) { fail(); return; }
+    if (sec == 1 && n > 2) {
+      const double dd = sqrt((a.x - sx) * (a.x - sx) + (a.y - sz) * (a.y - sz));
+      if (dd < s_tol) { set_w(sx, sz); end_section(n); return; }
     }
     if (st.headwave || st.diffracted) {
       if (n > 2) {
-        const double ax = pt[n - 2].x - pt[n - 3].x, az = pt[n - 2].y - pt[n - 3].y;
+        const double ax = a.x - b.x, az = a.y - b.y;
         dist_headwave = dist_headwave + sqrt(ax * ax + az * az);
       }
-      if (dist_headwave > 200.0) { st.valid = false; *npts = 0; return; }
+      if (dist_headwave > 200.0) { fail(); return; }
     }
     dl = dlmax;
     if (n == 2 && (start_on_bot || start_on_top)) dl = 10.0 * dlmax;
-    interp_grad(G, R, *S.F, pt[n - 2].x, pt[n - 2].y, grad0[0], grad0[1]);
-    pp[0] = pt[n - 2].x - dl * grad0[0];
-    pp[1] = pt[n - 2].y - dl * grad0[1];
-    if (start_on_bot && pp[1] > pt[n - 2].y) pp[1] = 2.0 * pt[n - 2].y - pp[1];
-    if (start_on_top && pp[1] < pt[n - 2].y) pp[1] = 2.0 * pt[n - 2].y - pp[1];
+    qx = a.x; qz = a.y;
+    phase = PRED;
+  }
+
+  // the step loop left through the ending interface (rays2d.f90:680-718)
+  __device__ __attribute__((always_inline)) void exited() {
+    if (sec == 1) { fail(); return; }
+    if (ifend) {
+      if (!interface_crossing(G, pt, n, pp, ifend, grad0, dl)) { fail(); return; }
+    }
+    x_out = pt[n - 1].x;
+    z_out = pt[n - 1].y;
+    end_section(n);
+  }
+
+  // the step's point put on an interface the predictor or corrector
+  // crossed: head wave or diffraction, then the next step
+  __device__ __attribute__((always_inline)) void along_interface(double x, double z, bool headwave, bool snap) {
+    set_w(x, z);
+    if (headwave) st.headwave = true; else st.diffracted = true;
+    if (snap && sec == 1 && snap_w()) { end_section(n); return; }
+    enter_step();
+  }
+
+  // predictor (rays2d.f90:313-562), with grad0 at pt[n-2]
+  __device__ __attribute__((always_inline)) void predict(double gx, double gz) {
+    grad0[0] = gx; grad0[1] = gz;
+    const int seqnum = sec;
+    pp[0] = a.x - dl * grad0[0];
+    pp[1] = a.y - dl * grad0[1];
+    if (start_on_bot && pp[1] > a.y) pp[1] = 2.0 * a.y - pp[1];
+    if (start_on_top && pp[1] < a.y) pp[1] = 2.0 * a.y - pp[1];
     if (n == 2 && start_direction != 0 && !section_is_refraction && seqnum != 1) {
       double nx_, nz_, h;
       iface_normal(G, S.istart, x_in, nx_, nz_, h);
-      if (grad0[0] * (start_direction * nx_) + grad0[1] * (start_direction * nz_) > 0.0) {
-        st.valid = false; *npts = 0; return;
-      }
+      if (grad0[0] * (start_direction * nx_) + grad0[1] * (start_direction * nz_) > 0.0) { fail(); return; }
     }
     // ending interface inside the region (rays2d.f90:400-434)
     if (ifend && ifend != iftop && ifend != ifbot) {
-      const double zp = iface_z(G, ifend, pt[n - 2].x), zn = iface_z(G, ifend, pp[0]);
-      const double fp = pt[n - 2].y - zp, fn = pp[1] - zn;
+      const double zp = iface_z(Iend, G.dx, a.x), zn = iface_z(Iend, G.dx, pp[0]);
+      const double fp = a.y - zp, fn = pp[1] - zn;
       if (same_internal_ifend && !left_start_interface) {
         if (fabs(fn) > 0.5 * G.dz) left_start_interface = true;
       } else if (fabs(fp) > 1.0e-3) {
         if (fp * fn <= 0.0) {
           double frac = fabs(fp - fn) > 1.0e-12 ? fp / (fp - fn) : 1.0;
           frac = clamp01(frac);
-          pt[n - 1].x = pt[n - 2].x + frac * (pp[0] - pt[n - 2].x);
-          pt[n - 1].y = iface_z(G, ifend, pt[n - 1].x);
-          if (seqnum == 1) snap_source(n);
-          *npts = n;
+          const double xe = a.x + frac * (pp[0] - a.x);
+          set_w(xe, iface_z(Iend, G.dx, xe));
+          if (seqnum == 1) snap_w();
+          end_section(n);
           return;
         }
       }
     }
     // top interface reached by the predictor (rays2d.f90:436-496)
-    zi = iface_z(G, iftop, pp[0]);
+    double zi = iface_z(Itop, G.dx, pp[0]);
     if (pp[1] <= zi) {
       if (iftop == ifend) {
         if (section_is_refraction) {
-          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited = true; break; }
-          pt[n - 1] = make_double2(pp[0], zi);
-          st.headwave = true;
-          if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
-          continue;
+          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited(); return; }
+          along_interface(pp[0], zi, true, true);
+          return;
         }
-        exited = true;
-        break;
+        exited();
+        return;
       } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
         // leave the start interface
       } else {
-        pt[n - 1] = make_double2(pp[0], zi);
-        st.diffracted = true;
-        if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
-        continue;
+        along_interface(pp[0], zi, false, true);
+        return;
       }
     }
     // bottom interface reached by the predictor (rays2d.f90:498-562)
-    zi = iface_z(G, ifbot, pp[0]);
+    zi = iface_z(Ibot, G.dx, pp[0]);
     if (pp[1] >= zi) {
       if (ifbot == ifend) {
         if (section_is_refraction) {
-          if (S.path == 4 && seqnum == 2) { exited = true; break; }
-          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited = true; break; }
-          pt[n - 1] = make_double2(pp[0], zi);
-          st.headwave = true;
-          if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
-          continue;
+          if (S.path == 4 && seqnum == 2) { exited(); return; }
+          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited(); return; }
+          along_interface(pp[0], zi, true, true);
+          return;
         }
-        exited = true;
-        break;
+        exited();
+        return;
       } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
       } else {
-        pt[n - 1] = make_double2(pp[0], zi);
-        st.diffracted = true;
-        if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
-        continue;
+        along_interface(pp[0], zi, false, true);
+        return;
       }
     }
-    // corrector with step halving (rays2d.f90:564-668)
-    int iter = 1;
-    bool next_ploop = false;
-    while (true) {
-      pp[0] = pt[n - 2].x - dl * grad0[0];
-      pp[1] = pt[n - 2].y - dl * grad0[1];
-      interp_grad(G, R, *S.F, pp[0], pp[1], grad1[0], grad1[1]);
-      pt[n - 1].x = pt[n - 2].x - dl * (grad0[0] + grad1[0]) * 0.5;
-      pt[n - 1].y = pt[n - 2].y - dl * (grad0[1] + grad1[1]) * 0.5;
-      zi = iface_z(G, iftop, pt[n - 1].x);
-      if (pt[n - 1].y <= zi) {
-        if (iftop == ifend) {
-          if (section_is_refraction) {
-            if (refracts(G, S, ifend, pt[n - 1].x, zi, grad0, st)) {
-              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
-              exited = true;
-              break;
-            }
-            pt[n - 1].y = zi;
-            st.headwave = true;
-            next_ploop = true;
-            break;
+    // corrector (rays2d.f90:564-668): its first try at the full step
+    iter = 1;
+    pp[0] = a.x - dl * grad0[0];
+    pp[1] = a.y - dl * grad0[1];
+    qx = pp[0]; qz = pp[1];
+    phase = CORR;
+  }
+
+  // one corrector try, with grad1 at pp
+  __device__ __attribute__((always_inline)) void correct(double g1x, double g1z) {
+    const int seqnum = sec;
+    set_w(a.x - dl * (grad0[0] + g1x) * 0.5, a.y - dl * (grad0[1] + g1z) * 0.5);
+    double zi = iface_z(Itop, G.dx, w.x);
+    if (w.y <= zi) {
+      if (iftop == ifend) {
+        if (section_is_refraction) {
+          if (refracts(G, S, ifend, w.x, zi, grad0, st)) {
+            pp[0] = w.x; pp[1] = w.y;
+            exited();
+            return;
           }
-          pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
-          exited = true;
-          break;
-        } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
-        } else {
-          pt[n - 1].y = zi;
-          st.diffracted = true;
-          next_ploop = true;
-          break;
+          along_interface(w.x, zi, true, false);
+          return;
         }
+        pp[0] = w.x; pp[1] = w.y;
+        exited();
+        return;
+      } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
+      } else {
+        along_interface(w.x, zi, false, false);
+        return;
       }
-      zi = iface_z(G, ifbot, pt[n - 1].x);
-      if (pt[n - 1].y >= zi) {
-        if (ifbot == ifend) {
-          if (section_is_refraction) {
-            if (S.path == 4 && seqnum == 2) {
-              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
-              exited = true;
-              break;
-            }
-            if (refracts(G, S, ifend, pt[n - 1].x, zi, grad0, st)) {
-              pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
-              exited = true;
-              break;
-            }
-            pt[n - 1].y = zi;
-            st.headwave = true;
-            next_ploop = true;
-            break;
-          }
-          pp[0] = pt[n - 1].x; pp[1] = pt[n - 1].y;
-          exited = true;
-          break;
-        } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
-        } else {
-          pt[n - 1].y = zi;
-          st.diffracted = true;
-          next_ploop = true;
-          break;
-        }
-      }
-      const double ex = pt[n - 1].x - pp[0], ez = pt[n - 1].y - pp[1];
-      if (sqrt(ex * ex + ez * ez) < p_tol) break;
-      iter = iter + 1;
-      if (iter > 10) { st.valid = false; *npts = 0; return; }
-      dl = 0.5 * dl;
     }
-    if (exited) break;
-    if (next_ploop) continue;
-    if (seqnum == 1 && snap_source(n)) { *npts = n; return; }
+    zi = iface_z(Ibot, G.dx, w.x);
+    if (w.y >= zi) {
+      if (ifbot == ifend) {
+        if (section_is_refraction) {
+          if (S.path == 4 && seqnum == 2) {
+            pp[0] = w.x; pp[1] = w.y;
+            exited();
+            return;
+          }
+          if (refracts(G, S, ifend, w.x, zi, grad0, st)) {
+            pp[0] = w.x; pp[1] = w.y;
+            exited();
+            return;
+          }
+          along_interface(w.x, zi, true, false);
+          return;
+        }
+        pp[0] = w.x; pp[1] = w.y;
+        exited();
+        return;
+      } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
+      } else {
+        along_interface(w.x, zi, false, false);
+        return;
+      }
+    }
+    const double ex = w.x - pp[0], ez = w.y - pp[1];
+    if (sqrt(ex * ex + ez * ez) < p_tol) {
+      if (seqnum == 1 && snap_w()) { end_section(n); return; }
+      enter_step();
+      return;
+    }
+    iter = iter + 1;
+    if (iter > 10) { fail(); return; }
+    dl = 0.5 * dl;
+    pp[0] = a.x - dl * grad0[0];
+    pp[1] = a.y - dl * grad0[1];
+    qx = pp[0]; qz = pp[1];
   }
-  // left ploop through the ending interface
-  if (seqnum == 1) { st.valid = false; *npts = 0; return; }
-  if (ifend) {
-    if (!interface_crossing(G, pt, n, pp, ifend, grad0, dl)) { st.valid = false; *npts = 0; return; }
-  }
-  x_out = pt[n - 1].x;
-  z_out = pt[n - 1].y;
-  *npts = n;
-}
+};
 
 // ---------------------------------------------------------------- kernels
 
@@ -697,41 +811,46 @@ __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict_
   // the forward program keeps the water leg of PsedP and PmP rays only
   // (fmt2d_main.f90:1290-1294, 1334-1337); it is in the receiver time either way
   o.water_time = (program && J.path == 1) ? 0.0 : wt;
-  // trace_ray_from_receiver2d (rays2d.f90:7-153)
-  RayState st{true, false, false};
-  double x_in = rx, z_in = rz, x_out = 0.0, z_out = 0.0;
-  double2* P = pts + pt_off[r];
-  for (int n = nsec; n >= 1; n--) {
-    const int tf = c_path_tf[J.path][n - 1];
-    SecInfo S;
-    S.seqnum = n; S.nsec = nsec; S.path = J.path;
-    S.F = &fields[s * 12 + tf - 1];
-    S.R = &G.reg[S.F->region];
-    S.rvel = G.reg[S.F->region].vel;
-    S.istart = n < nsec ? c_path_seq[J.path][2 * n - 1] : 0;
-    S.iend = n > 1 ? c_path_seq[J.path][2 * n - 2] : 0;
-    S.next_reg = n < nsec ? c_tf_region[c_path_tf[J.path][n]] : 0;
-    if (n > 1) {
-      const int tfn = c_path_tf[J.path][n - 2];
-      S.Fn = &fields[s * 12 + tfn - 1];
-      S.Rn = &G.reg[S.Fn->region];
-      S.prev_reg = c_tf_region[tfn];
-    } else {
-      S.Fn = nullptr; S.Rn = nullptr; S.prev_reg = 0;
+  // trace_ray_from_receiver2d (rays2d.f90:7-153): sections nsec..1
+  RayWalk W{G, fields, J.path, nsec, s, sx, sz, pts + pt_off[r]};
+  W.st = RayState{true, false, false};
+  W.sec = nsec;
+  W.x_in = rx; W.z_in = rz; W.x_out = 0.0; W.z_out = 0.0;
+  W.dlmax = 0.1 * G.dx * 5.0;  // 0.1 dx v_average
+  W.p_tol = 5.0e-4 * G.dx;
+  W.s_tol = 2.5 * G.dx;
+  W.phase = RayWalk::DONE;
+  {
+    // receiver time and gradient in the last section's field
+    const RayField& F = fields[s * 12 + c_path_tf[J.path][nsec - 1] - 1];
+    const RayRegionDev& R = G.reg[F.region];
+    const double tr = interp_time(G, R, F, rx, rz);
+    double gx, gz;
+    interp_grad(G, R, F, rx, rz, gx, gz);
+    o.t = tr;
+    if (tr > 1.0e10 || (gx == 0.0 && gz == 0.0)) W.st.valid = false;
+    else {
+      W.begin_section();
+      W.enter_step();
     }
-    if (n == nsec) {
-      const double tr = interp_time(G, *S.R, *S.F, rx, rz);
-      double gx, gz;
-      interp_grad(G, *S.R, *S.F, rx, rz, gx, gz);
-      o.t = tr;
-      if (tr > 1.0e10 || (gx == 0.0 && gz == 0.0)) { st.valid = false; break; }
-    }
-    int np = 0;
-    trace_section(G, S, sx, sz, x_in, z_in, P + (long long)(n - 1) * RAY_MAXP, &np, x_out, z_out, st);
-    o.npts[n - 1] = np;
-    if (!st.valid) break;
-    if (n > 1) { x_in = x_out; z_in = z_out; }
   }
+  for (;;) {
+    // a section that ended: record it, start the next one
+    while (W.phase == RayWalk::SEC_END) {
+      o.npts[W.sec - 1] = W.np_end;
+      if (!W.st.valid || W.sec == 1) { W.phase = RayWalk::DONE; break; }
+      W.x_in = W.x_out; W.z_in = W.z_out;
+      W.sec = W.sec - 1;
+      W.begin_section();
+      W.enter_step();
+    }
+    if (W.phase == RayWalk::DONE) break;
+    double gx, gz;
+    interp_grad(G, *W.S.R, *W.S.F, W.qx, W.qz, gx, gz);
+    if (W.phase == RayWalk::PRED) W.predict(gx, gz);
+    else W.correct(gx, gz);
+  }
+  const RayState st = W.st;
   o.valid = st.valid ? 1 : 0;
   if (o.valid) o.t = o.t + wt;
   else if (program) o.water_time = 0.0;  // set on valid rays only (:1336)
