@@ -33,6 +33,15 @@ int build_ray_geom(fmt2d_ctx* ctx) {
     HIPCHK(upload(ctx->risx[k], m.isec[k].x, st));
     HIPCHK(upload(ctx->risz[k], m.isec[k].z, st));
     G.isx[k] = ctx->risx[k].p; G.isz[k] = ctx->risz[k].p; G.isn[k] = (int)m.isec[k].x.size();
+    // intersections with the same node x (bit for bit) point at one copy:
+    // the tracer then computes their cell index and weight once
+    for (int j = 1; j < k; j++)
+      if (m.isec[j].x.size() == m.isec[k].x.size() &&
+          (m.isec[k].x.empty() ||
+           memcmp(m.isec[j].x.data(), m.isec[k].x.data(), sizeof(double) * m.isec[k].x.size()) == 0)) {
+        G.isx[k] = G.isx[j];
+        break;
+      }
   }
   for (int r = 1; r <= NREG; r++) {
     const HostRegion& R = m.reg[r];
@@ -528,14 +537,26 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   if (nray == 0) return FMT2D_OK;
   HIPCHK(upload(ctx->rjobs, ctx->hjobs, st));
   HIPCHK(upload(ctx->rpoff, ctx->hpoff, st));
+  {
+    // trace order: by source, then path, input order within (a counting
+    // sort), so that a wave's lanes trace rays of one stage chain through
+    // neighbouring receivers, of similar length and through the same fields
+    const int nkey = ctx->nsrc * 6;
+    std::vector<int> start((size_t)nkey + 1, 0);
+    for (int r = 0; r < nray; r++) start[(size_t)src[r] * 6 + path[r] + 1]++;
+    for (int k = 0; k < nkey; k++) start[k + 1] += start[k];
+    ctx->hperm.resize(nray);
+    for (int r = 0; r < nray; r++) ctx->hperm[start[(size_t)src[r] * 6 + path[r]]++] = r;
+    HIPCHK(upload(ctx->rperm, ctx->hperm, st));
+  }
   if (ctx->rpts.n < (size_t)off) HIPCHK(ctx->rpts.alloc((size_t)off));
   if (ctx->rout.n < (size_t)nray) HIPCHK(ctx->rout.alloc((size_t)nray));
   const int nsf = ctx->hgeom.nsf;
   if (ctx->rtsf.n < (size_t)ctx->nsrc * 3 * nsf) HIPCHK(ctx->rtsf.alloc((size_t)ctx->nsrc * 3 * nsf));
   HIPCHK(nearest_calls_reset(st));
   HIPCHK(launch_seafloor_times(ctx->rgeom.p, ctx->rfields.p, ctx->nsrc, nsf, ctx->rtsf.p, st));
-  HIPCHK(launch_trace(ctx->rgeom.p, ctx->rfields.p, ctx->d_srcx.p, ctx->d_srcz.p, ctx->rtsf.p, ctx->rjobs.p, nray,
-                      ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, ctx->fwd_program ? 1 : 0, st));
+  HIPCHK(launch_trace(ctx->rgeom.p, ctx->rfields.p, ctx->d_srcx.p, ctx->d_srcz.p, ctx->rtsf.p, ctx->rjobs.p,
+                      ctx->rperm.p, nray, ctx->rpoff.p, ctx->rpts.p, ctx->rout.p, ctx->fwd_program ? 1 : 0, st));
   std::vector<RayOut> out(nray);
   HIPCHK(hipMemcpyAsync(out.data(), ctx->rout.p, sizeof(RayOut) * nray, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

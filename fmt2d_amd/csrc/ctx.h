@@ -168,6 +168,8 @@ struct fmt2d_ctx {
   std::vector<RayJob> hjobs;
   std::vector<long long> hpoff;
   DBuf<RayJob> rjobs;
+  DBuf<int> rperm;                    // trace order of the rays (by source, then path; input order within)
+  std::vector<int> hperm;
   DBuf<RayOut> rout;
   DBuf<long long> rpoff;
   DBuf<double2> rpts;

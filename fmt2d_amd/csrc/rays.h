@@ -96,9 +96,11 @@ hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, in
 hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, const RayOut* out,
                                   const double* g, int nray, const int* field_of, long long nslot, double* lam,
                                   int* active, hipStream_t st);
+// order: the rays in the order they are traced (a permutation of 0..nray-1;
+// neighbouring lanes of a wave then trace rays of one source and path)
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
-                        const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
-                        RayOut* out, int program, hipStream_t st);
+                        const double* tsf, const RayJob* jobs, const int* order, int nray, const long long* pt_off,
+                        double2* pts, RayOut* out, int program, hipStream_t st);
 // Frechet contributions of each ray in the reference's accumulation order:
 // pass 1 (keys == nullptr) counts them per ray, pass 2 writes
 // key = ray << 32 | (parameter - 1), value = weight * length (or the

@@ -259,6 +259,25 @@ __device__ __attribute__((always_inline)) inline double iface_z(const IfaceRef& 
   return (1.0 - w) * zz.lo + w * zz.hi;
 }
 
+// iface_z of two intersections at one x.  Intersections whose node x are the
+// same array (the host shares one copy of equal arrays) share the cell
+// index and weight, which are then the same numbers either call computes.
+__device__ __attribute__((always_inline)) inline void iface_z2(const IfaceRef& f, const IfaceRef& g, double dx,
+                                                               double px, double& zf, double& zg) {
+  if (f.x != g.x || f.n != g.n || f.n < 2) {
+    zf = iface_z(f, dx, px);
+    zg = iface_z(g, dx, px);
+    return;
+  }
+  const int ix = ifloor((px - f.x0) / dx) + 1;
+  if (ix < 1) { zf = f.z[0]; zg = g.z[0]; return; }
+  if (ix >= f.n) { zf = f.z[f.n - 1]; zg = g.z[g.n - 1]; return; }
+  const double w = clamp01((px - f.x[ix - 1]) / dx);
+  const Pair64 a = load_pair(f.z + ix - 1), b = load_pair(g.z + ix - 1);
+  zf = (1.0 - w) * a.lo + w * a.hi;
+  zg = (1.0 - w) * b.lo + w * b.hi;
+}
+
 // interface_normal2d (rays2d.f90:1266-1308)
 __device__ __attribute__((always_inline)) inline void iface_normal(const RayGeomDev& G, int k, double px, double& nx_, double& nz_, double& h) {
   const int n = G.isn[k];
@@ -569,8 +588,9 @@ struct RayWalk {
     enter_step();
   }
 
-  // predictor (rays2d.f90:313-562), with grad0 at pt[n-2]
-  __device__ __attribute__((always_inline)) void predict(double gx, double gz) {
+  // predictor, first part (rays2d.f90:313-434), with grad0 at pt[n-2]:
+  // false when the section ended
+  __device__ __attribute__((always_inline)) bool predict_start(double gx, double gz) {
     grad0[0] = gx; grad0[1] = gz;
     const int seqnum = sec;
     pp[0] = a.x - dl * grad0[0];
@@ -580,7 +600,7 @@ struct RayWalk {
     if (n == 2 && start_direction != 0 && !section_is_refraction && seqnum != 1) {
       double nx_, nz_, h;
       iface_normal(G, S.istart, x_in, nx_, nz_, h);
-      if (grad0[0] * (start_direction * nx_) + grad0[1] * (start_direction * nz_) > 0.0) { fail(); return; }
+      if (grad0[0] * (start_direction * nx_) + grad0[1] * (start_direction * nz_) > 0.0) { fail(); return false; }
     }
     // ending interface inside the region (rays2d.f90:400-434)
     if (ifend && ifend != iftop && ifend != ifbot) {
@@ -596,17 +616,22 @@ struct RayWalk {
           set_w(xe, iface_z(Iend, G.dx, xe));
           if (seqnum == 1) snap_w();
           end_section(n);
-          return;
+          return false;
         }
       }
     }
-    // top interface reached by the predictor (rays2d.f90:436-496)
-    double zi = iface_z(Itop, G.dx, pp[0]);
-    if (pp[1] <= zi) {
+    return true;
+  }
+
+  // predictor, second part (rays2d.f90:436-562): the top and bottom
+  // interfaces at pp (depths zt, zb), then the corrector's first try
+  __device__ __attribute__((always_inline)) void predict_end(double zt, double zb) {
+    const int seqnum = sec;
+    if (pp[1] <= zt) {
       if (iftop == ifend) {
         if (section_is_refraction) {
-          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited(); return; }
-          along_interface(pp[0], zi, true, true);
+          if (refracts(G, S, ifend, pp[0], zt, grad0, st)) { exited(); return; }
+          along_interface(pp[0], zt, true, true);
           return;
         }
         exited();
@@ -614,25 +639,23 @@ struct RayWalk {
       } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
         // leave the start interface
       } else {
-        along_interface(pp[0], zi, false, true);
+        along_interface(pp[0], zt, false, true);
         return;
       }
     }
-    // bottom interface reached by the predictor (rays2d.f90:498-562)
-    zi = iface_z(Ibot, G.dx, pp[0]);
-    if (pp[1] >= zi) {
+    if (pp[1] >= zb) {
       if (ifbot == ifend) {
         if (section_is_refraction) {
           if (S.path == 4 && seqnum == 2) { exited(); return; }
-          if (refracts(G, S, ifend, pp[0], zi, grad0, st)) { exited(); return; }
-          along_interface(pp[0], zi, true, true);
+          if (refracts(G, S, ifend, pp[0], zb, grad0, st)) { exited(); return; }
+          along_interface(pp[0], zb, true, true);
           return;
         }
         exited();
         return;
       } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
       } else {
-        along_interface(pp[0], zi, false, true);
+        along_interface(pp[0], zb, false, true);
         return;
       }
     }
@@ -644,20 +667,19 @@ struct RayWalk {
     phase = CORR;
   }
 
-  // one corrector try, with grad1 at pp
-  __device__ __attribute__((always_inline)) void correct(double g1x, double g1z) {
+  // a corrector try, second part: the top and bottom interfaces at the new
+  // point w (depths zt, zb), convergence, halving
+  __device__ __attribute__((always_inline)) void correct_end(double zt, double zb) {
     const int seqnum = sec;
-    set_w(a.x - dl * (grad0[0] + g1x) * 0.5, a.y - dl * (grad0[1] + g1z) * 0.5);
-    double zi = iface_z(Itop, G.dx, w.x);
-    if (w.y <= zi) {
+    if (w.y <= zt) {
       if (iftop == ifend) {
         if (section_is_refraction) {
-          if (refracts(G, S, ifend, w.x, zi, grad0, st)) {
+          if (refracts(G, S, ifend, w.x, zt, grad0, st)) {
             pp[0] = w.x; pp[1] = w.y;
             exited();
             return;
           }
-          along_interface(w.x, zi, true, false);
+          along_interface(w.x, zt, true, false);
           return;
         }
         pp[0] = w.x; pp[1] = w.y;
@@ -665,12 +687,11 @@ struct RayWalk {
         return;
       } else if (n == 2 && start_on_top && S.istart && S.istart == iftop) {
       } else {
-        along_interface(w.x, zi, false, false);
+        along_interface(w.x, zt, false, false);
         return;
       }
     }
-    zi = iface_z(Ibot, G.dx, w.x);
-    if (w.y >= zi) {
+    if (w.y >= zb) {
       if (ifbot == ifend) {
         if (section_is_refraction) {
           if (S.path == 4 && seqnum == 2) {
@@ -678,12 +699,12 @@ struct RayWalk {
             exited();
             return;
           }
-          if (refracts(G, S, ifend, w.x, zi, grad0, st)) {
+          if (refracts(G, S, ifend, w.x, zb, grad0, st)) {
             pp[0] = w.x; pp[1] = w.y;
             exited();
             return;
           }
-          along_interface(w.x, zi, true, false);
+          along_interface(w.x, zb, true, false);
           return;
         }
         pp[0] = w.x; pp[1] = w.y;
@@ -691,7 +712,7 @@ struct RayWalk {
         return;
       } else if (n == 2 && start_on_bot && S.istart && S.istart == ifbot) {
       } else {
-        along_interface(w.x, zi, false, false);
+        along_interface(w.x, zb, false, false);
         return;
       }
     }
@@ -707,6 +728,27 @@ struct RayWalk {
     pp[0] = a.x - dl * grad0[0];
     pp[1] = a.y - dl * grad0[1];
     qx = pp[0]; qz = pp[1];
+  }
+
+  // the rest of a round, with the gradient (gx, gz) interpolated at the
+  // round's query point: the predictor, or a corrector try (its new point,
+  // with grad1 = (gx, gz)).  Both then test one point against the region's
+  // top and bottom interfaces, which every lane does at once.
+  __device__ __attribute__((always_inline)) void advance(double gx, double gz) {
+    const bool pred = phase == PRED;
+    double cx, cy;
+    if (pred) {
+      if (!predict_start(gx, gz)) return;
+      cx = pp[0]; cy = pp[1];
+    } else {
+      set_w(a.x - dl * (grad0[0] + gx) * 0.5, a.y - dl * (grad0[1] + gz) * 0.5);
+      cx = w.x; cy = w.y;
+    }
+    (void)cy;
+    double zt, zb;
+    iface_z2(Itop, Ibot, G.dx, cx, zt, zb);
+    if (pred) predict_end(zt, zb);
+    else correct_end(zt, zb);
   }
 };
 
@@ -729,10 +771,12 @@ __global__ void seafloor_times_kernel(const RayGeomDev* __restrict__ Gp, const R
 
 __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict__ Gp, const RayField* __restrict__ fields,
                              const double* __restrict__ src_x, const double* __restrict__ src_z,
-                             const double* __restrict__ tsf, const RayJob* __restrict__ jobs, int nray,
-                             const long long* __restrict__ pt_off, double2* pts, RayOut* out, int program) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nray) return;
+                             const double* __restrict__ tsf, const RayJob* __restrict__ jobs,
+                             const int* __restrict__ order, int nray, const long long* __restrict__ pt_off,
+                             double2* pts, RayOut* out, int program) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nray) return;
+  const int r = order[k];
   const RayGeomDev& G = *Gp;
   const RayJob J = jobs[r];
   RayOut o;
@@ -847,8 +891,7 @@ __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict_
     if (W.phase == RayWalk::DONE) break;
     double gx, gz;
     interp_grad(G, *W.S.R, *W.S.F, W.qx, W.qz, gx, gz);
-    if (W.phase == RayWalk::PRED) W.predict(gx, gz);
-    else W.correct(gx, gz);
+    W.advance(gx, gz);
   }
   const RayState st = W.st;
   o.valid = st.valid ? 1 : 0;
@@ -1079,11 +1122,11 @@ hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, c
 }
 
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
-                        const double* tsf, const RayJob* jobs, int nray, const long long* pt_off, double2* pts,
-                        RayOut* out, int program, hipStream_t st) {
+                        const double* tsf, const RayJob* jobs, const int* order, int nray, const long long* pt_off,
+                        double2* pts, RayOut* out, int program, hipStream_t st) {
   if (nray == 0) return hipSuccess;
-  hipLaunchKernelGGL(trace_kernel, dim3((nray + 63) / 64), dim3(64), 0, st, G, fields, src_x, src_z, tsf, jobs, nray,
-                     pt_off, pts, out, program);
+  hipLaunchKernelGGL(trace_kernel, dim3((nray + 63) / 64), dim3(64), 0, st, G, fields, src_x, src_z, tsf, jobs, order,
+                     nray, pt_off, pts, out, program);
   return hipGetLastError();
 }
 
