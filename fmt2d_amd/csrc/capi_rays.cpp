@@ -519,6 +519,13 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   HIPCHK(upload(ctx->rfields, fl, st));
   HIPCHK(upload(ctx->d_srcx, ctx->src_x, st));
   HIPCHK(upload(ctx->d_srcz, ctx->src_z, st));
+  // the work per ray of the last trace predicts this one's if the rays are the same
+  bool same_rays = (int)ctx->hjobs.size() == nray && (int)ctx->hwork.size() == nray;
+  for (int r = 0; same_rays && r < nray; r++) {
+    const RayJob& j = ctx->hjobs[r];
+    same_rays = j.src == src[r] && j.path == path[r] && j.rx == rx[r] && j.rz == rz[r];
+  }
+  if (!same_rays) ctx->hwork.clear();
   ctx->hjobs.resize(nray);
   ctx->hpoff.resize((size_t)nray + 1);
   long long off = 0;
@@ -538,15 +545,33 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   HIPCHK(upload(ctx->rjobs, ctx->hjobs, st));
   HIPCHK(upload(ctx->rpoff, ctx->hpoff, st));
   {
-    // trace order: by source, then path, input order within (a counting
-    // sort), so that a wave's lanes trace rays of one stage chain through
-    // neighbouring receivers, of similar length and through the same fields
+    // Trace order: by source, then path, input order within, so that a
+    // wave's lanes trace rays of one stage chain through neighbouring
+    // receivers.  When the last trace was of the same rays, its
+    // interpolations per ray order them first (descending, in buckets; the
+    // chain order within a bucket): waves start in order as earlier ones
+    // finish, and the longest rays then start first instead of running on
+    // alone at the end, and a wave's lanes do about the same work.  Stable
+    // counting sorts.
     const int nkey = ctx->nsrc * 6;
-    std::vector<int> start((size_t)nkey + 1, 0);
+    std::vector<int> start((size_t)nkey + 1, 0), by_chain(nray);
     for (int r = 0; r < nray; r++) start[(size_t)src[r] * 6 + path[r] + 1]++;
     for (int k = 0; k < nkey; k++) start[k + 1] += start[k];
+    for (int r = 0; r < nray; r++) by_chain[start[(size_t)src[r] * 6 + path[r]]++] = r;
+    const int NB = 1024;
+    auto bucket = [&](int r) {
+      if (ctx->hwork.empty()) return 0;
+      const int c = ctx->hwork[r] / 32;
+      return NB - 1 - (c >= NB - 1 ? NB - 1 : c);  // descending
+    };
+    std::vector<int> bstart(NB + 1, 0);
+    for (int r = 0; r < nray; r++) bstart[bucket(r) + 1]++;
+    for (int k = 0; k < NB; k++) bstart[k + 1] += bstart[k];
     ctx->hperm.resize(nray);
-    for (int r = 0; r < nray; r++) ctx->hperm[start[(size_t)src[r] * 6 + path[r]]++] = r;
+    for (int i = 0; i < nray; i++) {
+      const int r = by_chain[i];
+      ctx->hperm[bstart[bucket(r)]++] = r;
+    }
     HIPCHK(upload(ctx->rperm, ctx->hperm, st));
   }
   if (ctx->rpts.n < (size_t)off) HIPCHK(ctx->rpts.alloc((size_t)off));
@@ -567,9 +592,11 @@ int fmt2d_trace(fmt2d_ctx* ctx, int nray, const int* src, const int* path, const
   }
   ctx->h_tcalc.resize(nray);
   ctx->h_valid.resize(nray);
+  ctx->hwork.resize(nray);
   for (int r = 0; r < nray; r++) {
     t_rcv[r] = out[r].t; valid[r] = out[r].valid;
     ctx->h_tcalc[r] = out[r].t; ctx->h_valid[r] = out[r].valid;
+    ctx->hwork[r] = out[r].work;
   }
   ctx->g_nnz = -1;
   ctx->gp_nnz = -1;

@@ -168,8 +168,9 @@ struct fmt2d_ctx {
   std::vector<RayJob> hjobs;
   std::vector<long long> hpoff;
   DBuf<RayJob> rjobs;
-  DBuf<int> rperm;                    // trace order of the rays (by source, then path; input order within)
+  DBuf<int> rperm;                    // trace order of the rays (longest first; by source and path within)
   std::vector<int> hperm;
+  std::vector<int> hwork;             // interpolations per ray in the last trace of hjobs
   DBuf<RayOut> rout;
   DBuf<long long> rpoff;
   DBuf<double2> rpts;

@@ -86,6 +86,7 @@ struct RayOut {
   int npts[RAY_MAXSEC];
   int headwave, diffracted;
   int direct;          // Pg taken as the direct water wave (no sections)
+  int work;            // gradient interpolations the walk took (rounds of the trace loop)
 };
 
 hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, int nsrc, int nsf, double* tsf,
@@ -96,8 +97,9 @@ hipError_t launch_seafloor_times(const RayGeomDev* G, const RayField* fields, in
 hipError_t launch_adjoint_sources(const RayGeomDev* G, const RayField* fields, const RayJob* jobs, const RayOut* out,
                                   const double* g, int nray, const int* field_of, long long nslot, double* lam,
                                   int* active, hipStream_t st);
-// order: the rays in the order they are traced (a permutation of 0..nray-1;
-// neighbouring lanes of a wave then trace rays of one source and path)
+// order: the rays in the order they are traced (a permutation of 0..nray-1:
+// the longest first, so that no long wave starts late; rays of one source
+// and path side by side)
 hipError_t launch_trace(const RayGeomDev* G, const RayField* fields, const double* src_x, const double* src_z,
                         const double* tsf, const RayJob* jobs, const int* order, int nray, const long long* pt_off,
                         double2* pts, RayOut* out, int program, hipStream_t st);

@@ -45,14 +45,21 @@ __device__ inline int rnode(const RayGeomDev& G, const RayRegionDev& R, int ix, 
   return (R.flags[q] & RF_IN) ? q : -1;
 }
 
-// find_nearby_region_node (rays2d.f90:1010-1034)
+// find_nearby_region_node (rays2d.f90:1010-1034): the first region node of
+// rows iz+1, iz-1, ..., iz+5, iz-5 in column ix.  The ten flags are read at
+// once (one memory round trip instead of up to ten in a row), then taken in
+// that order.
 __device__ inline int rnode_nearby(const RayGeomDev& G, const RayRegionDev& R, int ix, int iz) {
-  for (int d = 1; d <= 5; d++) {
-    int n = rnode(G, R, ix, iz + d);
-    if (n >= 0) return n;
-    n = rnode(G, R, ix, iz - d);
-    if (n >= 0) return n;
+  if (ix < 1 || ix > G.nx) return -1;
+  uint16_t f[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    const int j = iz + (k & 1 ? -(k / 2 + 1) : k / 2 + 1);
+    f[k] = (j >= 1 && j <= G.nz) ? R.flags[(j - 1) * G.nx + (ix - 1)] : (uint16_t)0;
   }
+#pragma unroll
+  for (int k = 0; k < 10; k++)
+    if (f[k] & RF_IN) return (iz + (k & 1 ? -(k / 2 + 1) : k / 2 + 1) - 1) * G.nx + (ix - 1);
   return -1;
 }
 
@@ -164,7 +171,9 @@ __device__ __attribute__((always_inline)) inline void interp_grad(const RayGeomD
   if (ix >= G.nx) ix = G.nx - 1;
   if (iz < 1) iz = 1;
   if (iz >= G.nz) iz = G.nz - 1;
-  const double w1 = clamp01((px - G.xg[ix - 1]) / G.dx), w2 = clamp01((pz - G.zg[iz - 1]) / G.dz);
+  // grid coordinates of the cell's corners (xg[ix-1], xg[ix], zg[iz-1], zg[iz])
+  const Pair64 cx = load_pair(G.xg + ix - 1), cz = load_pair(G.zg + iz - 1);
+  const double w1 = clamp01((px - cx.lo) / G.dx), w2 = clamp01((pz - cz.lo) / G.dz);
   // the cell's corners lie in the grid (ix < nx, iz < nz): their gradients
   // are loaded together with their region flags, and a corner outside the
   // region takes the nearby node's instead
@@ -175,6 +184,8 @@ __device__ __attribute__((always_inline)) inline void interp_grad(const RayGeomD
   double g1x = x12.lo, g1z = z12.lo, g2x = x12.hi, g2z = z12.hi;
   double g3x = x34.lo, g3z = z34.lo, g4x = x34.hi, g4z = z34.hi;
   const bool in1 = f12.lo & RF_IN, in2 = f12.hi & RF_IN, in3 = f34.lo & RF_IN, in4 = f34.hi & RF_IN;
+  // the corners' own gradients (the nearest-node fallback below takes one)
+  const double r1x = g1x, r1z = g1z, r2x = g2x, r2z = g2z, r3x = g3x, r3z = g3z, r4x = g4x, r4z = g4z;
   if (!in1) { const int m = rnode_nearby(G, R, ix, iz); g1x = m >= 0 ? F.GX[m] : 0.0; g1z = m >= 0 ? F.GZ[m] : 0.0; }
   if (!in2) { const int m = rnode_nearby(G, R, ix + 1, iz); g2x = m >= 0 ? F.GX[m] : 0.0; g2z = m >= 0 ? F.GZ[m] : 0.0; }
   if (!in3) { const int m = rnode_nearby(G, R, ix, iz + 1); g3x = m >= 0 ? F.GX[m] : 0.0; g3z = m >= 0 ? F.GZ[m] : 0.0; }
@@ -185,10 +196,28 @@ __device__ __attribute__((always_inline)) inline void interp_grad(const RayGeomD
   double gx = (1.0 - w1) * (1.0 - w2) * g1x + w1 * (1.0 - w2) * g2x + (1.0 - w1) * w2 * g3x + w1 * w2 * g4x;
   double gz = (1.0 - w1) * (1.0 - w2) * g1z + w1 * (1.0 - w2) * g2z + (1.0 - w1) * w2 * g3z + w1 * w2 * g4z;
   if (sqrt(gx * gx + gz * gz) < 0.01) {
-    // interpolate_gradient_reg (rays2d.f90:1036-1073): the nearest node's gradient
-    const int m = position_in_region(G, R, px, pz);
-    gx = m >= 0 ? F.GX[m] : 0.0;
-    gz = m >= 0 ? F.GZ[m] : 0.0;
+    // interpolate_gradient_reg (rays2d.f90:1036-1073): the nearest node's
+    // gradient.  find_position_in_region2d (position_in_region) takes the
+    // nearest region node among this cell's corners, in the order (ix, iz),
+    // (ix, iz+1), (ix+1, iz), (ix+1, iz+1), first minimum; the corners'
+    // flags, coordinates and gradients are at hand.  With no corner in the
+    // region it searches the whole region.
+    double best = 1.0e100;
+    bool found = false;
+    auto offer = [&](bool in, double x, double z, double ax, double az) {
+      if (!in) return;
+      const double dist = (px - x) * (px - x) + (pz - z) * (pz - z);
+      if (dist < best) { best = dist; gx = ax; gz = az; found = true; }
+    };
+    offer(in1, cx.lo, cz.lo, r1x, r1z);
+    offer(in3, cx.lo, cz.hi, r3x, r3z);
+    offer(in2, cx.hi, cz.lo, r2x, r2z);
+    offer(in4, cx.hi, cz.hi, r4x, r4z);
+    if (!found) {
+      const int m = nearest_any(G, R, px, pz, 1.0e100);
+      gx = m >= 0 ? F.GX[m] : 0.0;
+      gz = m >= 0 ? F.GZ[m] : 0.0;
+    }
   }
   dtdx = gx;
   dtdz = gz;
@@ -421,6 +450,8 @@ __device__ __attribute__((always_inline)) inline bool refracts(const RayGeomDev&
 // loop's end.  Every point is computed with the nested form's arithmetic,
 // in its order.  The step's points live in registers (a = pt[n-2],
 // b = pt[n-3], w = pt[n-1]); every change of pt[n-1] is also stored.
+constexpr int RAY_LONG_ROUNDS = 12000;
+
 struct RayWalk {
   enum : int { PRED = 0, CORR = 1, SEC_END = 2, DONE = 3 };
   const RayGeomDev& G;
@@ -782,7 +813,7 @@ __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict_
   RayOut o;
   o.t = 0.0; o.water_time = 0.0; o.valid = 1; o.nsec = c_path_ntf[J.path];
   for (int k = 0; k < RAY_MAXSEC; k++) o.npts[k] = 0;
-  o.headwave = 0; o.diffracted = 0; o.direct = 0;
+  o.headwave = 0; o.diffracted = 0; o.direct = 0; o.work = 0;
   const int nsec = o.nsec, s = J.src;
   const double sx = src_x[s], sz = src_z[s];
   // receiver placement (invert2d_main.f90:517-531, inversion2d.f90:4502-4519):
@@ -889,6 +920,11 @@ __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict_
       W.enter_step();
     }
     if (W.phase == RayWalk::DONE) break;
+    o.work++;
+    // a ray this long (well beyond the usual few thousand steps) is one of
+    // the few that decide when the trace ends: its wave takes the issue
+    // priority from here on
+    if (o.work == RAY_LONG_ROUNDS) __builtin_amdgcn_s_setprio(3);
     double gx, gz;
     interp_grad(G, *W.S.R, *W.S.F, W.qx, W.qz, gx, gz);
     W.advance(gx, gz);
