@@ -450,8 +450,6 @@ __device__ __attribute__((always_inline)) inline bool refracts(const RayGeomDev&
 // loop's end.  Every point is computed with the nested form's arithmetic,
 // in its order.  The step's points live in registers (a = pt[n-2],
 // b = pt[n-3], w = pt[n-1]); every change of pt[n-1] is also stored.
-constexpr int RAY_LONG_ROUNDS = 12000;
-
 struct RayWalk {
   enum : int { PRED = 0, CORR = 1, SEC_END = 2, DONE = 3 };
   const RayGeomDev& G;
@@ -921,10 +919,6 @@ __global__ void __launch_bounds__(64) trace_kernel(const RayGeomDev* __restrict_
     }
     if (W.phase == RayWalk::DONE) break;
     o.work++;
-    // a ray this long (well beyond the usual few thousand steps) is one of
-    // the few that decide when the trace ends: its wave takes the issue
-    // priority from here on
-    if (o.work == RAY_LONG_ROUNDS) __builtin_amdgcn_s_setprio(3);
     double gx, gz;
     interp_grad(G, *W.S.R, *W.S.F, W.qx, W.qz, gx, gz);
     W.advance(gx, gz);
