@@ -424,14 +424,8 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
     unsigned long long* front[2] = {ctx->tfront.p, ctx->tfront.p + nf};
     HIPCHK(hipMemsetAsync(ctx->tfront.p, 0x7F, sizeof(unsigned long long) * 2 * nf, st));
     if (f32) HIPCHK(launch_f32_reset(dl, nf, nx * nz, st));
-    const bool per_field = ctx->solver_launch == FMT2D_LAUNCH_FIELD;
-    if (per_field) {
-      // per-field lists and counters (the per-field schedule)
-      if (ctx->fcnt.n < 4 * (size_t)nf) HIPCHK(ctx->fcnt.alloc(4 * (size_t)nf));
-      HIPCHK(hipMemsetAsync(ctx->fcnt.p, 0, sizeof(int) * 4 * (size_t)nf, st));
-    }
     HIPCHK(launch_seed_tiles(dl, ctx->regs.p, nf, nx, ntx, ntz, tx, tz, ctx->tin.p, ctx->tkey.p, ctx->tlist[0].p,
-                             per_field ? ctx->fcnt.p : ctx->tcount.p + 1, front[0], per_field ? 1 : 0, st));
+                             ctx->tcount.p + 1, front[0], st));
     constexpr int BATCH = 8;
     const int fim_grid = 8192, sched_grid = 512;
     const int trace_cap = max_launch + BATCH;
@@ -462,96 +456,7 @@ int fmt2d_forward(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags) {
       return fail(FMT2D_ENOCONV, msg + ")");
     };
     int launches = 0;
-    if (per_field) {
-      // one cooperative dispatch, each field on its own launches
-      if ((size_t)nf * ntiles >= (size_t)1 << 29) return fail(FMT2D_EINVAL, "too many field tiles for the per-field schedule");
-      const int gi = f32 ? 3 : (tx == 16 ? 1 : (tz == 16 ? 2 : 0));
-      if (ctx->field_grid[gi] <= 0) {
-        const hipError_t e = field_grid(ctx->device, tx, tz, f32 ? 1 : 0, &ctx->field_grid[gi]);
-        if (e == hipErrorNotSupported) return fail(FMT2D_EHIP, "the device does not run cooperative launches");
-        HIPCHK(e);
-      }
-      const size_t qcap = (size_t)nf * ntiles;
-      if (ctx->fqueue.n < qcap) HIPCHK(ctx->fqueue.alloc(qcap));
-      HIPCHK(hipMemsetAsync(ctx->fqueue.p, 0xFF, sizeof(unsigned long long) * qcap, st));
-      // head, tail, done, failed, stalled and the visit count, each on a
-      // cache line of its own (every block polls the first three)
-      constexpr int LINE = 16;  // 128 bytes in 8-byte words
-      if (ctx->fctl.n < 6 * LINE) HIPCHK(ctx->fctl.alloc(6 * LINE));
-      HIPCHK(hipMemsetAsync(ctx->fctl.p, 0, sizeof(unsigned long long) * 6 * LINE, st));
-      unsigned* ctl = reinterpret_cast<unsigned*>(ctx->fctl.p);
-      FieldSched a{};
-      a.descs = dl; a.regs = ctx->regs.p; a.heavy = ctx->theavy.p;
-      a.list[0] = ctx->tlist[0].p; a.list[1] = ctx->tlist[1].p;
-      a.cnt = ctx->fcnt.p; a.lno = ctx->fcnt.p + 2 * nf; a.pend = ctx->fcnt.p + 3 * nf;
-      a.queue = ctx->fqueue.p; a.stage = reinterpret_cast<unsigned*>(ctx->tlist[2].p); a.head = ctl; a.tail = ctl + 2 * LINE;
-      a.done = reinterpret_cast<int*>(ctl + 4 * LINE); a.failed = reinterpret_cast<int*>(ctl + 6 * LINE);
-      a.stalled = reinterpret_cast<int*>(ctl + 8 * LINE);
-      a.visits = ctx->fctl.p + 5 * LINE;
-      a.in_set = ctx->tin.p; a.tkey = ctx->tkey.p; a.front[0] = front[0]; a.front[1] = front[1];
-      a.acc = ctx->acc.p; a.flag = ctx->stop_flag.p; a.window = window;
-      a.nfield = nf; a.ntx = ntx; a.ntz = ntz; a.ntiles = ntiles; a.qcap = (int)qcap;
-      a.max_sweeps = ctx->max_sweeps; a.max_sweeps_heavy = ctx->max_sweeps_heavy; a.colors = ctx->colors;
-      a.max_launch = max_launch;
-      if (!ctx->fev[0]) {
-        HIPCHK(hipEventCreate(&ctx->fev[0]));
-        HIPCHK(hipEventCreate(&ctx->fev[1]));
-      }
-      HIPCHK(hipEventRecord(ctx->fev[0], st));
-      HIPCHK(launch_fim_field(a, tx, tz, f32 ? 1 : 0, ctx->field_grid[gi], st));
-      HIPCHK(hipEventRecord(ctx->fev[1], st));
-      nev += 1;
-      unsigned long long hc[6 * LINE];
-      HIPCHK(hipMemcpyAsync(hc, ctx->fctl.p, sizeof(hc), hipMemcpyDeviceToHost, st));
-      std::vector<int> lno(nf);
-      HIPCHK(hipMemcpyAsync(lno.data(), ctx->fcnt.p + 2 * nf, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, ctx->fev[0], ctx->fev[1]));
-      fim_ms += ms;
-      const unsigned* c32 = reinterpret_cast<const unsigned*>(hc);
-      if (c32[8 * LINE]) {
-        std::vector<int> fc(4 * (size_t)nf);
-        HIPCHK(hipMemcpy(fc.data(), ctx->fcnt.p, sizeof(int) * fc.size(), hipMemcpyDeviceToHost));
-        std::string msg = std::string("the per-field tile schedule ") +
-                          (c32[8 * LINE] == 2 ? "overran its queue" : "stopped moving") + " (level " +
-                          std::to_string(L) + ": " + std::to_string((int)c32[4 * LINE]) + " of " + std::to_string(nf) +
-                          " fields done, queue head " + std::to_string(c32[0]) + " tail " +
-                          std::to_string(c32[2 * LINE]) + "; fields with tiles pending:";
-        int shown = 0;
-        for (int f = 0; f < nf && shown < 6; f++)
-          if (fc[3 * nf + f] != 0) {
-            msg += " f" + std::to_string(f) + " pend " + std::to_string(fc[3 * nf + f]) + " launch " +
-                   std::to_string(fc[2 * nf + f]) + " lists " + std::to_string(fc[2 * f]) + "/" +
-                   std::to_string(fc[2 * f + 1]);
-            shown++;
-          }
-        return fail(FMT2D_EHIP, msg + ")");
-      }
-      if (c32[6 * LINE]) {
-        const int fbad = (int)c32[6 * LINE] - 1;
-        const int k = lno[fbad];
-        int n = 0;
-        HIPCHK(hipMemcpy(&n, ctx->fcnt.p + 2 * fbad + (k & 1), sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<int2> tail(std::min(n, 8));
-        if (!tail.empty())
-          HIPCHK(hipMemcpy(tail.data(), ctx->tlist[k & 1].p + (size_t)fbad * ntiles, tail.size() * sizeof(int2),
-                           hipMemcpyDeviceToHost));
-        std::string msg = "eikonal tile solver did not converge within the launch cap (level " + std::to_string(L) +
-                          ", " + std::to_string(n) + " tiles active; field/tile:";
-        for (auto& p : tail) {
-          const int itf = order[f0 + p.x] % FMT2D_NTF + 1;
-          msg += " tf" + std::to_string(itf) + "/src" + std::to_string(order[f0 + p.x] / FMT2D_NTF) + "/t" +
-                 std::to_string(p.y) + "(" + std::to_string((p.y % ntx) * tx) + "," +
-                 std::to_string((p.y / ntx) * tz) + ")";
-        }
-        return fail(FMT2D_ENOCONV, msg + ")");
-      }
-      int most = 0;
-      for (int v : lno) most = std::max(most, v);
-      ctx->stats.launches += most;
-      ctx->stats.tiles += (long long)hc[5 * LINE];
-    } else if (ctx->solver_launch == FMT2D_LAUNCH_LEVEL) {
+    if (ctx->solver_launch == FMT2D_LAUNCH_LEVEL) {
       // one cooperative dispatch runs every launch of the level
       const int gi = f32 ? 3 : (tx == 16 ? 1 : (tz == 16 ? 2 : 0));
       if (ctx->level_grid[gi] <= 0) {
@@ -881,9 +786,8 @@ int fmt2d_autotune(fmt2d_ctx* ctx, int nsrc, const double* src_x, int flags, int
 }
 
 int fmt2d_set_solver_launch(fmt2d_ctx* ctx, int mode) {
-  if (!ctx || (mode != FMT2D_LAUNCH_GRAPH && mode != FMT2D_LAUNCH_LEVEL && mode != FMT2D_LAUNCH_FIELD))
-    return fail(FMT2D_EINVAL,
-                "the solver launch mode must be FMT2D_LAUNCH_GRAPH, FMT2D_LAUNCH_LEVEL or FMT2D_LAUNCH_FIELD");
+  if (!ctx || (mode != FMT2D_LAUNCH_GRAPH && mode != FMT2D_LAUNCH_LEVEL))
+    return fail(FMT2D_EINVAL, "the solver launch mode must be FMT2D_LAUNCH_GRAPH or FMT2D_LAUNCH_LEVEL");
   ctx->solver_launch = mode;
   return FMT2D_OK;
 }

@@ -135,11 +135,6 @@ struct fmt2d_ctx {
   int solver_launch = 0;              // FMT2D_LAUNCH_GRAPH or FMT2D_LAUNCH_LEVEL
   int level_grid[4] = {0, 0, 0, 0};   // cooperative grid of the level kernel (8x8, 16x8, 8x16, 8x8 FP32)
   DBuf<unsigned> tbar;                // its grid barrier's counters
-  int field_grid[4] = {0, 0, 0, 0};   // cooperative grid of the per-field kernel
-  DBuf<int> fcnt;                     // per field: list sizes (2), launches split, due tiles pending
-  DBuf<unsigned long long> fqueue;    // the per-field schedule's tile queue
-  DBuf<unsigned long long> fctl;      // its head, tail, done, failed, stalled; tile visits
-  hipEvent_t fev[2] = {nullptr, nullptr};
   size_t adj_cap_bytes = (size_t)4 << 30;  // adjoint sweep: edge records of at most this many bytes at once
   int lsqr_fast = 0;                  // LSQR reductions: 0 in the reference's order, 1 tree / wave reductions
   DBuf<unsigned long long> acc;       // sweep statistics [sum, max, capped visits]
@@ -239,8 +234,6 @@ struct fmt2d_ctx {
   std::vector<int> trace_sweeps;       // most sweeps of a tile visit per launch
   long long sweep_hist[64] = {};       // tile visits by sweep count (last bin: 64 or more)
   ~fmt2d_ctx() {
-    if (fev[0]) (void)hipEventDestroy(fev[0]);
-    if (fev[1]) (void)hipEventDestroy(fev[1]);
     if (h_count) (void)hipHostFree(h_count);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);

@@ -72,7 +72,7 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
                                    int nz, const IsecDev* isecs, hipStream_t st);
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
                              int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                             int* count_next, unsigned long long* front_next, int per_field, hipStream_t st);
+                             int* count_next, unsigned long long* front_next, hipStream_t st);
 // per launch: clear the due / next counters and the next list's field
 // fronts, then split the active list (its count on the device) into due
 // tiles and carry-over (front: the active list's fronts)
@@ -121,42 +121,6 @@ struct LevelSched {
 // blocks of the level kernel's cooperative grid (all resident on the device)
 hipError_t level_grid(int device, int tx, int tz, int f32, int* grid);
 size_t level_bar_words(int grid);
-// All launches of a dependency level, each field on its own (one cooperative
-// dispatch): per field f, lists list[0] / list[1] + f * ntiles with sizes
-// cnt[2 f] / cnt[2 f + 1] (list 0 seeded), its launches split so far lno[f]
-// and due tiles not yet visited pend[f] (both 0 at the start), fronts
-// front[0][f] (list 0's) and front[1][f] (no key); queue: qcap = nfield *
-// ntiles slots, all bytes 0xFF at the start; head, tail, done, failed,
-// stalled: 0.
-// failed: 1 + a field that reached the launch cap; visits: tile visits
-// (added to).
-struct FieldSched {
-  const FieldDesc* descs;
-  const RegionDev* regs;
-  const uint8_t* heavy;
-  int2* list[2];
-  int* cnt;
-  int* lno;
-  int* pend;
-  unsigned long long* queue;
-  unsigned* stage;  // nfield * ntiles: each field's launch entries before they are published
-  unsigned* head;
-  unsigned* tail;
-  int* done;
-  int* failed;
-  int* stalled;  // set when the schedule stopped moving (then every block leaves)
-  unsigned long long* visits;
-  unsigned int* in_set;
-  unsigned long long* tkey;
-  unsigned long long* front[2];
-  unsigned long long* acc;
-  int* flag;
-  double window;
-  int nfield, ntx, ntz, ntiles, qcap;
-  int max_sweeps, max_sweeps_heavy, colors, max_launch;
-};
-hipError_t field_grid(int device, int tx, int tz, int f32, int* grid);
-hipError_t launch_fim_field(const FieldSched& a, int tx, int tz, int f32, int grid, hipStream_t st);
 hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st);
 // per field: max |T - snap| over slots both reach, count of slots one
 // reaches, count of slots both reach with |T - snap| > tol

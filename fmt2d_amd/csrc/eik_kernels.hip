@@ -282,7 +282,7 @@ __device__ inline void add_tile(int f, int t, int ntiles, unsigned long long key
 __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
                                   int nfield, int ntx, int ntz, int tx, int tz, unsigned int* in_set,
                                   unsigned long long* tkey, int2* list_next, int* count_next,
-                                  unsigned long long* front_next, int per_field) {
+                                  unsigned long long* front_next) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const RegionDev& R0 = regs[0];
   const int nx = R0.nx, nz = R0.nz;
@@ -291,8 +291,6 @@ __global__ void seed_tiles_kernel(const FieldDesc* __restrict__ descs, const Reg
   const FieldDesc& d = descs[f];
   const RegionDev& R = regs[d.region];
   const int ntiles = ntx * ntz;
-  // per-field lists: field f's entries from f * ntiles, its count at 2 f
-  if (per_field) { list_next += (size_t)f * ntiles; count_next += 2 * f; }
   if (d.kind == SEED_POINT) {
     if (c != 0) return;
     if (d.fixed_key >= 0) {  // on-node source: alive from the start (:2366-2369)
@@ -878,218 +876,6 @@ __global__ void __launch_bounds__(64, 2) fim_level_kernel(LevelSched a) {
   }
 }
 
-// ---------------------------------------------------------------- per-field schedule
-//
-// The fields of a dependency level do not read each other, so nothing but
-// the schedule makes one field's launch wait for another's.  In the
-// per-field form each field runs its own launches: a launch's due tiles go
-// to one device-wide queue that every block of a cooperative grid takes
-// tiles from, and the block that finishes a field's last due tile splits
-// that field's next launch at once.  A field's launch then lasts as long as
-// its own slowest visit, not the slowest visit of any field, and a block
-// that finished early takes the next field's tiles instead of idling.
-//
-// Queue: a ring of nfield * ntiles slots holding (position << 32) | entry,
-// entry = f * ntiles + t, bit 29 the launch's parity, bit 30 the tile's
-// heavy flag.  At most one entry per (field, tile) is queued at a time (a
-// tile is pushed again only after its field's next split, which follows its
-// visit), so the ring never holds more than nfield * ntiles entries not
-// yet read.  Blocks take positions in order from a second counter (the
-// head) and wait for their slot.
-
-constexpr unsigned FQ_PARITY = 1u << 29, FQ_HEAVY = 1u << 30, FQ_TILE = FQ_PARITY - 1u;
-// 10 s of the 100 MHz constant clock
-constexpr unsigned long long FIELD_STALL_TICKS = 1000000000ull;
-
-// The split of field f's next launch, by one block (one wave): due tiles
-// (key within the window of the field's front, the launch's colour) to the
-// queue, cut-by-interface tiles first, the others carried into the field's
-// other list, whose front they set.  A launch with nothing due (the front's
-// tiles have the other colour) is followed by the next one at once.  The
-// launch's entries are staged (stage + f * ntiles) and published only after
-// every change of the field's state: once published, visits of the launch
-// run at once and activate tiles of the field.
-__device__ void field_split(const FieldSched& a, int f) {
-  const int lane = threadIdx.x;
-  const size_t base_t = (size_t)f * a.ntiles;
-  const int region = a.descs[f].region;
-  const uint8_t* hv_tab = a.heavy + (size_t)region * a.ntiles;
-  unsigned* stage = a.stage + base_t;
-  // the launch to split, its list's size and front (after a launch with
-  // nothing due, the next one's are the values just computed)
-  int k = load_count(a.lno + f);
-  int n = load_count(a.cnt + 2 * f + (k & 1));
-  double fr = __longlong_as_double((long long)__hip_atomic_load(a.front[k & 1] + f, __ATOMIC_RELAXED,
-                                                                  __HIP_MEMORY_SCOPE_AGENT));
-  for (;;) {
-    const int c = k & 1, nxt = c ^ 1;
-    if (n == 0 || k >= a.max_launch) {
-      if (lane == 0) {
-        if (n > 0) atomicExch(a.failed, f + 1);  // launch cap: the field stops with tiles left
-        atomicAdd(a.done, 1);
-      }
-      return;
-    }
-    const int2* in = a.list[c] + base_t;
-    int2* carry = a.list[nxt] + base_t;
-    // pass 1: how many are due, cut by an interface or not; the carried
-    // keys' minimum (the next list's front)
-    int nh = 0, nl = 0;
-    unsigned long long kmin = KEY_INF;
-    for (int i = lane; i < n; i += 64) {
-      const int t = in[i].y;
-      const unsigned long long kb = a.tkey[base_t + t];
-      const double key = __longlong_as_double((long long)kb);
-      const bool due = key <= fr + a.window && (((t % a.ntx) + (t / a.ntx)) & 1) == c;
-      const bool hv = due && hv_tab[t] != 0;
-      nh += hv ? 1 : 0;
-      nl += (due && !hv) ? 1 : 0;
-      if (!due) kmin = kb < kmin ? kb : kmin;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      nh += __shfl_xor(nh, o, 64);
-      nl += __shfl_xor(nl, o, 64);
-      const unsigned long long v = (unsigned long long)__shfl_xor((long long)kmin, o, 64);
-      kmin = v < kmin ? v : kmin;
-    }
-    const int nd = nh + nl, ncar = n - nd;
-    unsigned qbase = 0;
-    if (lane == 0) {
-      a.cnt[2 * f + nxt] = ncar;
-      a.front[nxt][f] = kmin;
-      a.pend[f] = nd;
-      a.lno[f] = k + 1;
-      if (nd > 0) qbase = atomicAdd(a.tail, (unsigned)nd);
-    }
-    qbase = (unsigned)__shfl((int)qbase, 0, 64);
-    // pass 2: due tiles leave the set and are staged, cut by an interface
-    // first; the others are carried
-    int hpos = 0, lpos = 0, cpos = 0;
-    for (int r = 0; r * 64 < n; r++) {
-      const int i = r * 64 + lane;
-      bool due = false, hv = false, is_carry = false;
-      int2 e = make_int2(0, 0);
-      if (i < n) {
-        e = in[i];
-        const double key = __longlong_as_double((long long)a.tkey[base_t + e.y]);
-        due = key <= fr + a.window && (((e.y % a.ntx) + (e.y / a.ntx)) & 1) == c;
-        hv = due && hv_tab[e.y] != 0;
-        is_carry = !due;
-      }
-      const unsigned long long mh = __ballot(hv), ml = __ballot(due && !hv), mc = __ballot(is_carry);
-      const unsigned long long below = (1ull << lane) - 1ull;
-      if (due) {
-        a.tkey[base_t + e.y] = KEY_INF;
-        a.in_set[base_t + e.y] = 0u;
-        const int j = hv ? hpos + __popcll(mh & below) : nh + lpos + __popcll(ml & below);
-        stage[j] = (unsigned)(base_t + e.y) | (c ? FQ_PARITY : 0u) | (hv ? FQ_HEAVY : 0u);
-      }
-      if (is_carry) carry[cpos + __popcll(mc & below)] = e;
-      hpos += __popcll(mh);
-      lpos += __popcll(ml);
-      cpos += __popcll(mc);
-    }
-    if (nd == 0) {
-      k = k + 1;
-      n = ncar;
-      fr = __longlong_as_double((long long)kmin);
-      continue;
-    }
-    // the launch's state before its entries: visits of this launch read it
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    // pass 3: publish the staged entries
-    for (int j = lane; j < nd; j += 64) {
-      const unsigned pos = qbase + (unsigned)j;
-      __hip_atomic_store(a.queue + pos % (unsigned)a.qcap, ((unsigned long long)pos << 32) | stage[j],
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return;
-  }
-}
-
-// The next queue entry for this block (lane 0 takes it, every lane gets
-// it); false once every field is done.  Lane 0 takes the next position
-// with one atomic add (positions are handed out in order, also ahead of
-// the tail: the block then waits for its entry to be written) and waits
-// for the slot to hold that position.  Every wait ends: an entry below the
-// tail was reserved by a block that writes it without waiting for
-// anything; a position at or past the final tail is never written, and its
-// block leaves once every field is done (a field that is not done always
-// has a launch with queued or running tiles, or a block splitting it).  A
-// slot already holding a later position (the ring came round before the
-// entry was read, which the ring's size rules out) and a schedule that
-// stopped moving (no position taken, no entry written and no field
-// finished for FIELD_STALL_TICKS of the constant clock) set the stall flag,
-// and every block leaves.
-__device__ bool field_pop(const FieldSched& a, unsigned& entry) {
-  int got = 0;
-  unsigned v = 0;
-  if (threadIdx.x == 0) {
-    const unsigned h = atomicAdd(a.head, 1u);
-    unsigned long long* slot = a.queue + h % (unsigned)a.qcap;
-    unsigned long long seen_at = wall_clock64();
-    unsigned last_h = 0xFFFFFFFFu, last_t = 0xFFFFFFFFu;
-    int last_d = -1, idle = 0;
-    for (;;) {
-      const unsigned long long q = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned seq = (unsigned)(q >> 32);
-      if (seq == h) { v = (unsigned)q; got = 1; break; }
-      if (seq != 0xFFFFFFFFu && (int)(seq - h) > 0) { atomicExch(a.stalled, 2); break; }
-      const int d = __hip_atomic_load(a.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (d >= a.nfield) break;
-      if (__hip_atomic_load(a.stalled, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      const unsigned hh = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned tt = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (hh != last_h || tt != last_t || d != last_d) {
-        last_h = hh; last_t = tt; last_d = d;
-        seen_at = wall_clock64();
-      } else if (wall_clock64() - seen_at > FIELD_STALL_TICKS) {
-        atomicExch(a.stalled, 1);
-        break;
-      }
-      // wait for the entry: briefly at first, longer while nothing comes
-      idle = idle < 6 ? idle + 1 : 6;
-      for (int i = 0; i < (1 << idle); i++) __builtin_amdgcn_s_sleep(4);
-    }
-  }
-  got = __shfl(got, 0, 64);
-  entry = (unsigned)__shfl((int)v, 0, 64);
-  // what the splitter and the field's earlier visits wrote, before the visit
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return got != 0;
-}
-
-template <int TX, int TZ, bool F32>
-__global__ void __launch_bounds__(64, 2) fim_field_kernel(FieldSched a) {
-  __shared__ TileLds<TX, TZ> L;
-  // every field's first launch, fields spread over the blocks
-  for (int f = blockIdx.x; f < a.nfield; f += gridDim.x) field_split(a, f);
-  unsigned long long visits = 0;
-  unsigned entry;
-  while (field_pop(a, entry)) {
-    const unsigned ft = entry & FQ_TILE;
-    const int f = (int)(ft / (unsigned)a.ntiles), t = (int)(ft % (unsigned)a.ntiles);
-    const int nxt = (entry & FQ_PARITY) ? 0 : 1;
-    fim_tile<TX, TZ, F32>(a.descs, a.regs, make_int2(f, t), a.ntx, a.ntz, a.in_set, a.tkey,
-                          a.list[nxt] + (size_t)f * a.ntiles, a.cnt + 2 * f + nxt, a.front[nxt],
-                          (entry & FQ_HEAVY) ? a.max_sweeps_heavy : a.max_sweeps, a.colors, a.acc, nullptr, a.flag,
-                          L);
-    __syncthreads();
-    visits++;
-    // the visit's writes before the count that lets the field's next split start
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    int last = 0;
-    if (threadIdx.x == 0)
-      last = __hip_atomic_fetch_add(a.pend + f, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1;
-    last = __shfl(last, 0, 64);
-    if (last) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      field_split(a, f);
-    }
-  }
-  if (threadIdx.x == 0) atomicAdd(a.visits, visits);
-}
-
 // Post-pass: refit one-sided gradients of grid nodes; everything else keeps
 // the raw gradient.  One thread per (field, slot).
 __global__ void refit_gradients_kernel(const FieldDesc* __restrict__ descs, const RegionDev* __restrict__ regs,
@@ -1590,10 +1376,10 @@ hipError_t launch_transfer_reflect(const FieldDesc* descs_all, int f0, const Sta
 
 hipError_t launch_seed_tiles(const FieldDesc* descs, const RegionDev* regs, int nfield, int nx, int ntx, int ntz,
                              int tx, int tz, unsigned int* in_set, unsigned long long* tkey, int2* list_next,
-                             int* count_next, unsigned long long* front_next, int per_field, hipStream_t st) {
+                             int* count_next, unsigned long long* front_next, hipStream_t st) {
   const int total = nfield * nx;
   hipLaunchKernelGGL(seed_tiles_kernel, dim3((total + 255) / 256), dim3(256), 0, st, descs, regs, nfield, ntx, ntz,
-                     tx, tz, in_set, tkey, list_next, count_next, front_next, per_field);
+                     tx, tz, in_set, tkey, list_next, count_next, front_next);
   return hipGetLastError();
 }
 
@@ -1745,37 +1531,6 @@ hipError_t level_grid(int device, int tx, int tz, int f32, int* grid) {
 
 size_t level_bar_words(int grid) {
   return (size_t)LVL_BAR_STRIDE * (1 + (grid + LVL_BAR_GROUP - 1) / LVL_BAR_GROUP);
-}
-
-static const void* field_kernel(int tx, int tz, int f32) {
-  if (f32) return tx == 8 && tz == 8 ? (const void*)fim_field_kernel<8, 8, true> : nullptr;
-  if (tx == 8 && tz == 8) return (const void*)fim_field_kernel<8, 8, false>;
-  if (tx == 16 && tz == 8) return (const void*)fim_field_kernel<16, 8, false>;
-  if (tx == 8 && tz == 16) return (const void*)fim_field_kernel<8, 16, false>;
-  return nullptr;
-}
-
-hipError_t field_grid(int device, int tx, int tz, int f32, int* grid) {
-  const void* k = field_kernel(tx, tz, f32);
-  if (!k) return hipErrorInvalidValue;
-  int coop = 0, ncu = 0, per_cu = 0;
-  hipError_t e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
-  if (e != hipSuccess) return e;
-  if (!coop) return hipErrorNotSupported;
-  e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-  if (e != hipSuccess) return e;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0);
-  if (e != hipSuccess) return e;
-  *grid = per_cu * ncu;
-  return *grid > 0 ? hipSuccess : hipErrorInvalidConfiguration;
-}
-
-hipError_t launch_fim_field(const FieldSched& a, int tx, int tz, int f32, int grid, hipStream_t st) {
-  const void* k = field_kernel(tx, tz, f32);
-  if (!k) return hipErrorInvalidValue;
-  FieldSched arg = a;
-  void* args[] = {&arg};
-  return hipLaunchCooperativeKernel(k, dim3(grid), dim3(64), args, 0, st);
 }
 
 hipError_t launch_fim_level(const LevelSched& a, int tx, int tz, int f32, int grid, hipStream_t st) {

@@ -287,9 +287,9 @@ class Context:
         of launches replayed as HIP graphs) or "level" (one cooperative
         dispatch per stage level, its launches separated by grid-wide
         barriers).  Both reach the same fields bit for bit."""
-        modes = {"graph": 0, "level": 1, "field": 2}
+        modes = {"graph": 0, "level": 1}
         if mode not in modes:
-            raise ValueError(f"solver launch mode {mode!r}: 'graph', 'level' or 'field'")
+            raise ValueError(f"solver launch mode {mode!r}: 'graph' or 'level'")
         _check(self._lib.fmt2d_set_solver_launch(self._h, modes[mode]))
 
     def stats(self) -> ForwardStats:

@@ -178,7 +178,6 @@ int fmt2d_set_launch_cap(fmt2d_ctx* ctx, int max_launches);
  * of this build, no counterpart in the reference.) */
 #define FMT2D_LAUNCH_GRAPH 0
 #define FMT2D_LAUNCH_LEVEL 1
-#define FMT2D_LAUNCH_FIELD 2
 int fmt2d_set_solver_launch(fmt2d_ctx* ctx, int mode);
 /* Due-tile count and the most sweeps of one tile visit, per solver launch of
  * the last forward (diagnostics); either array may be NULL. */
