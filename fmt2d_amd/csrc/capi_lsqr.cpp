@@ -211,7 +211,7 @@ int fmt2d_build_system(fmt2d_ctx* ctx, const double* t_obs, const fmt2d_system_p
   HIPCHK(ctx->s_crow.alloc((size_t)std::max(nz_, 1)));
   HIPCHK(ctx->s_cval.alloc((size_t)std::max(nz_, 1)));
   HIPCHK(build_csc(ctx->s_rp.p, ctx->s_col.p, ctx->s_val.p, nd, nparm, nz_, ctx->s_cp.p, ctx->s_crow.p,
-                   ctx->s_cval.p, st));
+                   ctx->s_cval.p, ctx->csr_work, st));
   HIPCHK(ctx->s_gtg.alloc((size_t)std::max(nparm, 1)));
   HIPCHK(launch_gtg(ctx->s_cp.p, ctx->s_cval.p, nparm, ctx->s_gtg.p, st));
   ctx->h_gtg.resize(nparm);

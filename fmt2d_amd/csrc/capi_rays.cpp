@@ -262,17 +262,20 @@ static int assemble_rows(fmt2d_ctx* ctx, int nrow, const std::vector<long long>&
     std::vector<long long> ho(std::max(nr, 1), 0);
     long long total = 0;
     for (int r = 0; r < nr; r++) { ho[r] = total; total += cnt[r0 + r]; }
-    DBuf<long long> coff;
-    DBuf<unsigned long long> keys;
-    DBuf<double> vals;
-    hipError_t e = upload(coff, ho, st);
-    if (e == hipSuccess) e = keys.alloc((size_t)std::max<long long>(total, 1));
-    if (e == hipSuccess) e = vals.alloc((size_t)std::max<long long>(total, 1));
-    if (e == hipSuccess) e = emit(r0, nr, coff.p, keys.p, vals.p);
+    // the chunk's offsets and summands in scratch kept between assemblies
+    long long* coff = nullptr;
+    unsigned long long* keys = nullptr;
+    double* vals = nullptr;
+    hipError_t e = ctx->asm_work.get(0, ho.size(), &coff);
+    if (e == hipSuccess) e = hipMemcpyAsync(coff, ho.data(), sizeof(long long) * ho.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // ho is a local vector
+    if (e == hipSuccess) e = ctx->asm_work.get(1, (size_t)std::max<long long>(total, 1), &keys);
+    if (e == hipSuccess) e = ctx->asm_work.get(2, (size_t)std::max<long long>(total, 1), &vals);
+    if (e == hipSuccess) e = emit(r0, nr, coff, keys, vals);
     int rbits = 1;
     while ((1LL << rbits) < nr) rbits++;
     CsrOut co;
-    if (e == hipSuccess) e = build_csr(keys.p, vals.p, total, nr, 32 + rbits, 0, co, st);
+    if (e == hipSuccess) e = build_csr(keys, vals, total, nr, 32 + rbits, 0, co, ctx->csr_work, st);
     parts.push_back(co);
     if (e != hipSuccess) {
       free_parts();
@@ -402,7 +405,7 @@ int fmt2d_frechet_apply(fmt2d_ctx* ctx, int transpose, const double* x, double* 
     HIPCHK(ctx->gt_row.alloc((size_t)std::max(nnz, 1)));
     HIPCHK(ctx->gt_val.alloc((size_t)std::max(nnz, 1)));
     HIPCHK(build_csc(ctx->g_rp.p, ctx->g_col.p, ctx->g_val.p, nray, np, nnz, ctx->gt_cp.p, ctx->gt_row.p,
-                     ctx->gt_val.p, st));
+                     ctx->gt_val.p, ctx->csr_work, st));
     ctx->gt_ok = true;
   }
   const int nin = transpose ? nray : np, nout = transpose ? np : nray;

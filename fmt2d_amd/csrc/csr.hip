@@ -93,14 +93,14 @@ hipError_t launch_prolong_rows(const int* rp, const int* col, const double* val,
 #define CK(x) do { hipError_t _e = (x); if (_e != hipSuccess) { err = _e; goto done; } } while (0)
 
 hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nray, int key_bits, int base,
-                     CsrOut& out, hipStream_t st) {
+                     CsrOut& out, CsrWork& w, hipStream_t st) {
   hipError_t err = hipSuccess;
   unsigned long long* k2 = nullptr;
   double* v2 = nullptr;
   int *head = nullptr, *rid = nullptr, *keep = nullptr, *pos = nullptr;
   long long* start = nullptr;
   double* sum = nullptr;
-  void* tmp = nullptr;
+  unsigned char* tmp = nullptr;
   size_t tb = 0, tb2 = 0;
   int nrun = 0, nnz = 0;
   int hl[2];
@@ -112,13 +112,13 @@ hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nr
     CK(dalloc(&out.val, 1));
     goto done;
   }
-  CK(dalloc(&k2, n));
-  CK(dalloc(&v2, n));
+  CK(w.get(0, n, &k2));
+  CK(w.get(1, n, &v2));
   CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, k2, vals, v2, n, 0, key_bits, st));
-  CK(dalloc(&head, n));
-  CK(dalloc(&rid, n));
+  CK(w.get(2, n, &head));
+  CK(w.get(3, n, &rid));
   CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, head, rid, n, st));
-  CK(hipMalloc(&tmp, std::max(tb, tb2)));
+  CK(w.get(4, std::max(tb, tb2), &tmp));
   CK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, k2, vals, v2, n, 0, key_bits, st));
   hipLaunchKernelGGL(run_heads_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, k2, n, head);
   CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, head, rid, n, st));
@@ -126,10 +126,10 @@ hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nr
   CK(hipMemcpyAsync(&hl[1], head + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
   nrun = hl[0] + hl[1];
-  CK(dalloc(&start, nrun));
-  CK(dalloc(&sum, nrun));
-  CK(dalloc(&keep, nrun));
-  CK(dalloc(&pos, nrun));
+  CK(w.get(5, nrun, &start));
+  CK(w.get(6, nrun, &sum));
+  CK(w.get(7, nrun, &keep));
+  CK(w.get(8, nrun, &pos));
   hipLaunchKernelGGL(run_start_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, head, rid, n, start);
   // row counts in rowptr[0..nray), scanned below
   hipLaunchKernelGGL(run_sum_kernel, dim3((nrun + 255) / 256), dim3(256), 0, st, k2, v2, start, nrun, n, sum, keep,
@@ -137,12 +137,7 @@ hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nr
   {
     size_t tb3 = 0;
     CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, keep, pos, nrun, st));
-    if (tb3 > std::max(tb, tb2)) {
-      CK(hipFree(tmp));
-      tmp = nullptr;
-      CK(hipMalloc(&tmp, tb3));
-      tb = tb3;
-    }
+    CK(w.get(4, tb3, &tmp));
     CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb3, keep, pos, nrun, st));
     CK(hipMemcpyAsync(&hl[0], pos + nrun - 1, sizeof(int), hipMemcpyDeviceToHost, st));
     CK(hipMemcpyAsync(&hl[1], keep + nrun - 1, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -157,25 +152,14 @@ hipError_t build_csr(unsigned long long* keys, double* vals, long long n, int nr
     size_t tb4 = 0;
     CK(dalloc(&rp, (size_t)nray + 1));
     CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, out.rowptr, rp, nray + 1, st));
-    void* tmp4 = nullptr;
-    CK(hipMalloc(&tmp4, tb4 ? tb4 : 1));
-    CK(hipcub::DeviceScan::ExclusiveSum(tmp4, tb4, out.rowptr, rp, nray + 1, st));
+    CK(w.get(4, tb4, &tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb4, out.rowptr, rp, nray + 1, st));
     CK(hipStreamSynchronize(st));
-    (void)hipFree(tmp4);
     (void)hipFree(out.rowptr);
     out.rowptr = rp;
   }
   out.nnz = nnz;
 done:
-  if (k2) (void)hipFree(k2);
-  if (v2) (void)hipFree(v2);
-  if (head) (void)hipFree(head);
-  if (rid) (void)hipFree(rid);
-  if (start) (void)hipFree(start);
-  if (sum) (void)hipFree(sum);
-  if (keep) (void)hipFree(keep);
-  if (pos) (void)hipFree(pos);
-  if (tmp) (void)hipFree(tmp);
   return err;
 }
 
@@ -205,15 +189,15 @@ __global__ void csc_gather_kernel(const int* __restrict__ perm, const int* __res
 }
 
 hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow, int ncol, int nnz, int* cp, int* crow,
-                     double* cval, hipStream_t st) {
+                     double* cval, CsrWork& w, hipStream_t st) {
   hipError_t err = hipSuccess;
   int *erow = nullptr, *eidx = nullptr, *kout = nullptr, *perm = nullptr, *cnt = nullptr;
-  void* tmp = nullptr;
+  unsigned char* tmp = nullptr;
   size_t tb = 0, tb2 = 0;
   int bits = 1;
   while ((1 << bits) < ncol + 1) bits++;
-  CK(dalloc(&erow, nnz)); CK(dalloc(&eidx, nnz)); CK(dalloc(&kout, nnz)); CK(dalloc(&perm, nnz));
-  CK(dalloc(&cnt, (size_t)ncol + 1));
+  CK(w.get(0, nnz, &erow)); CK(w.get(1, nnz, &eidx)); CK(w.get(2, nnz, &kout)); CK(w.get(3, nnz, &perm));
+  CK(w.get(5, (size_t)ncol + 1, &cnt));
   CK(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)ncol + 1), st));
   if (nnz > 0) {
     hipLaunchKernelGGL(entry_rows_kernel, dim3((nrow + 127) / 128), dim3(128), 0, st, rp, nrow, erow, eidx);
@@ -221,19 +205,13 @@ hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow,
     CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, col, kout, eidx, perm, nnz, 0, bits, st));
   }
   CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt, cp, ncol + 1, st));
-  CK(hipMalloc(&tmp, std::max<size_t>(std::max(tb, tb2), 1)));
+  CK(w.get(4, std::max<size_t>(std::max(tb, tb2), 1), &tmp));
   if (nnz > 0) CK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, col, kout, eidx, perm, nnz, 0, bits, st));
   CK(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt, cp, ncol + 1, st));
   if (nnz > 0)
     hipLaunchKernelGGL(csc_gather_kernel, dim3((nnz + 255) / 256), dim3(256), 0, st, perm, erow, val, nnz, crow, cval);
   CK(hipStreamSynchronize(st));
 done:
-  if (erow) (void)hipFree(erow);
-  if (eidx) (void)hipFree(eidx);
-  if (kout) (void)hipFree(kout);
-  if (perm) (void)hipFree(perm);
-  if (cnt) (void)hipFree(cnt);
-  if (tmp) (void)hipFree(tmp);
   return err;
 }
 

@@ -168,6 +168,8 @@ struct fmt2d_ctx {
   std::vector<RayJob> hjobs;
   std::vector<long long> hpoff;
   DBuf<RayJob> rjobs;
+  CsrWork asm_work;                   // row assembly: chunk offsets and summands (grown, kept)
+  CsrWork csr_work;                   // scratch of build_csr / build_csc (grown, kept)
   DBuf<int> rperm;                    // trace order of the rays (longest first; by source and path within)
   std::vector<int> hperm;
   std::vector<int> hwork;             // interpolations per ray in the last trace of hjobs

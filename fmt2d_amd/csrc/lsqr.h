@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "csr.h"
+
 namespace fmt2d {
 
 struct LsqrScalars {
@@ -64,8 +66,9 @@ hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double
 hipError_t launch_ordered_spmv(const int* ptr, const int* idx, const double* val, int nrow, const double* x,
                                double* y, hipStream_t st);
 hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st);
-// column-major copy of a CSR matrix (stable: rows ascending within a column)
+// column-major copy of a CSR matrix (stable: rows ascending within a
+// column); its scratch comes from w
 hipError_t build_csc(const int* rp, const int* col, const double* val, int nrow, int ncol, int nnz, int* cp, int* crow,
-                     double* cval, hipStream_t st);
+                     double* cval, CsrWork& w, hipStream_t st);
 
 }  // namespace fmt2d

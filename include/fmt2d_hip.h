@@ -215,8 +215,9 @@ int fmt2d_set_inversion(fmt2d_ctx* ctx, int invert_velocity, int n_iface, const 
 /* Device memory bound of the Frechet-row and G P assembly: at most
  * max_summands (ray, node) contributions are sorted at once (about 40 bytes
  * each; default 2^28); rows are assembled in chunks of whole rays, so the
- * result does not depend on it.  The rows of the last trace are assembled
- * again on next use.  (No counterpart in the reference, which accumulates
+ * result does not depend on it.  The assembly's scratch (up to the chunk's
+ * ~40 bytes per summand) stays allocated in the context between assemblies.
+ * The rows of the last trace are assembled again on next use.  (No counterpart in the reference, which accumulates
  * each ray into a dense row, inversion2d.f90:4573-4577.) */
 int fmt2d_set_frechet_chunk(fmt2d_ctx* ctx, long long max_summands);
 /* Inversion mesh coarser than the propagation grid (no counterpart in the
