@@ -285,9 +285,9 @@ static int assemble_rows(fmt2d_ctx* ctx, int nrow, const std::vector<long long>&
   rp_out.release(); col_out.release(); val_out.release();
   if (parts.size() == 1) {
     const CsrOut& co = parts[0];
-    rp_out.p = co.rowptr; rp_out.n = (size_t)nrow + 1;
-    col_out.p = co.col; col_out.n = (size_t)std::max(co.nnz, 1);
-    val_out.p = co.val; val_out.n = (size_t)std::max(co.nnz, 1);
+    rp_out.p = co.rowptr; rp_out.n = rp_out.cap = (size_t)nrow + 1;
+    col_out.p = co.col; col_out.n = col_out.cap = (size_t)std::max(co.nnz, 1);
+    val_out.p = co.val; val_out.n = val_out.cap = (size_t)std::max(co.nnz, 1);
     nnz_out = co.nnz;
     parts.clear();
     return FMT2D_OK;

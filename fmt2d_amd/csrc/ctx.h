@@ -31,16 +31,25 @@ namespace {
 template <typename T>
 struct DBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0;    // elements in use
+  size_t cap = 0;  // elements allocated: a request within it keeps the allocation
   hipError_t alloc(size_t count) {
+    if (p && count <= cap) {
+      n = count;
+      return hipSuccess;
+    }
     release();
+    const hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; return e; }
     n = count;
-    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+    cap = std::max<size_t>(count, 1);
+    return hipSuccess;
   }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    cap = 0;
   }
   ~DBuf() { release(); }
 };
