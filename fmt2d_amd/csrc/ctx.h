@@ -128,7 +128,7 @@ struct fmt2d_ctx {
   double clock_hz = 0.0;              // rate of the device's constant clock (wall_clock64)
   DBuf<uint8_t> theavy;               // per (region, tile): cut by an interface (slow visits)
   double window_tiles = 3.0;          // window in tile crossing times
-  int max_sweeps = 8;                 // a visit over the cap stays active and continues later
+  int max_sweeps = 16;                // a visit over the cap stays active and continues later
   int max_sweeps_heavy = 8;           // the cap for tiles cut by an interface
   int colors = 2;                     // Gauss-Seidel colours in a tile (4, 2) or 1: Jacobi
   int tile_x = 8, tile_z = 8;         // tile shape of the solver (8x8, 16x8, 8x16)

@@ -235,7 +235,7 @@ class Context:
         self._nray = 0   # a forward drops the rays of the last trace
         _check(self._lib.fmt2d_forward(self._h, len(xs), _p(xs), flags))
 
-    def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 8, colors: int = 2):
+    def set_tuning(self, window_tiles: float = 3.0, max_sweeps: int = 16, colors: int = 2):
         _check(self._lib.fmt2d_set_tuning(self._h, float(window_tiles), int(max_sweeps), int(colors)))
 
     def set_heavy_sweeps(self, max_sweeps_heavy: int = 8):

@@ -114,7 +114,7 @@ int fmt2d_forward_stats_get(fmt2d_ctx* ctx, fmt2d_forward_stats* st);
 /* Solver tuning: arrival-time window of the tile schedule in tile crossing
  * times (at 4 km/s), the sweep cap of one tile visit, and the in-tile sweep
  * order (4: four-colour Gauss-Seidel, 2: two-colour Gauss-Seidel, 1: Jacobi).
- * Defaults 3.0, 8, 2. */
+ * Defaults 3.0, 16, 2. */
 int fmt2d_set_tuning(fmt2d_ctx* ctx, double window_tiles, int max_sweeps, int colors);
 /* Sweep cap of one visit of a tile cut by an interface (its visits are the
  * slow ones; a capped visit continues in a later launch; results are
