@@ -136,33 +136,96 @@ __global__ void gauss_fwd_kernel(const LsqrSys S, const double* __restrict__ x, 
 
 // front-to-back sum of a[i]^2 into *dst.  One block: waves 1..3 square
 // the next chunk into LDS while lane 0 of wave 0 adds the current chunk in
-// order (the dependent add chain is the cost; the loads hide behind it).
+// order.  The dependent add chain is the cost: the loaders issue all of a
+// chunk's loads before they wait, and the adder reads the next 8 squares
+// from LDS while it adds the current 8, so only the adds' latency shows.
 constexpr int SEQ_CHUNK = 2048;
+constexpr int SEQ_LOADERS = 192;
+
+__device__ inline void stage_squares(const double* __restrict__ a, int m, double* __restrict__ dst, int t) {
+  constexpr int PER = (SEQ_CHUNK + SEQ_LOADERS - 1) / SEQ_LOADERS;
+  double x[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int k = t + SEQ_LOADERS * i;
+    x[i] = k < m ? a[k] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int k = t + SEQ_LOADERS * i;
+    if (k < m) dst[k] = x[i] * x[i];
+  }
+}
+
+__device__ inline double2 lds_pair(const double* b, int k) { return *reinterpret_cast<const double2*>(b + k); }
+
+// 16 squares a group: the next group's reads are issued (and kept ahead of
+// the adds by the scheduling barrier) before the current group is added
+constexpr int SEQ_GROUP = 16;
+struct SeqGroup {
+  double2 p[SEQ_GROUP / 2];
+};
+__device__ inline SeqGroup read_group(const double* b, int k) {
+  SeqGroup g;
+#pragma unroll
+  for (int i = 0; i < SEQ_GROUP / 2; i++) g.p[i] = lds_pair(b, k + 2 * i);
+  return g;
+}
+__device__ inline double add_group(const SeqGroup& g, double acc) {
+#pragma unroll
+  for (int i = 0; i < SEQ_GROUP / 2; i++) {
+    acc = acc + g.p[i].x;
+    acc = acc + g.p[i].y;
+  }
+  return acc;
+}
+
+// two register sets in turn (no copies between them): g0 holds group
+// i - 1 at the top of the loop, groups i and i + 1 are read and added there
+__device__ inline double add_in_order(const double* b, int m, double acc) {
+  const int ng = m / SEQ_GROUP;
+  if (ng > 0) {
+    SeqGroup g0 = read_group(b, 0);
+    int i = 1;
+    for (; i + 1 < ng; i += 2) {
+      const SeqGroup g1 = read_group(b, i * SEQ_GROUP);
+      __builtin_amdgcn_sched_barrier(0);
+      acc = add_group(g0, acc);
+      g0 = read_group(b, (i + 1) * SEQ_GROUP);
+      __builtin_amdgcn_sched_barrier(0);
+      acc = add_group(g1, acc);
+    }
+    if (i < ng) {
+      const SeqGroup g1 = read_group(b, i * SEQ_GROUP);
+      __builtin_amdgcn_sched_barrier(0);
+      acc = add_group(g0, acc);
+      acc = add_group(g1, acc);
+    } else {
+      acc = add_group(g0, acc);
+    }
+  }
+  for (int k = ng * SEQ_GROUP; k < m; k++) acc = acc + b[k];
+  return acc;
+}
+
 __global__ void __launch_bounds__(256) seqnorm_kernel(const double* __restrict__ a, long long n, double* dst,
                                                      const LsqrScalars* s) {
   if (stopped(s)) return;
-  __shared__ double buf[2][SEQ_CHUNK];
+  __shared__ __align__(16) double buf[2][SEQ_CHUNK];
   const int t = threadIdx.x;
   const long long nch = (n + SEQ_CHUNK - 1) / SEQ_CHUNK;
   double acc = 0.0;
-  // first chunk
-  if (t >= 64 && nch > 0) {
-    const int m = (int)min((long long)SEQ_CHUNK, n);
-    for (int k = t - 64; k < m; k += 192) { const double x = a[k]; buf[0][k] = x * x; }
-  }
+  if (t >= 64 && nch > 0) stage_squares(a, (int)min((long long)SEQ_CHUNK, n), buf[0], t - 64);
   __syncthreads();
   for (long long c = 0; c < nch; c++) {
     const int cur = (int)(c & 1);
     if (t >= 64) {
       if (c + 1 < nch) {
         const long long base = (c + 1) * SEQ_CHUNK;
-        const int m = (int)min((long long)SEQ_CHUNK, n - base);
-        for (int k = t - 64; k < m; k += 192) { const double x = a[base + k]; buf[cur ^ 1][k] = x * x; }
+        stage_squares(a + base, (int)min((long long)SEQ_CHUNK, n - base), buf[cur ^ 1], t - 64);
       }
     } else if (t == 0) {
-      const int m = (int)min((long long)SEQ_CHUNK, n - c * SEQ_CHUNK);
-      const double* b = buf[cur];
-      for (int k = 0; k < m; k++) acc = acc + b[k];
+      acc = add_in_order(buf[cur], (int)min((long long)SEQ_CHUNK, n - c * SEQ_CHUNK), acc);
     }
     __syncthreads();
   }
