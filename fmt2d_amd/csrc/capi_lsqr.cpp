@@ -244,6 +244,20 @@ int fmt2d_set_lsqr_mode(fmt2d_ctx* ctx, int mode) {
   return FMT2D_OK;
 }
 
+int fmt2d_ordered_sumsq(fmt2d_ctx* ctx, const double* x, long long n, double* out) {
+  if (!ctx || !out || n < 0 || (n > 0 && !x)) return fail(FMT2D_EINVAL, "NULL argument or negative length");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  DBuf<double> dx, dsum;
+  HIPCHK(dx.alloc((size_t)n));
+  HIPCHK(dsum.alloc(1));
+  if (n > 0) HIPCHK(hipMemcpyAsync(dx.p, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_ordered_sumsq(dx.p, n, dsum.p, st));
+  HIPCHK(hipMemcpyAsync(out, dsum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return FMT2D_OK;
+}
+
 int fmt2d_lsqr(fmt2d_ctx* ctx, const fmt2d_lsqr_params* p, const double* dm_sum, double* dm_out,
                fmt2d_lsqr_stats* stats) {
   return lsqr_solve(ctx, p, dm_sum, dm_out, stats);

@@ -65,6 +65,9 @@ hipError_t launch_scale_rows(const int* src_rp, const int* src_col, const double
 // (one wave per row)
 hipError_t launch_ordered_spmv(const int* ptr, const int* idx, const double* val, int nrow, const double* x,
                                double* y, hipStream_t st);
+// front-to-back sum of a[i]^2 into *dst (device pointers), the ordered
+// norms' kernel on its own
+hipError_t launch_ordered_sumsq(const double* a, long long n, double* dst, hipStream_t st);
 hipError_t launch_gtg(const int* cp, const double* cval, int nparm, double* gtg, hipStream_t st);
 // column-major copy of a CSR matrix (stable: rows ascending within a
 // column); its scratch comes from w

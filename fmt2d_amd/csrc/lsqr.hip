@@ -13,7 +13,9 @@
 // nothing, so the iteration count is the reference's.  A context setting
 // (fmt2d_set_lsqr_mode) switches the norms and sparse products to tree and
 // wave reductions instead: faster, with rounding of their own.
+#include "ordered_sum.h"
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <math.h>
 
 #include <algorithm>
@@ -134,102 +136,184 @@ __global__ void gauss_fwd_kernel(const LsqrSys S, const double* __restrict__ x, 
   V[j] = v;
 }
 
-// front-to-back sum of a[i]^2 into *dst.  One block: waves 1..3 square
-// the next chunk into LDS while lane 0 of wave 0 adds the current chunk in
-// order.  The dependent add chain is the cost: the loaders issue all of a
-// chunk's loads before they wait, and the adder reads the next 8 squares
-// from LDS while it adds the current 8, so only the adds' latency shows.
-constexpr int SEQ_CHUNK = 2048;
-constexpr int SEQ_LOADERS = 192;
+// front-to-back sum of a[i]^2 into *dst, the same bits as the loop
+// s = s + a[i] * a[i], in the bulk form of ordered_sum.h.  The first 512
+// squares (where the sum changes binade every few terms) are added one by
+// one by lane 0 from LDS.  Then one block walks the vector in chunks of
+// 4096 squares (8 per thread, the next chunk's loads in flight meanwhile):
+// each thread forms its terms' mantissa increments at the running sum's
+// exponent, a block scan of the thread totals finds the first term that
+// changes the sum's binade, ties or is not finite; the terms before it are
+// added as one integer, that term the ordinary way, and the chunk goes on
+// after it.  Past the head a sum of squares changes binade a few times per
+// doubling of the term count, so nearly every chunk is one pass.
+constexpr int SEQ_THREADS = 512;
+constexpr int SEQ_PER = 8;
+constexpr int SEQ_CHUNK = SEQ_THREADS * SEQ_PER;
+constexpr int SEQ_WAVES = SEQ_THREADS / 64;
+constexpr int SEQ_HEAD = 512;
 
-__device__ inline void stage_squares(const double* __restrict__ a, int m, double* __restrict__ dst, int t) {
-  constexpr int PER = (SEQ_CHUNK + SEQ_LOADERS - 1) / SEQ_LOADERS;
-  double x[PER];
+// n >= 1; loads past the end read a[n - 1] and are replaced by 0 (no
+// branches around the loads, so that later waits count them exactly)
+__device__ __forceinline__ void seq_load(const double* __restrict__ a, long long n, long long b0,
+                                         double (&r)[SEQ_PER]) {
 #pragma unroll
-  for (int i = 0; i < PER; i++) {
-    const int k = t + SEQ_LOADERS * i;
-    x[i] = k < m ? a[k] : 0.0;
-  }
+  for (int i = 0; i < SEQ_PER; i++) r[i] = __builtin_nontemporal_load(a + min(b0 + i, n - 1));
 #pragma unroll
-  for (int i = 0; i < PER; i++) {
-    const int k = t + SEQ_LOADERS * i;
-    if (k < m) dst[k] = x[i] * x[i];
-  }
+  for (int i = 0; i < SEQ_PER; i++) r[i] = b0 + i < n ? r[i] : 0.0;
 }
 
-__device__ inline double2 lds_pair(const double* b, int k) { return *reinterpret_cast<const double2*>(b + k); }
-
-// 16 squares a group: the next group's reads are issued (and kept ahead of
-// the adds by the scheduling barrier) before the current group is added
-constexpr int SEQ_GROUP = 16;
-struct SeqGroup {
-  double2 p[SEQ_GROUP / 2];
+struct SeqShared {
+  double wtot[2][SEQ_WAVES];  // by pass parity: one barrier per pass
+  int wbad[2][SEQ_WAVES];
+  int wbrk[SEQ_WAVES];
+  double acc;
 };
-__device__ inline SeqGroup read_group(const double* b, int k) {
-  SeqGroup g;
+
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int i = 0; i < SEQ_GROUP / 2; i++) g.p[i] = lds_pair(b, k + 2 * i);
-  return g;
-}
-__device__ inline double add_group(const SeqGroup& g, double acc) {
-#pragma unroll
-  for (int i = 0; i < SEQ_GROUP / 2; i++) {
-    acc = acc + g.p[i].x;
-    acc = acc + g.p[i].y;
-  }
-  return acc;
+  for (int o = 32; o >= 1; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
 }
 
-// two register sets in turn (no copies between them): g0 holds group
-// i - 1 at the top of the loop, groups i and i + 1 are read and added there
-__device__ inline double add_in_order(const double* b, int m, double acc) {
-  const int ng = m / SEQ_GROUP;
-  if (ng > 0) {
-    SeqGroup g0 = read_group(b, 0);
-    int i = 1;
-    for (; i + 1 < ng; i += 2) {
-      const SeqGroup g1 = read_group(b, i * SEQ_GROUP);
-      __builtin_amdgcn_sched_barrier(0);
-      acc = add_group(g0, acc);
-      g0 = read_group(b, (i + 1) * SEQ_GROUP);
-      __builtin_amdgcn_sched_barrier(0);
-      acc = add_group(g1, acc);
-    }
-    if (i < ng) {
-      const SeqGroup g1 = read_group(b, i * SEQ_GROUP);
-      __builtin_amdgcn_sched_barrier(0);
-      acc = add_group(g0, acc);
-      acc = add_group(g1, acc);
-    } else {
-      acc = add_group(g0, acc);
-    }
-  }
-  for (int k = ng * SEQ_GROUP; k < m; k++) acc = acc + b[k];
-  return acc;
-}
-
-__global__ void __launch_bounds__(256) seqnorm_kernel(const double* __restrict__ a, long long n, double* dst,
-                                                     const LsqrScalars* s) {
-  if (stopped(s)) return;
-  __shared__ __align__(16) double buf[2][SEQ_CHUNK];
-  const int t = threadIdx.x;
-  const long long nch = (n + SEQ_CHUNK - 1) / SEQ_CHUNK;
-  double acc = 0.0;
-  if (t >= 64 && nch > 0) stage_squares(a, (int)min((long long)SEQ_CHUNK, n), buf[0], t - 64);
-  __syncthreads();
-  for (long long c = 0; c < nch; c++) {
-    const int cur = (int)(c & 1);
-    if (t >= 64) {
-      if (c + 1 < nch) {
-        const long long base = (c + 1) * SEQ_CHUNK;
-        stage_squares(a + base, (int)min((long long)SEQ_CHUNK, n - base), buf[cur ^ 1], t - 64);
+// One chunk [c0, c0 + SEQ_CHUNK) of squares, this thread's in r (squared
+// here; +0 past the end).  False when the sum has become inf or NaN: then
+// thread 0 has added the rest one by one and written *dst.  Positions are
+// chunk offsets: this thread's terms are j0 .. j0 + SEQ_PER - 1, terms
+// before lo are summed already, terms from lim on are past the end.
+__device__ __forceinline__ bool seq_chunk(const double* __restrict__ a, long long n, long long c0,
+                                          const double (&r)[SEQ_PER], double& acc, double* dst, SeqShared& sh,
+                                          int& parity) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int j0 = t * SEQ_PER;
+  const int lim = (int)min(n - c0, (long long)SEQ_CHUNK);
+  double x[SEQ_PER];
+#pragma unroll
+  for (int i = 0; i < SEQ_PER; i++) x[i] = r[i] * r[i];
+  int lo = 0;
+  for (;;) {
+    if (!isfinite(acc)) {  // inf or NaN stays so: the rest one by one
+      if (t == 0) {
+        for (long long k = c0 + lo; k < n; k++) {
+          const double v = a[k];
+          acc = acc + v * v;
+        }
+        *dst = acc;
       }
-    } else if (t == 0) {
-      acc = add_in_order(buf[cur], (int)min((long long)SEQ_CHUNK, n - c * SEQ_CHUNK), acc);
+      return false;
+    }
+    const SumBase B = sum_base(acc);
+    const double room = sum_room(B);
+    double d[SEQ_PER];
+    unsigned ok = 0;
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) {
+      d[i] = 0.0;
+      if (j0 + i < lo || sum_incr(x[i], B.e, &d[i])) ok |= 1u << i;
+    }
+    // the thread's total up to its first term that must go the ordinary way
+    double tot = 0.0;
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) {
+      if (!((ok >> i) & 1)) break;
+      tot = tot + d[i];
+    }
+    const bool bad = ok != (1u << SEQ_PER) - 1;
+    const double wt = wave_sum(tot);
+    const int wb = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+      sh.wtot[parity][w] = wt;
+      sh.wbad[parity][w] = wb;
     }
     __syncthreads();
+    double total = 0.0;
+    int anybad = 0;
+    for (int v = 0; v < SEQ_WAVES; v++) {
+      total = total + sh.wtot[parity][v];
+      anybad |= sh.wbad[parity][v];
+    }
+    if (!anybad && total < room) {  // every term in bulk (total exact below 2^53)
+      acc = sum_value(B.m + (uint64_t)total, B.e);
+      parity ^= 1;
+      return true;
+    }
+    // locate the first term that breaks: exclusive prefix of the totals
+    double inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc = inc + y;
+    }
+    double P = __shfl_up(inc, 1, 64);
+    if (lane == 0) P = 0.0;
+    for (int v = 0; v < w; v++) P = P + sh.wtot[parity][v];
+    int mine = INT_MAX;
+    double before = 0.0, xb = 0.0;
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) {
+      if (mine != INT_MAX || j0 + i < lo || j0 + i >= lim) continue;
+      const double Q = P + d[i];
+      if (!((ok >> i) & 1) || Q >= room) {
+        mine = j0 + i;
+        before = P;
+        xb = x[i];
+      } else {
+        P = Q;
+      }
+    }
+    int wm = mine;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) wm = min(wm, __shfl_xor(wm, o, 64));
+    if (lane == 0) sh.wbrk[w] = wm;
+    __syncthreads();
+    int kb = sh.wbrk[0];
+    for (int v = 1; v < SEQ_WAVES; v++) kb = min(kb, sh.wbrk[v]);
+    if (mine == kb) sh.acc = sum_value(B.m + (uint64_t)before, B.e) + xb;
+    __syncthreads();
+    acc = sh.acc;
+    parity ^= 1;
+    lo = kb + 1;
+    if (lo >= lim) return true;
+  }
+}
+
+__global__ void __launch_bounds__(SEQ_THREADS) seqnorm_kernel(const double* __restrict__ a, long long n, double* dst,
+                                                             const LsqrScalars* s) {
+  if (stopped(s)) return;
+  __shared__ double head[SEQ_HEAD];
+  __shared__ SeqShared sh;
+  const int t = threadIdx.x;
+  const long long nh = min(n, (long long)SEQ_HEAD);
+  for (int k = t; k < nh; k += SEQ_THREADS) {
+    const double v = a[k];
+    head[k] = v * v;
+  }
+  // two register sets in turn: the chunk after the current one is loading
+  // while the current one is summed (no copies between the sets)
+  double ra[SEQ_PER], rb[SEQ_PER];
+  if (n > 0) seq_load(a, n, nh + (long long)t * SEQ_PER, ra);
+  __syncthreads();
+  if (t == 0) {
+    double h = 0.0;
+    for (int k = 0; k < (int)nh; k++) h = h + head[k];
+    sh.acc = h;
+  }
+  __syncthreads();
+  double acc = sh.acc;
+  int parity = 0;
+  for (long long c0 = nh; c0 < n; c0 += 2 * SEQ_CHUNK) {
+    seq_load(a, n, c0 + SEQ_CHUNK + (long long)t * SEQ_PER, rb);
+    if (!seq_chunk(a, n, c0, ra, acc, dst, sh, parity)) return;
+    if (c0 + SEQ_CHUNK >= n) break;
+    seq_load(a, n, c0 + 2 * SEQ_CHUNK + (long long)t * SEQ_PER, ra);
+    if (!seq_chunk(a, n, c0 + SEQ_CHUNK, rb, acc, dst, sh, parity)) return;
   }
   if (t == 0) *dst = acc;
+}
+
+hipError_t launch_ordered_sumsq(const double* a, long long n, double* dst, hipStream_t st) {
+  hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(SEQ_THREADS), 0, st, a, n, dst, nullptr);
+  return hipGetLastError();
 }
 
 // Parallel-reduction mode: the norms as trees (blocks of 256, then the
@@ -477,7 +561,7 @@ static void launch_norm(const LsqrSys& S, LsqrWork& W, const double* a, long lon
     hipLaunchKernelGGL(tree_norm_part_kernel, dim3(g), dim3(256), 0, st, a, n, W.part, W.sc);
     hipLaunchKernelGGL(tree_norm_final_kernel, dim3(1), dim3(256), 0, st, W.part, g, dst, W.sc);
   } else {
-    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(256), 0, st, a, n, dst, W.sc);
+    hipLaunchKernelGGL(seqnorm_kernel, dim3(1), dim3(SEQ_THREADS), 0, st, a, n, dst, W.sc);
   }
 }
 

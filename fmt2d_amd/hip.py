@@ -134,6 +134,7 @@ def load() -> C.CDLL:
     lib.fmt2d_set_solver_launch.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.fmt2d_set_lsqr_mode.argtypes = [C.c_void_p, C.c_int]
+    lib.fmt2d_ordered_sumsq.argtypes = [C.c_void_p, _dp, C.c_longlong, _dp]
     lib.fmt2d_set_field_precision.argtypes = [C.c_void_p, C.c_int]
     lib.fmt2d_field_snapshot.argtypes = [C.c_void_p]
     lib.fmt2d_field_compare.argtypes = [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_longlong),
@@ -172,7 +173,7 @@ EXPORTS = ("fmt2d_ctx_create", "fmt2d_ctx_destroy", "fmt2d_last_error", "fmt2d_s
            "fmt2d_forward_sweep_hist", "fmt2d_forward_summary",
            "fmt2d_set_tuning", "fmt2d_set_heavy_sweeps", "fmt2d_set_launch_cap", "fmt2d_set_solver_launch",
            "fmt2d_set_tile",
-           "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_set_field_precision", "fmt2d_field_snapshot",
+           "fmt2d_autotune", "fmt2d_set_lsqr_mode", "fmt2d_ordered_sumsq", "fmt2d_set_field_precision", "fmt2d_field_snapshot",
            "fmt2d_field_compare", "fmt2d_set_ray_search", "fmt2d_ray_nearest_calls",
            "fmt2d_ray_nearest_check",
            "fmt2d_tf_region",
@@ -278,6 +279,14 @@ class Context:
         """0: LSQR reductions in the reference's order (bit-exact); 1: tree /
         wave reductions."""
         _check(self._lib.fmt2d_set_lsqr_mode(self._h, int(mode)))
+
+    def ordered_sumsq(self, x) -> float:
+        """Sum of x[i]**2 added front to back in FP64 on the device (the
+        ordered LSQR norm kernel)."""
+        v = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(1)
+        _check(self._lib.fmt2d_ordered_sumsq(self._h, _p(v), len(v), _p(out)))
+        return float(out[0])
 
     def set_launch_cap(self, max_launches: int = 0):
         _check(self._lib.fmt2d_set_launch_cap(self._h, int(max_launches)))

@@ -377,11 +377,16 @@ int fmt2d_system_get(fmt2d_ctx* ctx, double* gtg_diag, double* residual);
  * solves of the damping search): 0 (default) every norm and sparse product in the
  * reference's order, so the model update and iteration count are the
  * reference's bit for bit; 1 double-double tree-reduced norms and
- * wave-reduced products, 5-9x faster per iteration, with rounding of their
+ * wave-reduced products, about 2x faster per iteration, with rounding of their
  * own that the Krylov recurrences amplify: on the test systems the model
  * update moves by up to 7e-3 relative and the iteration count by up to 2.
  * No counterpart in the reference. */
 int fmt2d_set_lsqr_mode(fmt2d_ctx* ctx, int mode);
+/* *out = x[0]^2 + x[1]^2 + ... added front to back in FP64 (the ordered
+ * mode's norm kernel on a host vector of n entries), bit for bit the plain
+ * loop's sum.  (Diagnostics of this build, no counterpart in the
+ * reference.) */
+int fmt2d_ordered_sumsq(fmt2d_ctx* ctx, const double* x, long long n, double* out);
 /* One LSQR solve of [G; w L; wd I] dm = [r; -w L dm_sum; -wd dm_sum] with the
  * reference's column mask and stopping rule (test2 < 1e-4); smooth_type 2
  * solves [G; w R_h; w R_v; wd I] (run_lsqr, inversion2d.f90:6578-7167; the
