@@ -288,10 +288,19 @@ __global__ void __launch_bounds__(SEQ_THREADS) seqnorm_kernel(const double* __re
     const double v = a[k];
     head[k] = v * v;
   }
-  // two register sets in turn: the chunk after the current one is loading
-  // while the current one is summed (no copies between the sets)
-  double ra[SEQ_PER], rb[SEQ_PER];
-  if (n > 0) seq_load(a, n, nh + (long long)t * SEQ_PER, ra);
+  // the next chunk's loads are in flight while the current one is summed;
+  // they land in this thread's own LDS slots (strided by the block size: no
+  // bank conflicts) after the chunk (no values
+  // carried in registers from one iteration to the next, whose loads the
+  // compiler would wait for at the top of the loop)
+  __shared__ double stage[SEQ_CHUNK];
+  const int j0 = t * SEQ_PER;
+  if (nh < n) {
+    double r[SEQ_PER];
+    seq_load(a, n, nh + j0, r);
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) stage[i * SEQ_THREADS + t] = r[i];
+  }
   __syncthreads();
   if (t == 0) {
     double h = 0.0;
@@ -301,12 +310,14 @@ __global__ void __launch_bounds__(SEQ_THREADS) seqnorm_kernel(const double* __re
   __syncthreads();
   double acc = sh.acc;
   int parity = 0;
-  for (long long c0 = nh; c0 < n; c0 += 2 * SEQ_CHUNK) {
-    seq_load(a, n, c0 + SEQ_CHUNK + (long long)t * SEQ_PER, rb);
-    if (!seq_chunk(a, n, c0, ra, acc, dst, sh, parity)) return;
-    if (c0 + SEQ_CHUNK >= n) break;
-    seq_load(a, n, c0 + 2 * SEQ_CHUNK + (long long)t * SEQ_PER, ra);
-    if (!seq_chunk(a, n, c0 + SEQ_CHUNK, rb, acc, dst, sh, parity)) return;
+  for (long long c0 = nh; c0 < n; c0 += SEQ_CHUNK) {
+    double nxt[SEQ_PER], r[SEQ_PER];
+    seq_load(a, n, c0 + SEQ_CHUNK + j0, nxt);
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) r[i] = stage[i * SEQ_THREADS + t];
+    if (!seq_chunk(a, n, c0, r, acc, dst, sh, parity)) return;
+#pragma unroll
+    for (int i = 0; i < SEQ_PER; i++) stage[i * SEQ_THREADS + t] = nxt[i];
   }
   if (t == 0) *dst = acc;
 }
