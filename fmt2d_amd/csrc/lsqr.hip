@@ -147,8 +147,8 @@ __global__ void gauss_fwd_kernel(const LsqrSys S, const double* __restrict__ x, 
 // added as one integer, that term the ordinary way, and the chunk goes on
 // after it.  Past the head a sum of squares changes binade a few times per
 // doubling of the term count, so nearly every chunk is one pass.
-constexpr int SEQ_THREADS = 512;
-constexpr int SEQ_PER = 8;
+constexpr int SEQ_THREADS = 1024;
+constexpr int SEQ_PER = 4;
 constexpr int SEQ_CHUNK = SEQ_THREADS * SEQ_PER;
 constexpr int SEQ_WAVES = SEQ_THREADS / 64;
 constexpr int SEQ_HEAD = 512;
